@@ -1,0 +1,700 @@
+/*
+ * pd_oracle.c -- CPU FP64 restatement of PD's per-step forcefield hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the checker. Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product library never links or calls it.
+ *
+ * Written from the reference's behaviour (JonnyWideFoot/pd.arcus, read as
+ * text). Each function names the reference lines it restates. How each
+ * function is pinned (tests/test_oracle_*.py):
+ *   orc_bondorder        golden vectors from the compiled reference (band + sparse,
+ *                        amber99 peptides) and hand-built chains/rings
+ *   orc_nlist_build      golden vectors from the compiled reference (exact packed
+ *                        entries) and brute force on tiny inputs
+ *   orc_fixed_terms      golden V_j/terms123 from the compiled reference; closed
+ *                        form for an isolated atom and a diatomic
+ *   orc_born_radii       golden Born radii from the compiled reference; P4=0
+ *                        closed form; two-atom closed form
+ *   orc_gb_calc_forces   golden energies/forces from the compiled reference;
+ *                        two-atom closed forms; Newton's third law; finite
+ *                        differences with the chain-rule constant set to halfeconv
+ *   orc_nb_calc_forces   golden FF_NonBonded energies/forces from the compiled
+ *                        reference; finite differences (energy switch)
+ *   orc_gb_calc_energies golden calcEnergies totals from the compiled reference
+ *   orc_minimise_sd      golden per-step energies of the compiled reference's
+ *                        Minimisation (SteepestDescent)
+ *   orc_md_vv            energy conservation of a two-atom system (textbook VV is
+ *                        a deliberate choice; PD's own VV update differs, see
+ *                        protocols/md.cpp:788-838)
+ */
+#include "pd_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- constants: forcefields/forcefield.cpp:18-38, maths/maths.cpp:6-8 ---- */
+static const double kKcal2J = 4184.0;
+static const double kNa = 6.0221418E23;
+static const double kAngstrom = 1E-10;
+static const double kECharge = 1.60217645E-19;
+static const double kE0 = 8.854187817E-12;
+static const double kPI = 3.14159265358979323846;
+static const double kKB = 1.38065E-23;
+static const double kAmu = 1.66053878E-27;
+
+static double econv_joule(void) {
+    const double four_pi_e0 = (kPI * 4.0) * kE0;
+    return (kECharge * kECharge) / (four_pi_e0 * kAngstrom);
+}
+static double econv_kcal(void) {
+    const double four_pi_e0 = (kPI * 4.0) * kE0;
+    return ((1.0 / kKcal2J) * kNa * (kECharge * kECharge)) / (four_pi_e0 * kAngstrom);
+}
+
+double orc_const(int which) {
+    switch (which) {
+    case ORC_KCAL2J: return kKcal2J;
+    case ORC_NA: return kNa;
+    case ORC_ANGSTROM: return kAngstrom;
+    case ORC_ECONV_J: return econv_joule();
+    case ORC_HALFECONV_J: return econv_joule() * 0.5;
+    case ORC_ECONV: return econv_kcal();
+    case ORC_HALFECONV: return econv_kcal() * 0.5;
+    case ORC_KB: return kKB;
+    case ORC_AMU: return kAmu;
+    }
+    return NAN;
+}
+
+static inline double sqr(double x) { return x * x; }
+
+/* Defaults of a freshly constructed FF_GeneralizedBorn_Still
+ * (forcefields/gbff.cpp:17-37 over forcefields/nonbonded.cpp:968-1004).
+ * 1-4 scalings are 1.0 here; PD's AutoScaling14 copies them from the
+ * parameter file at setup (forcefields/nonbonded.cpp:1014-1018). */
+void orc_default_params(orc_params *p) {
+    memset(p, 0, sizeof(*p));
+    p->cutoff = 15.0;
+    p->inner_cutoff = 12.0;
+    p->vdw_cutoff = 15.0;
+    p->vdw_inner_cutoff = 12.0;
+    p->vdw14_scaling = 1.0;
+    p->elec14_scaling = 1.0;
+    p->dielectric_solute = 1.0;
+    p->dielectric_solvent = 80.0;
+    p->still_cutoff = 10.0;
+    p->exp_approx_threshold = -4.0;
+    p->dielectric_offset = -0.090;
+    p->born_radius_offset = 0.0;
+    p->P1 = 0.073; p->P2 = 0.921; p->P3 = 6.211; p->P4 = 15.236; p->P5 = 1.254;
+    p->padding = 0.5;
+    p->elec_mode = 3; /* GB constructor: EnergySwitch=true, ForceSwitch=false */
+    p->vdw_mode = 2;
+}
+
+/* ---------------- A2: workspace/bondorder.cpp:77-195 ---------------- */
+int orc_bondorder(int n, const int32_t *resid,
+                  int n12, const int32_t *a12, const int32_t *b12,
+                  int n13, const int32_t *a13, const int32_t *b13,
+                  int n14, const int32_t *a14, const int32_t *b14,
+                  int32_t *mid_out, int8_t *band,
+                  int32_t *sp_i, int32_t *sp_j, int32_t *sp_o) {
+    /* calcMaxIndexDelta: widest index gap among 1-2/1-3/1-4 partners whose
+     * residues differ by at most one, plus one (bondorder.cpp:77-127). */
+    int mid = 0;
+    const int32_t *A[3] = {a12, a13, a14}, *B[3] = {b12, b13, b14};
+    const int cnt[3] = {n12, n13, n14};
+    for (int l = 0; l < 3; ++l)
+        for (int k = 0; k < cnt[l]; ++k) {
+            int a = A[l][k], b = B[l][k];
+            if (abs(resid[a] - resid[b]) > 1) continue;
+            int d = abs(a - b);
+            if (d > mid) mid = d;
+        }
+    mid += 1;
+    *mid_out = mid;
+    if (!band) return 0;
+    /* reinit: band initialised to MaxBondOrder 7, then 1-4, 1-3, 1-2 written
+     * in that order so the lowest order wins (bondorder.cpp:170-195). Pairs
+     * with index gap > mid go to the sparse list, where the reference's
+     * later patches (correctOffDiagonalBondOrders) also let the lowest order
+     * win; here each sparse pair is stored once with that final order. */
+    if (n > 1) memset(band, 7, (size_t)mid * (size_t)(n - 1));
+    int nsp = 0;
+    for (int l = 2; l >= 0; --l) {
+        const int order = l + 1;
+        for (int k = 0; k < cnt[l]; ++k) {
+            int i = A[l][k], j = B[l][k];
+            if (i == j) continue;
+            if (i > j) { int t = i; i = j; j = t; }
+            int d = j - i;
+            if (d > mid) {
+                int found = -1;
+                for (int q = 0; q < nsp; ++q)
+                    if (sp_i[q] == i && sp_j[q] == j) { found = q; break; }
+                if (found < 0) { sp_i[nsp] = i; sp_j[nsp] = j; sp_o[nsp] = order; ++nsp; }
+                else sp_o[found] = order;
+            } else {
+                band[(size_t)i * mid + (d - 1)] = (int8_t)order;
+            }
+        }
+    }
+    return nsp;
+}
+
+/* Bond order as recorded by calcNewList_InfiniteSpace + the off-diagonal
+ * patch (workspace/neighbourlist.cpp:281-289, 162-203): band lookup only for
+ * |i-j| < mid, the sparse list otherwise, 7 when absent. */
+int orc_pair_bondorder(const orc_system *s, int i, int j) {
+    int d = abs(j - i);
+    if (d < s->mid) {
+        if (d == 0) return 0;
+        int lo = i < j ? i : j;
+        return s->band[(size_t)lo * s->mid + (d - 1)] & 7;
+    }
+    int lo = i < j ? i : j, hi = i < j ? j : i;
+    int bo = 7;
+    for (int q = 0; q < s->nsparse; ++q)
+        if (s->sp_i[q] == lo && s->sp_j[q] == hi) bo = s->sp_o[q] & 7;
+    return bo;
+}
+
+/* ---------------- A1: workspace/neighbourlist.cpp:205-316 ---------------- */
+int64_t orc_nlist_build(const orc_system *s, const double *xyz, double list_cutoff,
+                        double padding, int shadow, orc_nlist *nl) {
+    const int n = s->n;
+    const double lim = sqr(list_cutoff + padding);
+    int64_t cnt = 0;
+    for (int i = 0; i < n; ++i) {
+        nl->row_start[i] = (int32_t)cnt;
+        const int limit = shadow ? n : i;
+        for (int j = 0; j < limit; ++j) {
+            double dx = xyz[3 * j] - xyz[3 * i];
+            double dy = xyz[3 * j + 1] - xyz[3 * i + 1];
+            double dz = xyz[3 * j + 2] - xyz[3 * i + 2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > lim) continue;
+            if (j == i) continue;
+            int bo = orc_pair_bondorder(s, i, j);
+            if (cnt < nl->cap)
+                nl->entries[cnt] = ((uint32_t)j & 0x00FFFFFFu) | ((uint32_t)(bo & 7) << 24);
+            ++cnt;
+        }
+    }
+    nl->row_start[n] = (int32_t)cnt;
+    nl->count = cnt;
+    return cnt > nl->cap ? cnt : 0;
+}
+
+static inline int nl_j(uint32_t e) { return (int)(e & 0x00FFFFFFu); }
+static inline int nl_bo(uint32_t e) { return (int)((e >> 24) & 7u); }
+
+/* ---------------- A3: forcefields/gbff.cpp:307-381 ---------------- */
+void orc_fixed_terms(int n, const double *gb_radius,
+                     int n12, const int32_t *a12, const int32_t *b12, const double *d12,
+                     int n13, const int32_t *a13, const int32_t *b13, const double *d13,
+                     const orc_params *p, double *vj, double *terms123) {
+    /* hydrogen rule: a type radius <= 0.01 is replaced by 1.15 (gbff.cpp:326-327) */
+#define ORC_RADJ(r) ((r) <= 0.01 ? 1.15 : (r))
+    const double four_pi = 4.0 * kPI;
+    for (int j = 0; j < n; ++j) {
+        double Rj = ORC_RADJ(gb_radius[j]);
+        vj[j] = four_pi * Rj * Rj * Rj / 3;
+    }
+    /* subtract half-overlap caps of each 1-2 partner, both directions */
+    for (int k = 0; k < n12; ++k) {
+        for (int side = 0; side < 2; ++side) {
+            int j = side ? b12[k] : a12[k];
+            int o = side ? a12[k] : b12[k];
+            double Rj = ORC_RADJ(gb_radius[j]);
+            double Rk = ORC_RADJ(gb_radius[o]);
+            double rjk = 1.01 * d12[k];
+            double h = Rj * (1.0 + (sqr(Rk) - sqr(Rj) - sqr(rjk)) / (2.0 * Rj * rjk));
+            vj[j] -= kPI * sqr(h) * (3 * Rj - h) / 3;
+        }
+    }
+    for (int j = 0; j < n; ++j)
+        if (vj[j] < 0) vj[j] = 0;
+    for (int i = 0; i < n; ++i) {
+        double Ri = ORC_RADJ(gb_radius[i]);
+        terms123[i] = -166.0 / (Ri + p->dielectric_offset + p->P1);
+    }
+    double *t2 = (double *)calloc((size_t)n, sizeof(double));
+    double *t3 = (double *)calloc((size_t)n, sizeof(double));
+    for (int k = 0; k < n12; ++k) {
+        double r4 = sqr(d12[k]) * sqr(d12[k]);
+        t2[a12[k]] += p->P2 * vj[b12[k]] / r4;
+        t2[b12[k]] += p->P2 * vj[a12[k]] / r4;
+    }
+    for (int k = 0; k < n13; ++k) {
+        double r4 = sqr(d13[k]) * sqr(d13[k]);
+        t3[a13[k]] += p->P3 * vj[b13[k]] / r4;
+        t3[b13[k]] += p->P3 * vj[a13[k]] / r4;
+    }
+    for (int i = 0; i < n; ++i) terms123[i] = terms123[i] + t2[i] + t3[i];
+    free(t2);
+    free(t3);
+#undef ORC_RADJ
+}
+
+/* ---------------- A4: forcefields/gbff.cpp:415-466 ---------------- */
+void orc_born_radii(const orc_system *s, const double *xyz, const orc_nlist *nl,
+                    const orc_params *p, double *born) {
+    const double halfeconv = econv_kcal() * 0.5;
+    const double invP5 = 1 / p->P5;
+    const double still2 = sqr(p->still_cutoff);
+    for (int i = 0; i < s->n; ++i) {
+        const double Ri = s->gb_radius[i];
+        double Gpol = s->terms123[i];
+        for (int k = nl->row_start[i]; k < nl->row_start[i + 1]; ++k) {
+            const uint32_t e = nl->entries[k];
+            const int j = nl_j(e);
+            double g = xyz[3 * i] - xyz[3 * j];
+            double h = xyz[3 * i + 1] - xyz[3 * j + 1];
+            double q = xyz[3 * i + 2] - xyz[3 * j + 2];
+            double d2 = g * g + h * h + q * q;
+            if (d2 > still2) continue;
+            if (nl_bo(e) <= 2) continue; /* 1-2 and 1-3 pairs excluded */
+            const double Rj = s->gb_radius[j];
+            double rdivR = d2 / sqr(Ri + Rj);
+            double CCF = rdivR > invP5 ? 1.0 : sqr(0.5 - 0.5 * cos(rdivR * p->P5 * kPI));
+            Gpol += p->P4 * s->vj[j] * CCF / sqr(d2);
+        }
+        born[i] = -halfeconv / Gpol + p->born_radius_offset;
+    }
+}
+
+/* ---------------- A5 + A6: forcefields/gbff.cpp:1449-1818 ---------------- */
+void orc_gb_calc_forces(const orc_system *s, const double *xyz, const orc_nlist *nl,
+                        const orc_params *p, double *force, orc_energies *e,
+                        double *born_out, double *deda_out) {
+    const int n = s->n;
+    double *a = (double *)malloc(sizeof(double) * (size_t)n);
+    double *deda = (double *)calloc((size_t)n, sizeof(double));
+    double *dedi = (double *)calloc((size_t)n * 3, sizeof(double));
+    orc_born_radii(s, xyz, nl, p, a);
+
+    const double econvJ = econv_joule();
+    const double premul = -(econvJ * 0.5) * (1 / p->dielectric_solute - 1 / p->dielectric_solvent);
+    const double sqrcutoff = sqr(p->cutoff);
+    const double Swidth = p->cutoff - p->inner_cutoff;
+    const double invSwidth = 1.0 / Swidth;
+    const double vdwSwidth = p->vdw_cutoff - p->vdw_inner_cutoff;
+    const double vdwinvSwidth = 1.0 / vdwSwidth;
+    const double invdiel = 1 / p->dielectric_solute;
+    const double invsolv = 1 / p->dielectric_solvent;
+
+    double epot_vdw = 0, epot_elec = 0, epot = 0;
+    double epot_pol = 0, epot_pol_self = 0, epot_pol_cross = 0;
+
+    for (int i = 0; i < n; ++i) {
+        const double ai = a[i];
+        const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+        const double Ri = s->radius[i], ei = s->epsilon[i], qi = s->charge[i];
+        double dedai = 0;
+        for (int k = nl->row_start[i]; k < nl->row_start[i + 1]; ++k) {
+            const uint32_t ent = nl->entries[k];
+            const int j = nl_j(ent);
+            if (j > i) break; /* shadow entries are not part of the pair pass */
+            const int bo = nl_bo(ent);
+            const double aj = a[j];
+            const double qiqj = qi * s->charge[j];
+            const double dx = xi - xyz[3 * j];
+            const double dy = yi - xyz[3 * j + 1];
+            const double dz = zi - xyz[3 * j + 2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > sqrcutoff) continue;
+            const double r = sqrt(r2);
+            const double invr = 1 / r;
+            const double aiaj = ai * aj;
+            const double u = -r2 * 0.25 / aiaj;
+            const double s14e = (bo == 3) ? p->elec14_scaling : 1.0;
+            double elec_pot = 0, elec_force = 0;
+            if (u > p->exp_approx_threshold) {
+                const double t = exp(u);
+                const double L = 1 / (r2 + t * aiaj);
+                double epol = premul * qiqj * sqrt(L);
+                double dedr = -epol * (r - 0.25 * r * t) * L;
+                double dedalpha = -epol * t * (1.0 - u) * L;
+                if (bo >= 3) {
+                    elec_pot = econvJ * invdiel * s14e * qiqj * invr;
+                    elec_force = -elec_pot * invr;
+                }
+                if (r >= p->inner_cutoff) {
+                    double S = 1.0 - sqr(invSwidth * (r - p->inner_cutoff));
+                    double dSdd = -4.0 * sqr(invSwidth) * (r - p->inner_cutoff) * S;
+                    S = sqr(S);
+                    elec_force = S * elec_force + elec_pot * dSdd;
+                    elec_pot *= S;
+                    dedr = S * dedr + dSdd * epol;
+                    dedalpha *= S;
+                    epol *= S;
+                }
+                dedr *= 2 * invr;
+                epot_pol += epol;
+                epot_pol += epol;
+                epot_pol_cross += epol;
+                epot_pol_cross += epol;
+                dedi[3 * i] += dedr * dx; dedi[3 * i + 1] += dedr * dy; dedi[3 * i + 2] += dedr * dz;
+                dedi[3 * j] -= dedr * dx; dedi[3 * j + 1] -= dedr * dy; dedi[3 * j + 2] -= dedr * dz;
+                dedai += dedalpha * aj;
+                deda[j] += dedalpha * ai;
+            } else if (bo >= 3) {
+                /* far pair: GB dropped, solvent-dielectric Coulomb instead */
+                elec_pot = econvJ * invsolv * s14e * qiqj * invr;
+                elec_force = -elec_pot * invr;
+                if (r >= p->inner_cutoff) {
+                    double S = 1.0 - sqr(invSwidth * (r - p->inner_cutoff));
+                    double dSdd = -4.0 * sqr(invSwidth) * (r - p->inner_cutoff) * S;
+                    S = sqr(S);
+                    elec_force = S * elec_force + elec_pot * dSdd;
+                    elec_pot *= S;
+                }
+            }
+            if (bo >= 3) {
+                const double s14v = (bo == 3) ? p->vdw14_scaling : 1.0;
+                double vdw_force = 0;
+                if (r < p->vdw_cutoff) {
+                    const double rr = sqr(invr * (Ri + s->radius[j]));
+                    const double eps = s14v * ei * s->epsilon[j];
+                    const double B = rr * rr * rr;
+                    const double A = sqr(B);
+                    double vdw_pot = (2.0 * eps) * (0.5 * A - B);
+                    vdw_force = -(12.0 * eps * invr) * (A - B);
+                    if (r > p->vdw_inner_cutoff) {
+                        double vS = 1.0 - sqr(vdwinvSwidth * (r - p->vdw_inner_cutoff));
+                        double vdSdd = -4.0 * sqr(vdwinvSwidth) * (r - p->vdw_inner_cutoff) * vS;
+                        vS = sqr(vS);
+                        vdw_force = vS * vdw_force + vdw_pot * vdSdd;
+                        vdw_pot *= vS;
+                    }
+                    epot += vdw_pot;
+                    epot_vdw += vdw_pot;
+                }
+                epot += elec_pot;
+                epot_elec += elec_pot;
+                const double fm = (vdw_force + elec_force) * invr;
+                dedi[3 * i] += fm * dx; dedi[3 * i + 1] += fm * dy; dedi[3 * i + 2] += fm * dz;
+                dedi[3 * j] -= fm * dx; dedi[3 * j + 1] -= fm * dy; dedi[3 * j + 2] -= fm * dz;
+            }
+        }
+        deda[i] += dedai;
+        const double epol = premul * sqr(qi) / ai; /* self term */
+        epot_pol += epol;
+        epot_pol_self += epol;
+        deda[i] += -epol / ai;
+    }
+
+    /* chain rule through the Born radii over the full list (gbff.cpp:1751-1808).
+     * The literal 166.0 here (vs halfeconv in the radii) is PD's. */
+    const double invP5 = 1 / p->P5;
+    const double still2 = sqr(p->still_cutoff);
+    for (int i = 0; i < n; ++i) {
+        const double Ri = s->gb_radius[i];
+        const double gpi = -sqr(a[i]) / -166.0;
+        for (int k = nl->row_start[i]; k < nl->row_start[i + 1]; ++k) {
+            const uint32_t ent = nl->entries[k];
+            const int j = nl_j(ent);
+            const double dx = xyz[3 * j] - xyz[3 * i];
+            const double dy = xyz[3 * j + 1] - xyz[3 * i + 1];
+            const double dz = xyz[3 * j + 2] - xyz[3 * i + 2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 > still2) continue;
+            if (nl_bo(ent) <= 2) continue;
+            const double Rj = s->gb_radius[j];
+            const double r6 = r2 * r2 * r2;
+            const double rdivR = r2 / sqr(Ri + Rj);
+            double CCF, dCCF;
+            if (rdivR > invP5) {
+                CCF = 1.0;
+                dCCF = 0.0;
+            } else {
+                const double th = rdivR * p->P5 * kPI;
+                const double sq = 0.5 - 0.5 * cos(th);
+                CCF = sqr(sq);
+                dCCF = 2.0 * sq * sin(th) * th;
+            }
+            const double dedr = gpi * deda[i] * p->P4 * s->vj[j] * (4.0 * CCF - dCCF) / r6;
+            dedi[3 * i] += dedr * dx; dedi[3 * i + 1] += dedr * dy; dedi[3 * i + 2] += dedr * dz;
+            dedi[3 * j] -= dedr * dx; dedi[3 * j + 1] -= dedr * dy; dedi[3 * j + 2] -= dedr * dz;
+        }
+    }
+
+    if (force) {
+        const double conv = -1.0 / kAngstrom;
+        for (int i = 0; i < 3 * n; ++i) force[i] += dedi[i] * conv;
+    }
+    if (e) {
+        /* the reference overwrites epot_vdw/epot_elec, adds the rest */
+        e->epot_vdw = epot_vdw;
+        e->epot_elec = epot_elec;
+        e->epot += epot + epot_pol;
+        e->epot_pol += epot_pol;
+        e->epot_pol_cross += epot_pol_cross;
+        e->epot_pol_self += epot_pol_self;
+    }
+    if (born_out) memcpy(born_out, a, sizeof(double) * (size_t)n);
+    if (deda_out) memcpy(deda_out, deda, sizeof(double) * (size_t)n);
+    free(a);
+    free(deda);
+    free(dedi);
+}
+
+/* ---------------- A8: forcefields/nonbonded.cpp:542-893 ---------------- */
+void orc_nb_calc_forces(const orc_system *s, const double *xyz, const orc_nlist *nl,
+                        const orc_params *p, double *force, orc_energies *e) {
+    const int n = s->n;
+    const double econvJ = econv_joule();
+    const double sqrcutoff = sqr(p->cutoff);
+    const double sqrinner = sqr(p->inner_cutoff);
+    const double invdiel = 1.0 / p->dielectric_solute;
+    const double invSw = 1.0 / (p->cutoff - p->inner_cutoff);
+    const double vinvSw = 1.0 / (p->vdw_cutoff - p->vdw_inner_cutoff);
+    /* Steinbach-Brooks force-switch constants (nonbonded.cpp:601-616) */
+    const double rc = p->cutoff, ri = p->inner_cutoff;
+    const double rc2 = sqr(rc), ri2 = sqr(ri);
+    const double sA = 1.0 / ((rc2 - ri2) * (rc2 - ri2) * (rc2 - ri2));
+    const double sB = -(rc2 * rc2 * rc2 - 3.0 * rc2 * rc2 * ri2);
+    const double sC = 6.0 * rc2 * ri2;
+    const double sD = -(rc2 + ri2);
+    const double sE = 2.0 / 5.0;
+    const double fsw_in = sA * (sB * (1.0 / ri) + sC * ri + sD * ri * ri * ri + sE * ri * ri * ri * ri2);
+    const double fsw_cut = sA * (sB * (1.0 / rc) + sC * rc + sD * rc * rc * rc + sE * rc * rc * rc * rc2);
+    const double eshift = (ri < rc) ? 1 / ri + (fsw_in - fsw_cut) : 0;
+    const double tv[8] = {0, 0, 0, p->vdw14_scaling, 1, 1, 1, 1};
+    const double te[8] = {0, 0, 0, p->elec14_scaling, 1, 1, 1, 1};
+
+    double ev = 0, ee = 0, ep = 0, vir = 0;
+    for (int i = 0; i < n; ++i) {
+        const double Ri = s->radius[i], ei = s->epsilon[i], qi = s->charge[i];
+        double fix = 0, fiy = 0, fiz = 0;
+        for (int k = nl->row_start[i]; k < nl->row_start[i + 1]; ++k) {
+            const uint32_t ent = nl->entries[k];
+            const int j = nl_j(ent);
+            if (j >= i) break;
+            const int bo = nl_bo(ent);
+            double fx = xyz[3 * j] - xyz[3 * i];
+            double fy = xyz[3 * j + 1] - xyz[3 * i + 1];
+            double fz = xyz[3 * j + 2] - xyz[3 * i + 2];
+            const double d2 = fx * fx + fy * fy + fz * fz;
+            if (d2 > sqrcutoff) continue;
+            const double r = sqrt(d2);
+            const double inv = 1.0 / r;
+            double vdw_pot = 0, vdw_force = 0;
+            if (p->vdw_mode > 0 && r < p->vdw_cutoff) {
+                const double rij = Ri + s->radius[j];
+                const double eps = tv[bo] * ei * s->epsilon[j];
+                double B = sqr(rij * inv);
+                B *= B * B;
+                const double A = sqr(B);
+                vdw_pot = (2.0 * eps) * (0.5 * A - B);
+                vdw_force = -(12.0E10 * eps * inv) * (A - B);
+                if (p->vdw_mode == 2 && r > p->vdw_inner_cutoff) {
+                    double vS = 1.0 - sqr(vinvSw * (r - p->vdw_inner_cutoff));
+                    double vd = -4.0E10 * sqr(vinvSw) * (r - p->vdw_inner_cutoff) * vS;
+                    vS = sqr(vS);
+                    vdw_force = vS * vdw_force + vdw_pot * vd;
+                    vdw_pot *= vS;
+                }
+            }
+            double elec_pot = 0, elec_force = 0;
+            const double qiqj = qi * s->charge[j];
+            if (p->elec_mode == 1 || p->elec_mode == 3) {
+                elec_pot = econvJ * invdiel * te[bo] * qiqj * inv;
+                elec_force = -elec_pot * inv * 1E10;
+                if (p->elec_mode == 3 && r > p->inner_cutoff) {
+                    double S = 1.0 - sqr(invSw * (r - p->inner_cutoff));
+                    double dS = -4.0E10 * sqr(invSw) * (r - p->inner_cutoff) * S;
+                    S = sqr(S);
+                    elec_force = S * elec_force + elec_pot * dS;
+                    elec_pot *= S;
+                }
+            } else if (p->elec_mode == 4) {
+                elec_pot = econvJ * invdiel * te[bo] * qiqj;
+                if (r > p->inner_cutoff) {
+                    elec_force = -1E10 * elec_pot * sA * sqr(inv) * sqr(sqrcutoff - d2) *
+                                 (sqrcutoff - 3.0 * sqrinner + 2.0 * d2);
+                    elec_pot *= -(sA * r * (sB * sqr(inv) + sC + sD * d2 + sE * sqr(d2)) - fsw_cut);
+                } else {
+                    elec_force = -1E10 * elec_pot * sqr(inv);
+                    elec_pot *= (inv - eshift);
+                }
+            }
+            ep += vdw_pot + elec_pot;
+            ev += vdw_pot;
+            ee += elec_pot;
+            const double fm = vdw_force + elec_force;
+            vir += r * fm * kAngstrom;
+            const double sc = inv * fm;
+            fx *= sc; fy *= sc; fz *= sc;
+            fix += fx; fiy += fy; fiz += fz;
+            if (force) { force[3 * j] -= fx; force[3 * j + 1] -= fy; force[3 * j + 2] -= fz; }
+        }
+        if (force) { force[3 * i] += fix; force[3 * i + 1] += fiy; force[3 * i + 2] += fiz; }
+    }
+    if (e) {
+        e->epot_vdw = ev;
+        e->epot_elec = ee;
+        e->epot += ep;
+        e->virial += vir;
+    }
+}
+
+/* ---------------- A7 energy path: gbff.cpp:161-166, 520-579 ---------------- */
+void orc_gb_calc_energies(const orc_system *s, const double *xyz, const orc_nlist *nl,
+                          const orc_params *p, double *force, orc_energies *e,
+                          double *born_out) {
+    const int n = s->n;
+    double *a = (double *)malloc(sizeof(double) * (size_t)n);
+    orc_born_radii(s, xyz, nl, p, a);
+    orc_nb_calc_forces(s, xyz, nl, p, force, e);
+    const double premul = -(econv_joule() * 0.5) * (1 / p->dielectric_solute - 1 / p->dielectric_solvent);
+    const double invSw = 1 / (p->cutoff - p->inner_cutoff);
+    double pol = 0, self = 0, cross = 0;
+    for (int i = 0; i < n; ++i) {
+        const double qi = s->charge[i], ai = a[i];
+        pol -= sqr(qi) / ai;
+        self -= sqr(qi) / ai;
+        for (int k = nl->row_start[i]; k < nl->row_start[i + 1]; ++k) {
+            const int j = nl_j(nl->entries[k]);
+            if (i > j) continue; /* only entries with j > i (shadow half) */
+            const double g = xyz[3 * i] - xyz[3 * j];
+            const double h = xyz[3 * i + 1] - xyz[3 * j + 1];
+            const double q = xyz[3 * i + 2] - xyz[3 * j + 2];
+            const double r = sqrt(g * g + h * h + q * q);
+            if (r > p->cutoff) continue;
+            double S;
+            if (r < p->inner_cutoff) S = 1.0;
+            else { S = 1.0 - sqr(invSw * (r - p->inner_cutoff)); S = sqr(S); }
+            const double r2 = sqr(r);
+            const double aiaj = ai * a[j];
+            const double epol = 2.0 * S * qi * s->charge[j] / sqrt(r2 + aiaj * exp(-r2 * 0.25 / aiaj));
+            pol -= epol;
+            cross -= epol;
+        }
+    }
+    pol *= -premul;
+    cross *= -premul;
+    self *= -premul;
+    if (e) {
+        e->epot += pol;
+        e->epot_pol += pol;
+        e->epot_pol_cross += cross;
+        e->epot_pol_self += self;
+    }
+    if (born_out) memcpy(born_out, a, sizeof(double) * (size_t)n);
+    free(a);
+}
+
+/* ---------------- A11: protocols/minimise.cpp:99-235 ---------------- */
+static int build_list(const orc_system *s, const double *xyz, const orc_params *p, orc_nlist *nl) {
+    const double lc = p->cutoff > p->vdw_cutoff ? p->cutoff : p->vdw_cutoff;
+    for (;;) {
+        int64_t need = orc_nlist_build(s, xyz, lc, p->padding, 1, nl);
+        if (need == 0) return 0;
+        free(nl->entries);
+        nl->cap = need + need / 2;
+        nl->entries = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nl->cap);
+        if (!nl->entries) return -1;
+    }
+}
+
+static void eval_forces(const orc_system *s, const double *xyz, const orc_nlist *nl,
+                        const orc_params *p, double *f, orc_energies *e) {
+    memset(e, 0, sizeof(*e));
+    memset(f, 0, sizeof(double) * 3 * (size_t)s->n);
+    orc_gb_calc_forces(s, xyz, nl, p, f, e, NULL, NULL);
+}
+
+int orc_minimise_sd(const orc_system *s, double *xyz, const orc_params *p,
+                    int steps, double step_size, int update_nlist,
+                    double *trace_e, orc_energies *e_final, double *force_final) {
+    const int n = s->n;
+    orc_nlist nl;
+    nl.row_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    nl.cap = (int64_t)n * 64 + 1024;
+    nl.entries = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nl.cap);
+    double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    double *op = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    orc_energies e;
+    int rc = 0;
+    build_list(s, xyz, p, &nl);
+    memcpy(op, xyz, sizeof(double) * 3 * (size_t)n); /* wspace.old = wspace.cur */
+    double mult = 1.0, old_e = 0;
+    int step;
+    for (step = 0; step < steps; ++step) {
+        if (update_nlist > 0 && step % update_nlist == 0) build_list(s, xyz, p, &nl);
+        eval_forces(s, xyz, &nl, p, f, &e);
+        if (trace_e) trace_e[step] = e.epot;
+        if (!isfinite(e.epot)) { rc = -1; goto done; }
+        if (step != 0) {
+            if (e.epot <= old_e) {
+                mult *= 1.2;
+                memcpy(op, xyz, sizeof(double) * 3 * (size_t)n);
+                old_e = e.epot;
+            } else {
+                mult *= 0.5 / 1.2;
+                memcpy(xyz, op, sizeof(double) * 3 * (size_t)n);
+                old_e += 1; /* PD adds 1 J/molecule: the next step is always accepted */
+                continue;
+            }
+        } else {
+            memcpy(op, xyz, sizeof(double) * 3 * (size_t)n);
+            old_e = e.epot;
+        }
+        for (int k = 0; k < 3 * n; ++k) xyz[k] += f[k] * (step_size * mult);
+    }
+    rc = step;
+    memcpy(xyz, op, sizeof(double) * 3 * (size_t)n); /* cur = old */
+    eval_forces(s, xyz, &nl, p, f, &e);
+    if (e_final) *e_final = e;
+    if (force_final) memcpy(force_final, f, sizeof(double) * 3 * (size_t)n);
+done:
+    free(nl.row_start);
+    free(nl.entries);
+    free(f);
+    free(op);
+    return rc;
+}
+
+/* Textbook velocity Verlet; see header. */
+int orc_md_vv(const orc_system *s, double *xyz, double *vel, const orc_params *p,
+              int steps, double dt, int update_nlist, double *trace_epot, double *trace_ekin) {
+    const int n = s->n;
+    orc_nlist nl;
+    nl.row_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    nl.cap = (int64_t)n * 64 + 1024;
+    nl.entries = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nl.cap);
+    double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    orc_energies e;
+    int rc = steps;
+    build_list(s, xyz, p, &nl);
+    eval_forces(s, xyz, &nl, p, f, &e);
+    for (int step = 0; step < steps; ++step) {
+        for (int i = 0; i < n; ++i) {
+            const double h = 0.5 * dt / s->mass[i];
+            for (int c = 0; c < 3; ++c) {
+                vel[3 * i + c] += f[3 * i + c] * h;
+                xyz[3 * i + c] += vel[3 * i + c] * dt / kAngstrom;
+            }
+        }
+        if (update_nlist > 0 && (step + 1) % update_nlist == 0) build_list(s, xyz, p, &nl);
+        eval_forces(s, xyz, &nl, p, f, &e);
+        double ekin = 0;
+        for (int i = 0; i < n; ++i) {
+            const double h = 0.5 * dt / s->mass[i];
+            for (int c = 0; c < 3; ++c) {
+                vel[3 * i + c] += f[3 * i + c] * h;
+                ekin += 0.5 * s->mass[i] * sqr(vel[3 * i + c]);
+            }
+        }
+        if (trace_epot) trace_epot[step] = e.epot;
+        if (trace_ekin) trace_ekin[step] = ekin;
+        if (!isfinite(e.epot)) { rc = -1; break; }
+    }
+    free(nl.row_start);
+    free(nl.entries);
+    free(f);
+    return rc;
+}

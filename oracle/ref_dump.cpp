@@ -1,0 +1,306 @@
+// ref_dump.cpp -- drives the compiled reference (oracle/_ref/libmmlib.a) to
+// write golden vectors for the hot path. Test infrastructure: it runs only in
+// the build container, where /root/reference and its parameter files exist.
+//
+// Usage: ref_dump <param.ff> <outdir>
+// For every case it builds a peptide with PD's NewProtein (extended builder,
+// system/genpolymer.cpp:256-270), sets up FF_Bonded (only to obtain the
+// equilibrium 1-2/1-3 distances GB's fixed terms read) and evaluates
+// FF_GeneralizedBorn_Still alone, writing inputs and outputs as a flat
+// record file <outdir>/<case>.bin read by tests/golden/make_golden.py.
+#include "global.h"
+#include "system/system.h"
+#include "system/genpolymer.h"
+#include "workspace/workspace.h"
+#include "workspace/neighbourlist.h"
+#include "workspace/bondorder.h"
+#include "forcefields/ffparam.h"
+#include "forcefields/forcefield.h"
+#include "forcefields/ffbonded.h"
+#include "forcefields/nonbonded.h"
+#include "forcefields/gbff.h"
+#include "protocols/minimise.h"
+
+#include <stdint.h>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+using namespace Physics;
+
+namespace {
+
+// --- flat record writer: [u32 namelen][name][u8 dtype][u64 count][data] ---
+struct Rec {
+    FILE *f;
+    explicit Rec(const std::string &path) { f = fopen(path.c_str(), "wb"); }
+    ~Rec() { if (f) fclose(f); }
+    void put(const std::string &name, char dt, const void *data, uint64_t count, size_t esz) {
+        uint32_t nl = (uint32_t)name.size();
+        fwrite(&nl, 4, 1, f);
+        fwrite(name.data(), 1, nl, f);
+        fwrite(&dt, 1, 1, f);
+        fwrite(&count, 8, 1, f);
+        if (count) fwrite(data, esz, count, f);
+    }
+    void d(const std::string &n, const std::vector<double> &v) { put(n, 'd', v.empty() ? 0 : &v[0], v.size(), 8); }
+    void i(const std::string &n, const std::vector<int32_t> &v) { put(n, 'i', v.empty() ? 0 : &v[0], v.size(), 4); }
+    void u(const std::string &n, const std::vector<uint32_t> &v) { put(n, 'u', v.empty() ? 0 : &v[0], v.size(), 4); }
+    void b(const std::string &n, const std::vector<int8_t> &v) { put(n, 'b', v.empty() ? 0 : &v[0], v.size(), 1); }
+    void s(const std::string &n, double x) { std::vector<double> v(1, x); d(n, v); }
+};
+
+// Gives the driver read access to GB internals it needs to dump.
+class GBProbe : public FF_GeneralizedBorn_Still {
+public:
+    explicit GBProbe(WorkSpace &ws) : FF_GeneralizedBorn_Still(ws) {}
+    double typeRadius(int fftype) const { return GBtype[fftype].radius; }
+    double vj(int i) const { return GB_atom_param_still[i].Vj; }
+    double terms123(int i) const { return GB_atom_param_still[i].terms123; }
+    double born(int i) const { return GB_atom_param[i].bornradius; }
+    double deda(int i) const { return GB_atom_param[i].deda; }
+    double polSelf() const { return epot_pol_self; }
+    double polCross() const { return epot_pol_cross; }
+    double pol() const { return epot_pol; }
+    double p(int k) const { const double v[5] = {P1, P2, P3, P4, P5}; return v[k]; }
+};
+
+// SplitMix64 over the 20 standard residues.
+std::string random_sequence(uint64_t seed, int len) {
+    static const char *aa = "ACDEFGHIKLMNPQRSTVWY";
+    uint64_t x = seed;
+    std::string s;
+    for (int k = 0; k < len; ++k) {
+        x += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = x;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        s += aa[z % 20];
+    }
+    return s;
+}
+
+std::string pd_grammar(const std::string &seq) {
+    if (seq.size() == 1) return "*" + seq + "*";
+    if (seq.size() == 2) return "*" + seq.substr(0, 1) + "-" + seq.substr(1, 1) + "*";
+    std::string mid;
+    for (size_t k = 1; k + 1 < seq.size(); ++k) mid += seq[k];
+    return "*" + seq.substr(0, 1) + "-(" + mid + ")-" + seq.substr(seq.size() - 1) + "*";
+}
+
+struct Case {
+    std::string name, seq;
+    double cutoff, inner, vdw_cutoff, vdw_inner, padding, eps_out;
+    bool set_vdw;   // false: leave VdwCutoff/VdwInnerCutoff at PD's defaults (quirk Q6)
+    int sd_steps;   // 0: no minimisation record
+};
+
+void dump_inputs(Rec &r, WorkSpace &ws, GBProbe &gb) {
+    const int n = (int)ws.atom.size();
+    std::vector<double> xyz, q, rad, eps, gbr, vj, t123, mass;
+    std::vector<int32_t> resid;
+    std::vector<int32_t> a12, b12, a13, b13, a14, b14;
+    std::vector<double> d12, d13;
+    for (int i = 0; i < n; ++i) {
+        xyz.push_back(ws.cur.atom[i].p.x);
+        xyz.push_back(ws.cur.atom[i].p.y);
+        xyz.push_back(ws.cur.atom[i].p.z);
+        q.push_back(ws.atom[i].charge);
+        rad.push_back(ws.atom[i].radius);
+        eps.push_back(ws.atom[i].epsilon);
+        gbr.push_back(gb.typeRadius(ws.atom[i].FFType));
+        vj.push_back(gb.vj(i));
+        t123.push_back(gb.terms123(i));
+        mass.push_back(ws.atom[i].mass);
+        resid.push_back(ws.atom[i].ir);
+        for (size_t k = 0; k < ws.atom[i].cov12atom.size(); ++k) {
+            int j = ws.atom[i].cov12atom[k].i;
+            if (j > i) { a12.push_back(i); b12.push_back(j); d12.push_back(ws.atom[i].cov12atom[k].d); }
+        }
+        for (size_t k = 0; k < ws.atom[i].cov13atom.size(); ++k) {
+            int j = ws.atom[i].cov13atom[k].i;
+            if (j > i) { a13.push_back(i); b13.push_back(j); d13.push_back(ws.atom[i].cov13atom[k].d); }
+        }
+        for (size_t k = 0; k < ws.atom[i].cov14atom.size(); ++k) {
+            int j = ws.atom[i].cov14atom[k].i;
+            if (j > i) { a14.push_back(i); b14.push_back(j); }
+        }
+    }
+    r.d("xyz", xyz); r.d("charge", q); r.d("radius", rad); r.d("epsilon", eps);
+    r.d("gb_radius", gbr); r.d("vj", vj); r.d("terms123", t123); r.d("mass", mass);
+    r.i("resid", resid);
+    r.i("a12", a12); r.i("b12", b12); r.d("d12", d12);
+    r.i("a13", a13); r.i("b13", b13); r.d("d13", d13);
+    r.i("a14", a14); r.i("b14", b14);
+    const BondOrder &bo = ws.bondorder();
+    std::vector<int32_t> mid(1, bo.getMaxIndexDelta());
+    r.i("mid", mid);
+    const std::vector<char> &band = bo.getDiagData();
+    std::vector<int8_t> b8(band.begin(), band.end());
+    r.b("band", b8);
+    std::vector<int32_t> spi, spj, spo;
+    const std::vector<LongBondOrder> &sp = bo.getOffDiagData();
+    for (size_t p = 0; p < sp.size(); ++p)
+        for (size_t k = 0; k < sp[p].partner.size(); ++k) {
+            spi.push_back(sp[p].i); spj.push_back(sp[p].partner[k].j); spo.push_back(sp[p].partner[k].order);
+        }
+    r.i("sp_i", spi); r.i("sp_j", spj); r.i("sp_o", spo);
+}
+
+void dump_nlist(Rec &r, WorkSpace &ws, const std::string &tag) {
+    const int n = (int)ws.atom.size();
+    const NeighbourData *nb = ws.nlist().getData();
+    std::vector<int32_t> rs(1, 0);
+    std::vector<uint32_t> ent;
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < nb[i].n; ++k) ent.push_back((uint32_t)nb[i].i[k]);
+        rs.push_back((int32_t)ent.size());
+    }
+    r.i(tag + "_row_start", rs);
+    r.u(tag + "_entries", ent);
+}
+
+void dump_forces(Rec &r, WorkSpace &ws, const std::string &tag) {
+    const int n = (int)ws.atom.size();
+    std::vector<double> f;
+    for (int i = 0; i < n; ++i) {
+        f.push_back(ws.cur.atom[i].f.x);
+        f.push_back(ws.cur.atom[i].f.y);
+        f.push_back(ws.cur.atom[i].f.z);
+    }
+    r.d(tag, f);
+}
+
+void configure(GBProbe &gb, const Case &c) {
+    gb.Cutoff = c.cutoff;
+    gb.InnerCutoff = c.inner;
+    if (c.set_vdw) { gb.VdwCutoff = c.vdw_cutoff; gb.VdwInnerCutoff = c.vdw_inner; }
+    gb.DielectricSolvent = c.eps_out;
+}
+
+void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
+    Rec r(outdir + "/" + c.name + ".bin");
+    {
+        System sys(ffps);
+        sys.add(NewProtein(ffps, pd_grammar(c.seq)));
+        WorkSpace ws(sys);
+        ws.nlist().setPadding(c.padding);
+        Forcefield ffb(ws);
+        FF_Bonded bonds(ws);
+        ffb.add(bonds);
+        ffb.setup(); // fills cov12atom/cov13atom equilibrium distances
+        Forcefield ff(ws);
+        GBProbe gb(ws);
+        configure(gb, c);
+        ff.add(gb);
+        ff.printEnergySummary(); // ensuresetup, calcNewList (shadows on), calcForces
+        dump_inputs(r, ws, gb);
+        std::vector<double> params;
+        params.push_back(gb.Cutoff); params.push_back(gb.InnerCutoff);
+        params.push_back(gb.VdwCutoff); params.push_back(gb.VdwInnerCutoff);
+        params.push_back(gb.Vdw14Scaling); params.push_back(gb.Elec14Scaling);
+        params.push_back(gb.DielectricSolute); params.push_back(gb.DielectricSolvent);
+        params.push_back(gb.GbsaStillCutoff); params.push_back(gb.ExpApproxThreshold);
+        params.push_back(gb.DielectricOffset); params.push_back(gb.BornRadiusOffset);
+        for (int k = 0; k < 5; ++k) params.push_back(gb.p(k));
+        params.push_back(c.padding);
+        r.d("params", params);
+        dump_nlist(r, ws, "nl_full");
+        std::vector<double> born, deda;
+        for (int i = 0; i < (int)ws.atom.size(); ++i) { born.push_back(gb.born(i)); deda.push_back(gb.deda(i)); }
+        r.d("born", born);
+        r.d("deda", deda);
+        dump_forces(r, ws, "force");
+        r.s("epot", ws.ene.epot); r.s("epot_vdw", ws.ene.epot_vdw); r.s("epot_elec", ws.ene.epot_elec);
+        r.s("epot_pol", ws.ene.epot_pol); r.s("epot_pol_self", ws.ene.epot_pol_self);
+        r.s("epot_pol_cross", ws.ene.epot_pol_cross);
+
+        // energy path (calcEnergies): Born radii, FF_NonBonded, calcBornEnergy
+        ff.calcEnergies();
+        dump_forces(r, ws, "en_force");
+        r.s("en_epot", ws.ene.epot); r.s("en_epot_vdw", ws.ene.epot_vdw); r.s("en_epot_elec", ws.ene.epot_elec);
+        r.s("en_pol", gb.pol()); r.s("en_pol_self", gb.polSelf()); r.s("en_pol_cross", gb.polCross());
+        r.s("en_virial", ws.ene.InternalVirial);
+
+        // half list (shadows off), as the reference builds it inside setup
+        ws.nlist().calcShadow(false);
+        ws.nlist().calcNewList();
+        dump_nlist(r, ws, "nl_half");
+    }
+    {
+        // FF_NonBonded alone with its own defaults (force switching)
+        System sys(ffps);
+        sys.add(NewProtein(ffps, pd_grammar(c.seq)));
+        WorkSpace ws(sys);
+        ws.nlist().setPadding(c.padding);
+        Forcefield ff(ws);
+        FF_NonBonded nb(ws);
+        ff.add(nb);
+        ff.printEnergySummary();
+        dump_forces(r, ws, "nb_force");
+        r.s("nb_epot", ws.ene.epot); r.s("nb_epot_vdw", ws.ene.epot_vdw); r.s("nb_epot_elec", ws.ene.epot_elec);
+        r.s("nb_virial", ws.ene.InternalVirial);
+        std::vector<double> np;
+        np.push_back(nb.Cutoff); np.push_back(nb.InnerCutoff); np.push_back(nb.VdwCutoff);
+        np.push_back(nb.VdwInnerCutoff); np.push_back(nb.Vdw14Scaling); np.push_back(nb.Elec14Scaling);
+        np.push_back(nb.Dielectric);
+        r.d("nb_params", np);
+    }
+    if (c.sd_steps > 0) {
+        System sys(ffps);
+        sys.add(NewProtein(ffps, pd_grammar(c.seq)));
+        WorkSpace ws(sys);
+        ws.nlist().setPadding(c.padding);
+        Forcefield ffb(ws);
+        FF_Bonded bonds(ws);
+        ffb.add(bonds);
+        ffb.setup();
+        Forcefield ff(ws);
+        GBProbe gb(ws);
+        configure(gb, c);
+        ff.add(gb);
+        Protocol::Minimisation mini(ff);
+        mini.Algorithm = Protocol::Minimisation::SteepestDescent;
+        mini.Steps = c.sd_steps;
+        mini.UpdateScr = 0;
+        mini.OutputLevel = Verbosity::Silent;
+        int done = mini.run();
+        std::vector<double> xyz;
+        for (int i = 0; i < (int)ws.atom.size(); ++i) {
+            xyz.push_back(ws.cur.atom[i].p.x); xyz.push_back(ws.cur.atom[i].p.y); xyz.push_back(ws.cur.atom[i].p.z);
+        }
+        r.d("sd_xyz", xyz);
+        dump_forces(r, ws, "sd_force");
+        r.s("sd_epot", ws.ene.epot);
+        r.s("sd_steps", (double)c.sd_steps);
+        r.s("sd_done", (double)done);
+        r.s("sd_stepsize", mini.StepSize);
+    }
+}
+
+} // namespace
+
+int main(int argc, char **argv) {
+    if (argc < 3) { fprintf(stderr, "usage: ref_dump <param.ff> <outdir>\n"); return 2; }
+    FFParamSet ffps;
+    ffps.readLib(argv[1]);
+    std::vector<Case> cases;
+    // name, seq, cutoff, inner, vdwcut, vdwinner, padding, eps_out, set_vdw, sd_steps
+    Case c1 = {"tri_ala", "AAA", 12, 9, 12, 9, 2.0, 78.5, true, 20}; cases.push_back(c1);
+    Case c2 = {"gly_ala", "GA", 12, 9, 12, 9, 2.0, 78.5, true, 0}; cases.push_back(c2);
+    for (int seed = 1; seed <= 3; ++seed) {
+        Case c = {"c1_seed" + std::string(1, char('0' + seed)), random_sequence(seed, 20),
+                  12, 9, 12, 9, 2.0, 78.5, true, seed == 1 ? 200 : 30};
+        cases.push_back(c);
+    }
+    Case c60 = {"res60_seed4", random_sequence(4, 60), 12, 9, 12, 9, 2.0, 78.5, true, 0}; cases.push_back(c60);
+    Case cdef = {"pd_defaults_seed5", random_sequence(5, 12), 15, 12, 15, 12, 0.5, 80.0, false, 0}; cases.push_back(cdef);
+    Case cq6 = {"vdw_default_seed6", random_sequence(6, 12), 12, 9, 0, 0, 0.5, 78.5, false, 0}; cases.push_back(cq6);
+    Case c10 = {"cut10_seed7", random_sequence(7, 30), 10, 7, 10, 7, 2.0, 78.5, true, 0}; cases.push_back(c10);
+    for (size_t k = 0; k < cases.size(); ++k) {
+        fprintf(stderr, "case %s seq %s\n", cases[k].name.c_str(), cases[k].seq.c_str());
+        run_case(ffps, cases[k], argv[2]);
+    }
+    return 0;
+}

@@ -1,0 +1,33 @@
+"""Loads the golden fixtures written from the compiled reference."""
+import glob
+import os
+
+import numpy as np
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+PARAM_NAMES = ("cutoff", "inner_cutoff", "vdw_cutoff", "vdw_inner_cutoff", "vdw14_scaling",
+               "elec14_scaling", "dielectric_solute", "dielectric_solvent", "still_cutoff",
+               "exp_approx_threshold", "dielectric_offset", "born_radius_offset",
+               "P1", "P2", "P3", "P4", "P5", "padding")
+
+
+def case_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
+
+
+def load(name):
+    with np.load(os.path.join(GOLDEN_DIR, name + ".npz")) as z:
+        d = {k: z[k] for k in z.files}
+    d["name"] = name
+    d["n"] = len(d["charge"])
+    d["xyz"] = d["xyz"].reshape(-1, 3)
+    for k in ("force", "en_force", "nb_force", "sd_force", "sd_xyz"):
+        if k in d:
+            d[k] = d[k].reshape(-1, 3)
+    d["mid"] = int(d["mid"][0])
+    d["param_dict"] = dict(zip(PARAM_NAMES, (float(x) for x in d["params"])))
+    for k in list(d):
+        if isinstance(d[k], np.ndarray) and d[k].shape == (1,) and d[k].dtype == np.float64:
+            d[k] = float(d[k][0])
+    return d
