@@ -1,0 +1,210 @@
+/*
+ * pdgpu.h -- C ABI of the MI355X (gfx950) implementation of PD's per-step
+ * forcefield evaluation: the Still Generalized Born Born-radius pass, the
+ * fused LJ + Coulomb + GB pair pass with its chain rule, PD's energy-only
+ * path, and the neighbour-list rebuild that feeds them.
+ *
+ * The interface replaced is Physics::ForcefieldBase as implemented by
+ * FF_GeneralizedBorn_Still (reference: src/mmlib/forcefields/forcefield.h:93-222,
+ * forcefields/gbff.h:120-200) together with the NeighbourList it reads
+ * (workspace/neighbourlist.h:187-198). Each entry point below names the
+ * reference member it stands in for.
+ *
+ * Conventions
+ *  - Plain C types only; no exceptions cross this boundary. Every function
+ *    returns PDGPU_OK (0) or a negative PDGPU_ERR_* code; the message of the
+ *    last failure is available from pdgpu_last_error().
+ *  - Units at the boundary are PD's (forcefields/forcefield.cpp:18-38):
+ *    positions in Angstrom, charges in e, LJ radius R_min/2 in Angstrom,
+ *    epsilon as sqrt(J/molecule) (how PD's Particle stores it,
+ *    forcefields/ffparam.cpp:1125), energies in J/molecule, forces in
+ *    Newton, masses in kg. Arrays of vectors are AoS: x0,y0,z0,x1,...
+ *  - Input arrays are copied; the caller keeps ownership.
+ *  - A context owns one GPU stream; calls are stream-ordered and return
+ *    after the results they produce are on the host. A context belongs to
+ *    one host thread.
+ *  - Instances (independent molecules) live resident on the device and are
+ *    evaluated together when PDGPU_ALL is passed: that is the batched path.
+ */
+#ifndef PDGPU_H
+#define PDGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDGPU_OK 0
+#define PDGPU_ERR_INVALID (-1)     /* bad argument or state                      */
+#define PDGPU_ERR_NOMEM (-2)       /* device memory budget or allocation failure */
+#define PDGPU_ERR_NONFINITE (-3)   /* non-finite or out-of-range result          */
+#define PDGPU_ERR_HIP (-4)         /* HIP runtime error                          */
+#define PDGPU_ERR_UNSUPPORTED (-5) /* option the implementation does not offer  */
+#define PDGPU_ERR_NODEVICE (-6)    /* no usable gfx950 device                    */
+
+#define PDGPU_ALL (-1)
+
+enum pdgpu_precision {
+    PDGPU_FP64 = 0,  /* FP64 throughout: the plain method                         */
+    PDGPU_MIXED = 1  /* FP32 pair maths, 64-bit fixed-point force accumulation,   */
+                     /* FP64 energy reductions                                    */
+};
+
+/* Forcefield knobs, mirrored one to one from the public fields of
+ * FF_GeneralizedBorn_Still / FF_NonBonded (forcefields/gbff.h:127-133,
+ * forcefields/nonbonded.h:55-126), plus implementation settings. */
+typedef struct pdgpu_options {
+    int32_t precision;            /* pdgpu_precision                               */
+    int32_t device;               /* HIP device ordinal                            */
+    int32_t fast_mode;            /* FastMode; only 1 (the reference default)      */
+    int32_t elec_mode;            /* energy path electrostatics, PD T_ElecMode:    */
+                                  /* 0 none, 1 plain, 3 energy switch, 4 force sw. */
+    int32_t vdw_mode;             /* energy path vdW: 0 none, 2 energy switch      */
+    int32_t tiles_per_wave;       /* 0: choose automatically                       */
+    double cutoff;                /* Cutoff                                        */
+    double inner_cutoff;          /* InnerCutoff                                   */
+    double vdw_cutoff;            /* VdwCutoff                                     */
+    double vdw_inner_cutoff;      /* VdwInnerCutoff                                */
+    double vdw14_scaling;         /* Vdw14Scaling                                  */
+    double elec14_scaling;        /* Elec14Scaling                                 */
+    double dielectric_solute;     /* DielectricSolute                              */
+    double dielectric_solvent;    /* DielectricSolvent                             */
+    double gbsa_still_cutoff;     /* GbsaStillCutoff                               */
+    double exp_approx_threshold;  /* ExpApproxThreshold                            */
+    double dielectric_offset;     /* DielectricOffset                              */
+    double born_radius_offset;    /* BornRadiusOffset                              */
+    double p1, p2, p3, p4, p5;    /* Still P1..P5                                  */
+    double nlist_padding;         /* NeighbourList Padding (skin)                  */
+    int64_t device_memory_budget; /* bytes the context may allocate; 0: 80% free  */
+    int32_t max_instances;        /* instance slots; 0: 8192                       */
+    int32_t reserved;
+} pdgpu_options;
+
+/* One molecule. Per-atom arrays have natoms entries. The bond-order band and
+ * the sparse list are PD's BondOrder (workspace/bondorder.h:43-112): band has
+ * max_index_delta*(natoms-1) entries, band[i*mid + (j-i-1)] = order of (i,j)
+ * for j>i; sparse holds pairs (i<j) beyond the band. */
+typedef struct pdgpu_topology {
+    int32_t natoms;
+    int32_t max_index_delta;
+    const double *charge;    /* e                                        */
+    const double *radius;    /* R_min/2, Angstrom                        */
+    const double *epsilon;   /* sqrt(J/molecule)                         */
+    const double *gb_radius; /* Still solvation type radius, Angstrom    */
+    const double *vj;        /* Still V_j (pdgpu_fixed_terms)            */
+    const double *terms123;  /* Still fixed terms (pdgpu_fixed_terms)    */
+    const double *mass;      /* kg; may be NULL when no integrator used  */
+    const int8_t *band;
+    int32_t nsparse;
+    int32_t reserved;
+    const int32_t *sparse_i, *sparse_j, *sparse_order;
+} pdgpu_topology;
+
+/* Energy components (workspace/hamiltonian.h:23-90), J/molecule. */
+typedef struct pdgpu_energies {
+    double epot, epot_vdw, epot_elec, epot_pol, epot_pol_self, epot_pol_cross;
+    double virial;          /* InternalVirial (energy path)                      */
+    int64_t pairs;          /* unordered pairs within Cutoff (pair pass)         */
+    int64_t born_pairs;     /* ordered pairs in the Born/chain-rule sums         */
+    int32_t status;         /* PDGPU_OK or PDGPU_ERR_NONFINITE for this instance */
+    int32_t reserved;
+} pdgpu_energies;
+
+typedef struct pdgpu_ctx pdgpu_ctx;
+
+/* Fills *o with the defaults of a new FF_GeneralizedBorn_Still
+ * (forcefields/gbff.cpp:17-37, forcefields/nonbonded.cpp:968-1004). */
+void pdgpu_options_default(pdgpu_options *o);
+
+/* Creation of the component and its device state; stands in for the
+ * constructor plus ForcefieldBase::ensuresetup (forcefields/forcefield.h:143). */
+int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out);
+void pdgpu_ctx_destroy(pdgpu_ctx *ctx);
+const char *pdgpu_last_error(const pdgpu_ctx *ctx);
+
+/* Setup of one molecule: FF_GeneralizedBorn_Still::setup's parameter copy
+ * (forcefields/gbff.cpp:40-87) and WorkSpace::bondorder() upload. */
+int pdgpu_instance_add(pdgpu_ctx *ctx, const pdgpu_topology *t, int32_t *id);
+int pdgpu_instance_remove(pdgpu_ctx *ctx, int32_t id);
+int pdgpu_instance_natoms(const pdgpu_ctx *ctx, int32_t id);
+
+/* WorkSpace::cur.atom[].p in / out. */
+int pdgpu_set_positions(pdgpu_ctx *ctx, int32_t id, const double *xyz);
+int pdgpu_get_positions(pdgpu_ctx *ctx, int32_t id, double *xyz);
+
+/* NeighbourList::calcNewList (workspace/neighbourlist.cpp:205-227): rebuilds
+ * the device pair-tile list of one instance or all (PDGPU_ALL). */
+int pdgpu_nlist_build(pdgpu_ctx *ctx, int32_t id);
+
+/* NeighbourList::getData() (workspace/neighbourlist.h:46): writes PD's packed
+ * list, entries j | bondorder<<24, rows in ascending j, row_start[natoms+1].
+ * shadow=1 gives the full list, 0 only j<i. When capacity is too small,
+ * nothing is written past it, *count is the size needed and PDGPU_ERR_NOMEM
+ * is returned. */
+int pdgpu_nlist_export(pdgpu_ctx *ctx, int32_t id, int32_t shadow, int32_t *row_start,
+                       uint32_t *entries, int64_t capacity, int64_t *count);
+
+/* FF_GeneralizedBorn_Still::calcForces (forcefields/gbff.cpp:168-180):
+ * Born radii, fused pair pass, chain rule. energies: one record for id, or
+ * an array indexed by instance id for PDGPU_ALL (may be NULL). forces:
+ * natoms*3 Newton for a single id, or NULL to keep them on the device. */
+int pdgpu_calc_forces(pdgpu_ctx *ctx, int32_t id, pdgpu_energies *energies, double *forces);
+
+/* FF_GeneralizedBorn_Still::calcEnergies (forcefields/gbff.cpp:161-166):
+ * Born radii, FF_NonBonded energy path (its forces included, as the
+ * reference writes them), exact pairwise GB energy. */
+int pdgpu_calc_energies(pdgpu_ctx *ctx, int32_t id, pdgpu_energies *energies, double *forces);
+
+/* Device results of the last evaluation: Born radii (Angstrom), forces (N). */
+int pdgpu_get_born_radii(pdgpu_ctx *ctx, int32_t id, double *born);
+int pdgpu_get_forces(pdgpu_ctx *ctx, int32_t id, double *forces);
+
+/* Minimisation::runcore with SteepestDescent (protocols/minimise.cpp:99-235),
+ * run on the device for one instance or all. PD semantics: step p += f *
+ * step_size * mult, mult x1.2 on downhill, x0.5/1.2 and restore on uphill.
+ * update_nlist: rebuild every that many steps (PD default 1). grms_tol > 0
+ * stops an instance once its force RMS falls below it (kcal/mol/Angstrom),
+ * an extension PD does not have. steps_done (per id, may be NULL) receives
+ * the steps each instance ran, or -1 if its energy became non-finite. */
+int pdgpu_minimise_sd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double step_size,
+                      int32_t update_nlist, double grms_tol, int32_t *steps_done);
+
+/* Velocity Verlet, textbook form (half kick, drift, force, half kick), on the
+ * device. vel in/out in m/s (natoms*3, single id; NULL keeps device state),
+ * dt in seconds. The neighbour list is rebuilt every update_nlist steps. */
+int pdgpu_set_velocities(pdgpu_ctx *ctx, int32_t id, const double *vel);
+int pdgpu_get_velocities(pdgpu_ctx *ctx, int32_t id, double *vel);
+int pdgpu_md_vv(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
+                pdgpu_energies *energies, double *ekin);
+
+/* Host-side setup helpers (no device work).
+ * BondOrder::reinit (workspace/bondorder.cpp:170-195) from undirected 1-2,
+ * 1-3, 1-4 pair lists and residue numbers. Call with band == NULL to obtain
+ * *max_index_delta (and size band as mid*(natoms-1)); sparse arrays need
+ * n12+n13+n14 entries. */
+int pdgpu_bondorder_build(int32_t natoms, const int32_t *resid,
+                          int32_t n12, const int32_t *a12, const int32_t *b12,
+                          int32_t n13, const int32_t *a13, const int32_t *b13,
+                          int32_t n14, const int32_t *a14, const int32_t *b14,
+                          int32_t *max_index_delta, int8_t *band, int32_t *nsparse,
+                          int32_t *sparse_i, int32_t *sparse_j, int32_t *sparse_order);
+
+/* calcFixedBornRadiiTerms (forcefields/gbff.cpp:307-381): V_j and terms123
+ * from type radii and the equilibrium 1-2 / 1-3 distances. */
+int pdgpu_fixed_terms(int32_t natoms, const double *gb_radius,
+                      int32_t n12, const int32_t *a12, const int32_t *b12, const double *d12,
+                      int32_t n13, const int32_t *a13, const int32_t *b13, const double *d13,
+                      const pdgpu_options *o, double *vj, double *terms123);
+
+/* Device bytes one instance of natoms atoms with the given bond-order sizes
+ * would take in this context (admission control). */
+int64_t pdgpu_instance_bytes(const pdgpu_ctx *ctx, int32_t natoms, int32_t max_index_delta,
+                             int32_t nsparse);
+int64_t pdgpu_bytes_free(const pdgpu_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,849 @@
+// pdgpu.hip -- context, device memory layout, launch sequences and the C ABI
+// declared in include/pdgpu.h.
+#include "../../include/pdgpu.h"
+#include "pdgpu_device.h"
+#include "pdgpu_integrate.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace pdgpu;
+
+namespace {
+
+constexpr double kKcal2J = 4184.0;
+constexpr double kNa = 6.0221418E23;
+constexpr double kAngstrom = 1E-10;
+constexpr double kPI = 3.14159265358979323846;
+const double kJPerKcal = kKcal2J / kNa;          // J/molecule per kcal/mol
+const double kForceToN = kJPerKcal / kAngstrom;  // N per kcal/mol/A
+
+// PD's electrostatic constant, kcal/mol*A (forcefields/forcefield.cpp:32-36)
+double econv_kcal() {
+    const double e = 1.60217645E-19, e0 = 8.854187817E-12;
+    const double four_pi_e0 = (kPI * 4.0) * e0;
+    return ((1.0 / kKcal2J) * kNa * (e * e)) / (four_pi_e0 * kAngstrom);
+}
+
+// First-fit allocator over [0, cap) with coalescing of freed ranges.
+struct Ranges {
+    std::vector<std::pair<int64_t, int64_t>> free_; // (begin, length), sorted
+    int64_t cap = 0;
+    void init(int64_t c) { cap = c; free_.clear(); if (c > 0) free_.push_back({0, c}); }
+    int64_t alloc(int64_t n) {
+        if (n == 0) return 0;
+        for (size_t k = 0; k < free_.size(); ++k) {
+            if (free_[k].second >= n) {
+                const int64_t b = free_[k].first;
+                free_[k].first += n;
+                free_[k].second -= n;
+                if (free_[k].second == 0) free_.erase(free_.begin() + k);
+                return b;
+            }
+        }
+        return -1;
+    }
+    void release(int64_t b, int64_t n) {
+        if (n == 0) return;
+        auto it = std::lower_bound(free_.begin(), free_.end(), std::make_pair(b, (int64_t)0));
+        it = free_.insert(it, {b, n});
+        size_t k = it - free_.begin();
+        if (k + 1 < free_.size() && free_[k].first + free_[k].second == free_[k + 1].first) {
+            free_[k].second += free_[k + 1].second;
+            free_.erase(free_.begin() + k + 1);
+        }
+        if (k > 0 && free_[k - 1].first + free_[k - 1].second == free_[k].first) {
+            free_[k - 1].second += free_[k].second;
+            free_.erase(free_.begin() + k);
+        }
+    }
+    // extend capacity to c (new space appended to the free list)
+    void grow(int64_t c) {
+        if (c <= cap) return;
+        release(cap, c - cap);
+        cap = c;
+    }
+    int64_t tail_free() const {
+        if (free_.empty()) return 0;
+        const auto &b = free_.back();
+        return (b.first + b.second == cap) ? b.second : 0;
+    }
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Instance {
+    bool active = false;
+    int natoms = 0, nblocks = 0, block0 = 0, mid = 0, nsparse = 0;
+    int64_t band_off = 0, band_len = 0, sp_off = 0;
+    bool listed = false;  // tile rows built
+    bool has_vel = false;
+    std::vector<double> mass;
+};
+
+} // namespace
+
+struct pdgpu_ctx {
+    pdgpu_options opt;
+    PairParams P;
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int num_cus = 0;
+    bool mixed = false;
+    int64_t budget = 0, used = 0;
+
+    // capacities
+    int64_t cap_blocks = 0, cap_band = 0, cap_sparse = 0, cap_tiles = 0;
+    int max_inst = 0;
+    Ranges blocks, bandr, spr;
+    std::vector<Instance> inst;
+    int64_t tri_sum = 0; // sum over instances of nb*(nb+1)/2 (tile bound)
+    int64_t ntiles = 0;  // tiles after the last build
+
+    // device buffers
+    // per atom
+    DevBuf px, py, pz, q, lj64, gb64, t123, xyzq32, lj32, gb32, born, acc_g, acc_deda, acc_fx, acc_fy, acc_fz;
+    DevBuf fx, fy, fz, deda_out, vx, vy, vz, minv, opx, opy, opz;
+    // per block
+    DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, row_count, row_off, tiles;
+    // per instance
+    DevBuf inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
+    DevBuf band, sparse;
+
+    // host mirrors of per-block metadata
+    std::vector<int> h_blk_inst, h_blk_n, h_blk_first;
+
+    DevState S() const {
+        DevState s;
+        s.px = (const double *)px.p; s.py = (const double *)py.p; s.pz = (const double *)pz.p;
+        s.q = (const double *)q.p;
+        s.lj64 = (const double2 *)lj64.p; s.gb64 = (const double2 *)gb64.p;
+        s.t123 = (const double *)t123.p;
+        s.xyzq32 = (float4 *)xyzq32.p; s.lj32 = (const float2 *)lj32.p; s.gb32 = (const float2 *)gb32.p;
+        s.born = born.p; s.acc_g = acc_g.p; s.acc_deda = acc_deda.p;
+        s.acc_fx = acc_fx.p; s.acc_fy = acc_fy.p; s.acc_fz = acc_fz.p;
+        s.fx = (double *)fx.p; s.fy = (double *)fy.p; s.fz = (double *)fz.p;
+        s.deda_out = (double *)deda_out.p;
+        s.blk_inst = (const int *)blk_inst.p; s.blk_n = (const int *)blk_n.p; s.blk_first = (const int *)blk_first.p;
+        s.blk_ctr = (double4 *)blk_ctr.p; s.blk_half = (float4 *)blk_half.p; s.blk_box = (double4 *)blk_box.p;
+        s.row_count = (int *)row_count.p; s.row_off = (int *)row_off.p; s.tiles = (int2 *)tiles.p;
+        s.inst_mid = (const int *)inst_mid.p; s.inst_band_off = (const long long *)inst_band_off.p;
+        s.inst_sp_off = (const int *)inst_sp_off.p; s.inst_sp_n = (const int *)inst_sp_n.p;
+        s.band = (const int8_t *)band.p; s.sparse = (const int4 *)sparse.p;
+        s.inst_energy = inst_energy.p; s.inst_count = (unsigned long long *)inst_count.p;
+        s.inst_status = (int *)inst_status.p;
+        return s;
+    }
+};
+
+namespace {
+
+int fail(pdgpu_ctx *c, int code, const std::string &m) {
+    if (c) c->err = m;
+    return code;
+}
+
+#define HIPCHECK(c, x)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess)                                                               \
+            return fail((c), PDGPU_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// (Re)allocate buf to hold n elements of esz bytes, preserving the old
+// contents; new bytes are zeroed.
+int resize(pdgpu_ctx *c, DevBuf &b, size_t n, size_t esz) {
+    const size_t nb = std::max<size_t>(n * esz, 256);
+    if (nb <= b.bytes) return PDGPU_OK;
+    if (c->budget > 0 && c->used + (int64_t)(nb - b.bytes) > c->budget)
+        return fail(c, PDGPU_ERR_NOMEM, "device memory budget exceeded");
+    void *p = nullptr;
+    if (hipMalloc(&p, nb) != hipSuccess) return fail(c, PDGPU_ERR_NOMEM, "hipMalloc failed");
+    HIPCHECK(c, hipMemsetAsync(p, 0, nb, c->stream));
+    if (b.p) {
+        HIPCHECK(c, hipMemcpyAsync(p, b.p, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(b.p);
+    }
+    c->used += (int64_t)(nb - b.bytes);
+    b.p = p;
+    b.bytes = nb;
+    return PDGPU_OK;
+}
+
+void release(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+#define TRY(x)                          \
+    do {                                \
+        int r_ = (x);                   \
+        if (r_ != PDGPU_OK) return r_;  \
+    } while (0)
+
+int grow_atoms(pdgpu_ctx *c, int64_t blocks) {
+    const size_t na = (size_t)blocks * kBlk;
+    DevBuf *d8[] = {&c->px, &c->py, &c->pz, &c->q, &c->t123, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy,
+                    &c->acc_fz, &c->fx, &c->fy, &c->fz, &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv,
+                    &c->opx, &c->opy, &c->opz};
+    for (DevBuf *b : d8) TRY(resize(c, *b, na, 8));
+    TRY(resize(c, c->lj64, na, 16));
+    TRY(resize(c, c->gb64, na, 16));
+    TRY(resize(c, c->born, na, c->mixed ? 4 : 8));
+    if (c->mixed) {
+        TRY(resize(c, c->xyzq32, na, 16));
+        TRY(resize(c, c->lj32, na, 8));
+        TRY(resize(c, c->gb32, na, 8));
+    }
+    TRY(resize(c, c->blk_inst, blocks, 4));
+    TRY(resize(c, c->blk_n, blocks, 4));
+    TRY(resize(c, c->blk_first, blocks, 4));
+    TRY(resize(c, c->blk_ctr, blocks, 32));
+    TRY(resize(c, c->blk_box, blocks, 32));
+    TRY(resize(c, c->blk_half, blocks, 16));
+    TRY(resize(c, c->row_count, blocks, 4));
+    TRY(resize(c, c->row_off, blocks + 1, 4));
+    const int64_t old = c->cap_blocks;
+    c->h_blk_inst.resize(blocks, -1);
+    c->h_blk_n.resize(blocks, 0);
+    c->h_blk_first.resize(blocks, 0);
+    if (blocks > old) {
+        // new blocks are inactive
+        std::vector<int> neg(blocks - old, -1);
+        HIPCHECK(c, hipMemcpyAsync((int *)c->blk_inst.p + old, neg.data(), neg.size() * 4, hipMemcpyHostToDevice,
+                                   c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    c->blocks.grow(blocks);
+    c->cap_blocks = blocks;
+    return PDGPU_OK;
+}
+
+void setup_params(pdgpu_ctx *c) {
+    const pdgpu_options &o = c->opt;
+    PairParams &P = c->P;
+    std::memset(&P, 0, sizeof(P));
+    const double econv = econv_kcal();
+    P.cut = o.cutoff;
+    P.cut2 = o.cutoff * o.cutoff;
+    P.inner = o.inner_cutoff;
+    P.inner2 = o.inner_cutoff * o.inner_cutoff;
+    P.invSw = 1.0 / (o.cutoff - o.inner_cutoff);
+    P.vcut = o.vdw_cutoff;
+    P.vinner = o.vdw_inner_cutoff;
+    P.vinvSw = 1.0 / (o.vdw_cutoff - o.vdw_inner_cutoff);
+    P.s14v = o.vdw14_scaling;
+    P.s14e = o.elec14_scaling;
+    P.econv_in = econv * (1.0 / o.dielectric_solute);
+    P.econv_out = econv * (1.0 / o.dielectric_solvent);
+    P.premul = -(econv * 0.5) * (1 / o.dielectric_solute - 1 / o.dielectric_solvent);
+    // the list radius is max(Cutoff, VdwCutoff) + Padding
+    // (forcefields/nonbonded.cpp:1013, workspace/neighbourlist.cpp:258)
+    const double lc = std::max(o.cutoff, o.vdw_cutoff);
+    P.list_r = lc + o.nlist_padding;
+    P.list_lim2 = (lc + o.nlist_padding) * (lc + o.nlist_padding);
+    const double sc = std::min(o.gbsa_still_cutoff, P.list_r);
+    P.still2 = sc * sc;
+    P.invP5 = 1.0 / o.p5;
+    P.P4 = o.p4;
+    P.P5 = o.p5;
+    P.exp_thresh = o.exp_approx_threshold;
+    P.halfeconv = econv * 0.5;
+    P.born_off = o.born_radius_offset;
+    // Steinbach-Brooks force switch constants (forcefields/nonbonded.cpp:601-616)
+    const double rc = o.cutoff, ri = o.inner_cutoff, rc2 = rc * rc, ri2 = ri * ri;
+    P.fs_sA = 1.0 / ((rc2 - ri2) * (rc2 - ri2) * (rc2 - ri2));
+    P.fs_sB = -(rc2 * rc2 * rc2 - 3.0 * rc2 * rc2 * ri2);
+    P.fs_sC = 6.0 * rc2 * ri2;
+    P.fs_sD = -(rc2 + ri2);
+    P.fs_sE = 2.0 / 5.0;
+    const double fin = P.fs_sA * (P.fs_sB * (1.0 / ri) + P.fs_sC * ri + P.fs_sD * ri * ri * ri +
+                                  P.fs_sE * ri * ri * ri * ri2);
+    P.fs_cutV = P.fs_sA * (P.fs_sB * (1.0 / rc) + P.fs_sC * rc + P.fs_sD * rc * rc * rc +
+                           P.fs_sE * rc * rc * rc * rc2);
+    P.fs_eshift = (ri < rc) ? 1 / ri + (fin - P.fs_cutV) : 0;
+    P.elec_mode = o.elec_mode;
+    P.vdw_mode = o.vdw_mode;
+}
+
+bool valid_id(const pdgpu_ctx *c, int id) {
+    return id >= 0 && id < c->max_inst && c->inst[id].active;
+}
+
+// block / atom / slot ranges of an evaluation
+struct Span {
+    int b0, b1, s0, s1;
+};
+
+Span span_of(const pdgpu_ctx *c, int id) {
+    if (id == PDGPU_ALL) return {0, (int)c->cap_blocks, 0, c->max_inst};
+    const Instance &I = c->inst[id];
+    return {I.block0, I.block0 + I.nblocks, id, id + 1};
+}
+
+int waves_cap(const pdgpu_ctx *c) { return std::max(64, c->num_cus * 16); }
+
+void pair_grid(const pdgpu_ctx *c, int64_t tiles, int *groups, int *tpw) {
+    const int wcap = waves_cap(c);
+    int t = c->opt.tiles_per_wave;
+    if (t <= 0) t = (int)std::min<int64_t>(16, std::max<int64_t>(1, (tiles + wcap - 1) / wcap));
+    int64_t waves = std::max<int64_t>(1, (tiles + t - 1) / t);
+    waves = std::min<int64_t>(waves, wcap);
+    int g = (int)((waves + kWavesPerGroup - 1) / kWavesPerGroup);
+    if (g > 8) g = (g + 7) / 8 * 8;
+    *groups = g;
+    *tpw = t;
+}
+
+template <typename R>
+int launch_build(pdgpu_ctx *c, int id) {
+    const DevState S = c->S();
+    const Span sp = span_of(c, id);
+    const int nb = sp.b1 - sp.b0;
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_block_bounds, dim3((nb + 3) / 4), dim3(256), 0, c->stream, S, sp.b0, sp.b1);
+        hipLaunchKernelGGL(k_tile_rows<0>, dim3((nb + 3) / 4), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1);
+    }
+    const int all = (int)c->cap_blocks;
+    hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, c->stream, S, all);
+    hipLaunchKernelGGL(k_tile_rows<1>, dim3((all + 3) / 4), dim3(256), 0, c->stream, S, c->P, 0, all);
+    HIPCHECK(c, hipGetLastError());
+    // no host round trip: kernels read the tile count from row_off; the
+    // host keeps the upper bound for launch sizing
+    c->ntiles = c->tri_sum;
+    if (id == PDGPU_ALL) {
+        for (auto &I : c->inst)
+            if (I.active) I.listed = true;
+    } else {
+        c->inst[id].listed = true;
+    }
+    return PDGPU_OK;
+}
+
+int ensure_listed(pdgpu_ctx *c, int id) {
+    bool need = false;
+    if (id == PDGPU_ALL) {
+        for (auto &I : c->inst)
+            if (I.active && !I.listed) need = true;
+    } else {
+        need = !c->inst[id].listed;
+    }
+    if (!need) return PDGPU_OK;
+    return c->mixed ? launch_build<float>(c, id) : launch_build<double>(c, id);
+}
+
+int64_t span_tiles(const pdgpu_ctx *c, int id) {
+    if (id == PDGPU_ALL) return c->ntiles;
+    const int nb = c->inst[id].nblocks;
+    return std::min<int64_t>(c->ntiles, (int64_t)nb * (nb + 1) / 2);
+}
+
+// One evaluation: MODE 0 = calcForces, MODE 1 = calcEnergies.
+template <typename R>
+int launch_eval(pdgpu_ctx *c, int id, int mode) {
+    const DevState S = c->S();
+    const Span sp = span_of(c, id);
+    const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk;
+    const int na = a1 - a0;
+    const int ns = sp.s1 - sp.s0;
+    int groups, tpw;
+    pair_grid(c, span_tiles(c, id), &groups, &tpw);
+    hipLaunchKernelGGL(k_inst_clear, dim3((ns + 255) / 256), dim3(256), 0, c->stream, S, sp.s0, sp.s1);
+    hipLaunchKernelGGL(k_prepare<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+    hipLaunchKernelGGL(k_born<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+    hipLaunchKernelGGL(k_born_final<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, c->P, a0, a1);
+    if (mode == 0) {
+        hipLaunchKernelGGL((k_pair<R, 0>), dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+        hipLaunchKernelGGL(k_chain<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+    } else {
+        hipLaunchKernelGGL((k_pair<R, 1>), dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+    }
+    hipLaunchKernelGGL(k_finalize<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+    HIPCHECK(c, hipGetLastError());
+    return PDGPU_OK;
+}
+
+int eval(pdgpu_ctx *c, int id, int mode) {
+    TRY(ensure_listed(c, id));
+    return c->mixed ? launch_eval<float>(c, id, mode) : launch_eval<double>(c, id, mode);
+}
+
+// Converts device per-instance records into PD's units.
+int read_energies(pdgpu_ctx *c, int id, pdgpu_energies *out, int mode) {
+    const Span sp = span_of(c, id);
+    const int ns = sp.s1 - sp.s0;
+    std::vector<long long> e((size_t)ns * kENum);
+    std::vector<unsigned long long> cnt((size_t)ns * kCNum);
+    std::vector<int> st(ns);
+    HIPCHECK(c, hipMemcpyAsync(e.data(), (char *)c->inst_energy.p + (size_t)sp.s0 * kENum * 8, e.size() * 8,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(cnt.data(), (char *)c->inst_count.p + (size_t)sp.s0 * kCNum * 8, cnt.size() * 8,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(st.data(), (int *)c->inst_status.p + sp.s0, st.size() * 4, hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    int rc = PDGPU_OK;
+    for (int k = 0; k < ns; ++k) {
+        const int slot = sp.s0 + k;
+        if (!c->inst[slot].active) continue;
+        double v[kENum];
+        for (int q = 0; q < kENum; ++q) {
+            const long long raw = e[(size_t)k * kENum + q];
+            double x;
+            if (c->mixed) x = (double)raw / kEnergyScale;
+            else std::memcpy(&x, &raw, 8);
+            v[q] = x * kJPerKcal;
+        }
+        pdgpu_energies E;
+        std::memset(&E, 0, sizeof(E));
+        E.epot_vdw = v[kEVdw];
+        E.epot_elec = v[kEElec];
+        E.epot_pol_cross = v[kEPolCross];
+        E.epot_pol_self = v[kEPolSelf];
+        E.epot_pol = v[kEPolCross] + v[kEPolSelf];
+        E.virial = (mode == 1) ? v[kEVirial] : 0.0;
+        E.epot = E.epot_vdw + E.epot_elec + E.epot_pol;
+        E.pairs = (int64_t)cnt[(size_t)k * kCNum + kCPairs];
+        E.born_pairs = (int64_t)cnt[(size_t)k * kCNum + kCBornPairs];
+        const bool bad = st[k] != 0 || !std::isfinite(E.epot);
+        E.status = bad ? PDGPU_ERR_NONFINITE : PDGPU_OK;
+        if (bad) {
+            E.epot = NAN; // in-band NaN, as the reference signals instability
+            rc = PDGPU_ERR_NONFINITE;
+        }
+        if (out) {
+            if (id == PDGPU_ALL) out[slot] = E;
+            else *out = E;
+        }
+    }
+    if (rc != PDGPU_OK) c->err = "non-finite or out-of-range result";
+    return rc;
+}
+
+// Gather per-atom doubles of one instance from the padded layout.
+int gather3(pdgpu_ctx *c, int id, const DevBuf &x, const DevBuf &y, const DevBuf &z, double *out, double scale) {
+    const Instance &I = c->inst[id];
+    const size_t a0 = (size_t)I.block0 * kBlk;
+    std::vector<double> hx(I.natoms), hy(I.natoms), hz(I.natoms);
+    HIPCHECK(c, hipMemcpyAsync(hx.data(), (double *)x.p + a0, I.natoms * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(hy.data(), (double *)y.p + a0, I.natoms * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(hz.data(), (double *)z.p + a0, I.natoms * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < I.natoms; ++i) {
+        out[3 * i] = hx[i] * scale;
+        out[3 * i + 1] = hy[i] * scale;
+        out[3 * i + 2] = hz[i] * scale;
+    }
+    return PDGPU_OK;
+}
+
+int scatter3(pdgpu_ctx *c, int id, DevBuf &x, DevBuf &y, DevBuf &z, const double *in, double scale) {
+    const Instance &I = c->inst[id];
+    const size_t a0 = (size_t)I.block0 * kBlk;
+    std::vector<double> hx(I.natoms), hy(I.natoms), hz(I.natoms);
+    for (int i = 0; i < I.natoms; ++i) {
+        hx[i] = in[3 * i] * scale;
+        hy[i] = in[3 * i + 1] * scale;
+        hz[i] = in[3 * i + 2] * scale;
+    }
+    HIPCHECK(c, hipMemcpyAsync((double *)x.p + a0, hx.data(), I.natoms * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync((double *)y.p + a0, hy.data(), I.natoms * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync((double *)z.p + a0, hz.data(), I.natoms * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return PDGPU_OK;
+}
+
+} // namespace
+
+// =================================================================== C ABI
+
+extern "C" const char *pdgpu_last_error(const pdgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
+    if (!o || !out) return PDGPU_ERR_INVALID;
+    *out = nullptr;
+    if (o->fast_mode != 1) return PDGPU_ERR_UNSUPPORTED;
+    if (o->precision != PDGPU_FP64 && o->precision != PDGPU_MIXED) return PDGPU_ERR_INVALID;
+    if (!(o->cutoff > 0) || !(o->vdw_cutoff > 0) || o->inner_cutoff > o->cutoff ||
+        o->vdw_inner_cutoff > o->vdw_cutoff || !(o->p5 > 0) || !(o->dielectric_solute > 0) ||
+        !(o->dielectric_solvent > 0) || o->nlist_padding < 0)
+        return PDGPU_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= o->device || o->device < 0) return PDGPU_ERR_NODEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, o->device) != hipSuccess) return PDGPU_ERR_NODEVICE;
+    if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) return PDGPU_ERR_NODEVICE;
+    if (hipSetDevice(o->device) != hipSuccess) return PDGPU_ERR_NODEVICE;
+    pdgpu_ctx *c = new pdgpu_ctx();
+    c->opt = *o;
+    c->device = o->device;
+    c->num_cus = prop.multiProcessorCount;
+    c->mixed = o->precision == PDGPU_MIXED;
+    c->max_inst = o->max_instances > 0 ? o->max_instances : 8192;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return PDGPU_ERR_HIP;
+    }
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    c->budget = o->device_memory_budget > 0 ? o->device_memory_budget : (int64_t)(0.8 * (double)fr);
+    setup_params(c);
+    c->inst.resize(c->max_inst);
+    int rc = PDGPU_OK;
+    const int ms = c->max_inst;
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_nat, ms, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_mid, ms, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_band_off, ms, 8);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_sp_off, ms, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_sp_n, ms, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_energy, (size_t)ms * kENum, 8);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_count, (size_t)ms * kCNum, 8);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_status, ms, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_md, (size_t)ms, sizeof(InstMD));
+    if (rc == PDGPU_OK) rc = grow_atoms(c, 64);
+    if (rc == PDGPU_OK) rc = resize(c, c->band, 4096, 1);
+    if (rc == PDGPU_OK) rc = resize(c, c->sparse, 64, 16);
+    if (rc == PDGPU_OK) rc = resize(c, c->tiles, 4096, 8);
+    if (rc != PDGPU_OK) {
+        pdgpu_ctx_destroy(c);
+        return rc;
+    }
+    c->cap_band = c->band.bytes;
+    c->bandr.init(c->cap_band);
+    c->cap_sparse = c->sparse.bytes / 16;
+    c->spr.init(c->cap_sparse);
+    c->cap_tiles = c->tiles.bytes / 8;
+    *out = c;
+    return PDGPU_OK;
+}
+
+extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevBuf *all[] = {&c->px, &c->py, &c->pz, &c->q, &c->lj64, &c->gb64, &c->t123, &c->xyzq32, &c->lj32, &c->gb32,
+                     &c->born, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy, &c->acc_fz, &c->fx, &c->fy, &c->fz,
+                     &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
+                     &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->row_count, &c->row_off,
+                     &c->tiles, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
+                     &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse};
+    for (DevBuf *b : all) release(*b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int64_t pdgpu_instance_bytes(const pdgpu_ctx *c, int32_t natoms, int32_t mid, int32_t nsparse) {
+    if (!c || natoms <= 0) return -1;
+    const int64_t nb = (natoms + kBlk - 1) / kBlk;
+    const int64_t per_atom = 8 * 21 + 16 * 2 + (c->mixed ? 4 + 16 + 16 : 8);
+    const int64_t per_block = 4 * 3 + 32 * 2 + 16 + 4 * 2;
+    return nb * kBlk * per_atom + nb * per_block + (int64_t)mid * std::max(0, natoms - 1) + nsparse * 16 +
+           nb * (nb + 1) / 2 * 8;
+}
+
+extern "C" int64_t pdgpu_bytes_free(const pdgpu_ctx *c) { return c ? c->budget - c->used : 0; }
+
+extern "C" int pdgpu_instance_natoms(const pdgpu_ctx *c, int32_t id) {
+    if (!c || !valid_id(c, id)) return PDGPU_ERR_INVALID;
+    return c->inst[id].natoms;
+}
+
+extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t *id_out) {
+    if (!c || !t || !id_out) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    const int n = t->natoms;
+    if (n <= 0 || n > (1 << 24) - 1 || !t->charge || !t->radius || !t->epsilon || !t->gb_radius || !t->vj ||
+        !t->terms123 || t->max_index_delta < 1 || (n > 1 && !t->band) || t->nsparse < 0 ||
+        (t->nsparse > 0 && (!t->sparse_i || !t->sparse_j || !t->sparse_order)))
+        return fail(c, PDGPU_ERR_INVALID, "invalid topology");
+    int slot = -1;
+    for (int k = 0; k < c->max_inst; ++k)
+        if (!c->inst[k].active) { slot = k; break; }
+    if (slot < 0) return fail(c, PDGPU_ERR_NOMEM, "no free instance slot");
+    const int nb = (n + kBlk - 1) / kBlk;
+    // blocks
+    int64_t b0 = c->blocks.alloc(nb);
+    if (b0 < 0) {
+        const int64_t need = c->cap_blocks - c->blocks.tail_free() + nb;
+        TRY(grow_atoms(c, std::max<int64_t>(need, c->cap_blocks * 3 / 2)));
+        b0 = c->blocks.alloc(nb);
+        if (b0 < 0) return fail(c, PDGPU_ERR_NOMEM, "block allocation failed");
+    }
+    // band
+    const int64_t blen = (int64_t)t->max_index_delta * (n - 1);
+    int64_t boff = c->bandr.alloc(blen);
+    if (boff < 0) {
+        const int64_t need = c->cap_band - c->bandr.tail_free() + blen;
+        const int64_t nc = std::max<int64_t>(need, c->cap_band * 3 / 2);
+        int rc = resize(c, c->band, nc, 1);
+        if (rc != PDGPU_OK) { c->blocks.release(b0, nb); return rc; }
+        c->bandr.grow(nc);
+        c->cap_band = nc;
+        boff = c->bandr.alloc(blen);
+    }
+    // sparse
+    int64_t soff = c->spr.alloc(t->nsparse);
+    if (soff < 0) {
+        const int64_t need = c->cap_sparse - c->spr.tail_free() + t->nsparse;
+        const int64_t nc = std::max<int64_t>(need, c->cap_sparse * 3 / 2);
+        int rc = resize(c, c->sparse, nc, 16);
+        if (rc != PDGPU_OK) { c->blocks.release(b0, nb); c->bandr.release(boff, blen); return rc; }
+        c->spr.grow(nc);
+        c->cap_sparse = nc;
+        soff = c->spr.alloc(t->nsparse);
+    }
+    // tiles: bound of sum nb(nb+1)/2
+    const int64_t tri = c->tri_sum + (int64_t)nb * (nb + 1) / 2;
+    if (tri > c->cap_tiles) {
+        const int64_t nc = std::max<int64_t>(tri, c->cap_tiles * 3 / 2);
+        int rc = resize(c, c->tiles, nc, 8);
+        if (rc != PDGPU_OK) {
+            c->blocks.release(b0, nb); c->bandr.release(boff, blen); c->spr.release(soff, t->nsparse);
+            return rc;
+        }
+        c->cap_tiles = nc;
+    }
+    c->tri_sum = tri;
+
+    // upload per-atom data in the padded layout
+    const size_t na = (size_t)nb * kBlk;
+    const size_t a0 = (size_t)b0 * kBlk;
+    const double sqrt_j = std::sqrt(kJPerKcal);
+    std::vector<double> q(na, 0.0), t123(na, 0.0), minv(na, 0.0);
+    std::vector<double2> lj(na, make_double2(0, 0)), gb(na, make_double2(1.0, 0.0));
+    for (int i = 0; i < n; ++i) {
+        q[i] = t->charge[i];
+        lj[i] = make_double2(t->radius[i], t->epsilon[i] / sqrt_j);
+        gb[i] = make_double2(t->gb_radius[i], t->vj[i]);
+        t123[i] = t->terms123[i];
+        minv[i] = (t->mass && t->mass[i] > 0) ? 1.0 / t->mass[i] : 0.0;
+    }
+    hipStream_t s = c->stream;
+    HIPCHECK(c, hipMemcpyAsync((double *)c->q.p + a0, q.data(), na * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((double *)c->t123.p + a0, t123.data(), na * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((double *)c->minv.p + a0, minv.data(), na * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((double2 *)c->lj64.p + a0, lj.data(), na * 16, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((double2 *)c->gb64.p + a0, gb.data(), na * 16, hipMemcpyHostToDevice, s));
+    std::vector<float2> lj32(na), gb32(na);
+    if (c->mixed) {
+        for (size_t i = 0; i < na; ++i) {
+            lj32[i] = make_float2((float)lj[i].x, (float)lj[i].y);
+            gb32[i] = make_float2((float)gb[i].x, (float)gb[i].y);
+        }
+        HIPCHECK(c, hipMemcpyAsync((float2 *)c->lj32.p + a0, lj32.data(), na * 8, hipMemcpyHostToDevice, s));
+        HIPCHECK(c, hipMemcpyAsync((float2 *)c->gb32.p + a0, gb32.data(), na * 8, hipMemcpyHostToDevice, s));
+    }
+    std::vector<double> zero(na, 0.0);
+    DevBuf *zs[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz};
+    for (DevBuf *b : zs) HIPCHECK(c, hipMemcpyAsync((double *)b->p + a0, zero.data(), na * 8, hipMemcpyHostToDevice, s));
+    // per block
+    std::vector<int> bi(nb, slot), bn(nb), bf(nb, (int)b0);
+    for (int k = 0; k < nb; ++k) bn[k] = std::min(kBlk, n - k * kBlk);
+    HIPCHECK(c, hipMemcpyAsync((int *)c->blk_inst.p + b0, bi.data(), nb * 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((int *)c->blk_n.p + b0, bn.data(), nb * 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((int *)c->blk_first.p + b0, bf.data(), nb * 4, hipMemcpyHostToDevice, s));
+    for (int k = 0; k < nb; ++k) {
+        c->h_blk_inst[b0 + k] = slot;
+        c->h_blk_n[b0 + k] = bn[k];
+        c->h_blk_first[b0 + k] = (int)b0;
+    }
+    // bond orders
+    if (blen > 0)
+        HIPCHECK(c, hipMemcpyAsync((int8_t *)c->band.p + boff, t->band, blen, hipMemcpyHostToDevice, s));
+    std::vector<int4> sp(t->nsparse);
+    for (int k = 0; k < t->nsparse; ++k) {
+        int i = t->sparse_i[k], j = t->sparse_j[k];
+        if (i > j) std::swap(i, j);
+        sp[k] = make_int4(i, j, t->sparse_order[k], 0);
+    }
+    if (t->nsparse > 0)
+        HIPCHECK(c, hipMemcpyAsync((int4 *)c->sparse.p + soff, sp.data(), sp.size() * 16, hipMemcpyHostToDevice, s));
+    const int mid = t->max_index_delta;
+    const long long boffl = boff;
+    const int soffi = (int)soff, sn = t->nsparse;
+    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_mid.p + slot, &mid, 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_nat.p + slot, &n, 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((long long *)c->inst_band_off.p + slot, &boffl, 8, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_sp_off.p + slot, &soffi, 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_sp_n.p + slot, &sn, 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(c, hipStreamSynchronize(s));
+
+    Instance &I = c->inst[slot];
+    I = Instance();
+    I.active = true;
+    I.natoms = n;
+    I.nblocks = nb;
+    I.block0 = (int)b0;
+    I.mid = mid;
+    I.nsparse = t->nsparse;
+    I.band_off = boff;
+    I.band_len = blen;
+    I.sp_off = soff;
+    I.listed = false;
+    if (t->mass) I.mass.assign(t->mass, t->mass + n);
+    *id_out = slot;
+    return PDGPU_OK;
+}
+
+extern "C" int pdgpu_instance_remove(pdgpu_ctx *c, int32_t id) {
+    if (!c || !valid_id(c, id)) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    Instance &I = c->inst[id];
+    std::vector<int> neg(I.nblocks, -1), zero(I.nblocks, 0);
+    HIPCHECK(c, hipMemcpyAsync((int *)c->blk_inst.p + I.block0, neg.data(), I.nblocks * 4, hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipMemcpyAsync((int *)c->row_count.p + I.block0, zero.data(), I.nblocks * 4, hipMemcpyHostToDevice,
+                               c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const int zero1 = 0;
+    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_nat.p + id, &zero1, 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < I.nblocks; ++k) c->h_blk_inst[I.block0 + k] = -1;
+    c->blocks.release(I.block0, I.nblocks);
+    c->bandr.release(I.band_off, I.band_len);
+    c->spr.release(I.sp_off, I.nsparse);
+    c->tri_sum -= (int64_t)I.nblocks * (I.nblocks + 1) / 2;
+    I = Instance();
+    // the tile list of the remaining instances is rewritten at the next build
+    for (auto &J : c->inst)
+        if (J.active) J.listed = false;
+    return PDGPU_OK;
+}
+
+extern "C" int pdgpu_set_positions(pdgpu_ctx *c, int32_t id, const double *xyz) {
+    if (!c || !valid_id(c, id) || !xyz) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    return scatter3(c, id, c->px, c->py, c->pz, xyz, 1.0);
+}
+
+extern "C" int pdgpu_get_positions(pdgpu_ctx *c, int32_t id, double *xyz) {
+    if (!c || !valid_id(c, id) || !xyz) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    return gather3(c, id, c->px, c->py, c->pz, xyz, 1.0);
+}
+
+extern "C" int pdgpu_set_velocities(pdgpu_ctx *c, int32_t id, const double *v) {
+    if (!c || !valid_id(c, id) || !v) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    c->inst[id].has_vel = true;
+    return scatter3(c, id, c->vx, c->vy, c->vz, v, 1.0);
+}
+
+extern "C" int pdgpu_get_velocities(pdgpu_ctx *c, int32_t id, double *v) {
+    if (!c || !valid_id(c, id) || !v) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    return gather3(c, id, c->vx, c->vy, c->vz, v, 1.0);
+}
+
+extern "C" int pdgpu_nlist_build(pdgpu_ctx *c, int32_t id) {
+    if (!c || (id != PDGPU_ALL && !valid_id(c, id))) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    return c->mixed ? launch_build<float>(c, id) : launch_build<double>(c, id);
+}
+
+extern "C" int pdgpu_nlist_export(pdgpu_ctx *c, int32_t id, int32_t shadow, int32_t *row_start, uint32_t *entries,
+                                  int64_t capacity, int64_t *count) {
+    if (!c || !valid_id(c, id) || !row_start || !count || capacity < 0 || (capacity > 0 && !entries))
+        return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    TRY(ensure_listed(c, id));
+    const Instance &I = c->inst[id];
+    const DevState S = c->S();
+    int *d_cnt = nullptr;
+    long long *d_off = nullptr;
+    uint32_t *d_ent = nullptr;
+    HIPCHECK(c, hipMalloc(&d_cnt, (size_t)I.natoms * 4 + 4));
+    HIPCHECK(c, hipMemsetAsync(d_cnt, 0, (size_t)I.natoms * 4 + 4, c->stream));
+    hipLaunchKernelGGL(k_export<0>, dim3((I.nblocks + 3) / 4), dim3(256), 0, c->stream, S, c->P, I.block0, I.nblocks,
+                       shadow ? 1 : 0, d_cnt, (const long long *)nullptr, (uint32_t *)nullptr);
+    std::vector<int> cnt(I.natoms);
+    hipError_t e = hipMemcpyAsync(cnt.data(), d_cnt, (size_t)I.natoms * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) { (void)hipFree(d_cnt); return fail(c, PDGPU_ERR_HIP, hipGetErrorString(e)); }
+    std::vector<long long> off(I.natoms + 1, 0);
+    for (int i = 0; i < I.natoms; ++i) off[i + 1] = off[i] + cnt[i];
+    const int64_t total = off[I.natoms];
+    *count = total;
+    for (int i = 0; i <= I.natoms; ++i) row_start[i] = (int32_t)std::min<long long>(off[i], INT32_MAX);
+    if (total > capacity) {
+        (void)hipFree(d_cnt);
+        return fail(c, PDGPU_ERR_NOMEM, "export capacity too small");
+    }
+    e = hipMalloc(&d_off, (size_t)(I.natoms + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&d_ent, (size_t)std::max<int64_t>(total, 1) * 4);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), off.size() * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_export<1>, dim3((I.nblocks + 3) / 4), dim3(256), 0, c->stream, S, c->P, I.block0,
+                           I.nblocks, shadow ? 1 : 0, (int *)nullptr, (const long long *)d_off, d_ent);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && total > 0)
+        e = hipMemcpyAsync(entries, d_ent, (size_t)total * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_cnt);
+    if (d_off) (void)hipFree(d_off);
+    if (d_ent) (void)hipFree(d_ent);
+    if (e != hipSuccess) return fail(c, PDGPU_ERR_HIP, hipGetErrorString(e));
+    return PDGPU_OK;
+}
+
+static int calc_common(pdgpu_ctx *c, int32_t id, pdgpu_energies *en, double *forces, int mode) {
+    if (!c || (id != PDGPU_ALL && !valid_id(c, id))) return PDGPU_ERR_INVALID;
+    if (id == PDGPU_ALL && forces) return fail(c, PDGPU_ERR_INVALID, "forces output needs a single instance");
+    (void)hipSetDevice(c->device);
+    TRY(eval(c, id, mode));
+    const int rc = read_energies(c, id, en, mode);
+    if (forces) TRY(gather3(c, id, c->fx, c->fy, c->fz, forces, kForceToN));
+    return rc;
+}
+
+extern "C" int pdgpu_calc_forces(pdgpu_ctx *c, int32_t id, pdgpu_energies *en, double *forces) {
+    return calc_common(c, id, en, forces, 0);
+}
+
+extern "C" int pdgpu_calc_energies(pdgpu_ctx *c, int32_t id, pdgpu_energies *en, double *forces) {
+    return calc_common(c, id, en, forces, 1);
+}
+
+extern "C" int pdgpu_get_forces(pdgpu_ctx *c, int32_t id, double *forces) {
+    if (!c || !valid_id(c, id) || !forces) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    return gather3(c, id, c->fx, c->fy, c->fz, forces, kForceToN);
+}
+
+extern "C" int pdgpu_get_born_radii(pdgpu_ctx *c, int32_t id, double *out) {
+    if (!c || !valid_id(c, id) || !out) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    const Instance &I = c->inst[id];
+    const size_t a0 = (size_t)I.block0 * kBlk;
+    if (c->mixed) {
+        std::vector<float> h(I.natoms);
+        HIPCHECK(c, hipMemcpyAsync(h.data(), (float *)c->born.p + a0, I.natoms * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < I.natoms; ++i) out[i] = h[i];
+    } else {
+        HIPCHECK(c, hipMemcpyAsync(out, (double *)c->born.p + a0, I.natoms * 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    return PDGPU_OK;
+}
+
+// Exposes the internals the integrators need.
+namespace pdgpu_internal {
+DevState state(pdgpu_ctx *c) { return c->S(); }
+hipStream_t stream(pdgpu_ctx *c) { return c->stream; }
+}
+
+#include "pdgpu_integrate.inc"

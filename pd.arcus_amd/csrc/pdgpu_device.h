@@ -1,0 +1,852 @@
+// pdgpu_device.h -- gfx950 kernels of the GB / nonbonded hot path.
+//
+// Work decomposition. Atoms of every instance are laid out in 64-atom
+// blocks (one block = one wavefront's worth of atoms; an instance starts on
+// a block boundary, its last block is padded). The neighbour list is a list
+// of block-pair tiles (I, J<=I) whose bounding boxes come within the list
+// radius. A wavefront processes a tile with one i-atom per lane and the 64
+// j-atoms held in registers, rotated one lane per step with DPP wave_ror:1,
+// so every lane meets every j once and the j-side accumulators travel with
+// their atom. No LDS is needed for the pair loop. Unordered pairs are
+// computed once (Newton's third law); the ordered Born-radius and
+// chain-rule sums get both directions from the one symmetric pair term.
+//
+// Precision policies. R=double is the plain FP64 path (double atomics).
+// R=float is the mixed path: FP32 pair maths on block-relative coordinates,
+// forces / dE/da / Born sums accumulated in 64-bit fixed point (integer adds
+// commute, so results are bitwise reproducible whatever the tile order),
+// energies summed per lane in FP64 and accumulated in 64-bit fixed point.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pdgpu {
+
+constexpr int kBlk = 64;             // atoms per block = wavefront width
+constexpr int kWavesPerGroup = 4;    // wavefronts per workgroup of pair kernels
+
+// tile flag bits (stored in the high byte of the J word)
+constexpr int kTileDiag = 1;   // I == J: only j < i pairs
+constexpr int kTileNear = 2;   // some pair has index gap < max_index_delta: band lookups
+constexpr int kTileSparse = 4; // some off-diagonal bond order falls in the tile
+
+// per-instance energy record slots
+enum { kEVdw = 0, kEElec, kEPolCross, kEPolSelf, kEVirial, kENum = 8 };
+enum { kCPairs = 0, kCBornPairs, kCNum = 2 };
+// per-instance status bits
+constexpr int kStatNonFinite = 1;
+constexpr int kStatOverflow = 2;
+
+// fixed-point scales of the mixed path
+constexpr double kForceScale = 4294967296.0;      // 2^32 per kcal/mol/A
+constexpr double kGpolScale = 1099511627776.0;    // 2^40 per Born-sum unit
+constexpr double kEnergyScale = 4294967296.0;     // 2^32 per kcal/mol
+constexpr double kFixedLimit = 4.0e18;            // |value*scale| beyond this is flagged
+
+// Uniform parameters of one context (kcal/mol, Angstrom, e).
+struct PairParams {
+    double cut2;           // Cutoff^2
+    double inner;          // InnerCutoff
+    double invSw;          // 1/(Cutoff-InnerCutoff)
+    double vcut, vinner;   // VdwCutoff, VdwInnerCutoff
+    double vinvSw;         // 1/(VdwCutoff-VdwInnerCutoff)
+    double s14v, s14e;     // 1-4 scalings
+    double econv_in;       // econv / DielectricSolute
+    double econv_out;      // econv / DielectricSolvent
+    double premul;         // -halfeconv*(1/eps_in - 1/eps_out)
+    double still2;         // min(GbsaStillCutoff, list radius)^2
+    double invP5, P4, P5;
+    double exp_thresh;     // ExpApproxThreshold
+    double halfeconv;      // Born radius numerator
+    double born_off;       // BornRadiusOffset
+    double list_r;         // list radius: max(Cutoff,VdwCutoff)+Padding
+    double list_lim2;      // (max(Cutoff,VdwCutoff)+Padding)^2 as the reference forms it
+    // force-switch constants of the energy path
+    double fs_sA, fs_sB, fs_sC, fs_sD, fs_sE, fs_cutV, fs_eshift;
+    double cut, inner2;
+    int elec_mode, vdw_mode;
+};
+
+// Device pointers of one context.
+struct DevState {
+    // per atom (padded layout, cap_atoms)
+    const double *px, *py, *pz;   // master positions, Angstrom
+    const double *q;              // charge
+    const double2 *lj64;          // (R_min/2, sqrt(eps) in sqrt(kcal/mol))
+    const double2 *gb64;          // (Still type radius, V_j)
+    const double *t123;           // terms123
+    float4 *xyzq32;               // mixed: block-relative position + charge
+    const float2 *lj32, *gb32;
+    void *born;                   // R[]: Born radii
+    void *acc_g;                  // Born sums (int64 fixed / double)
+    void *acc_deda;               // dE/da (int64 fixed / double)
+    void *acc_fx, *acc_fy, *acc_fz; // dE/dx (int64 fixed / double)
+    double *fx, *fy, *fz;         // finalized forces, kcal/mol/A (= -dE/dx)
+    double *deda_out;             // finalized dE/da, kcal/mol/A
+    // per block
+    const int *blk_inst;          // instance slot or -1
+    const int *blk_n;             // valid atoms in block
+    const int *blk_first;         // first block of the instance
+    double4 *blk_ctr;             // reference centre (x,y,z) of the block
+    float4 *blk_half;             // bounding-box half extents (x,y,z)
+    double4 *blk_box;             // bounding-box centre (x,y,z)
+    int *row_count;               // tiles per I row
+    int *row_off;                 // exclusive scan of row_count (+ total at [nblocks])
+    int2 *tiles;                  // (I, J | flags<<24)
+    // per instance
+    const int *inst_mid;
+    const long long *inst_band_off;
+    const int *inst_sp_off, *inst_sp_n;
+    const int8_t *band;
+    const int4 *sparse;           // (i_local, j_local, order, 0), i<j
+    void *inst_energy;            // [slot*kENum]
+    unsigned long long *inst_count; // [slot*kCNum]
+    int *inst_status;
+};
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int rot_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x13C, 0xF, 0xF, false); // wave_ror:1
+}
+__device__ __forceinline__ float rot(float v) { return __int_as_float(rot_i(__float_as_int(v))); }
+__device__ __forceinline__ double rot(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = rot_i((int)(b & 0xffffffffLL));
+    const int hi = rot_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ int rot(int v) { return rot_i(v); }
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// precision traits
+template <typename R> struct Prec;
+template <> struct Prec<double> {
+    static constexpr bool fixed = false;
+    __device__ static double rsqrt(double x) { return 1.0 / sqrt(x); }
+    __device__ static double rcp(double x) { return 1.0 / x; }
+    __device__ static double expo(double x) { return exp(x); }
+    // cos/sin of th = pi * f (f in [0,1])
+    __device__ static double cospi_(double f) { return cospi(f); }
+    __device__ static double sinpi_(double f) { return sinpi(f); }
+};
+template <> struct Prec<float> {
+    static constexpr bool fixed = true;
+    __device__ static float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+    __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    __device__ static float expo(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+    // v_cos_f32 / v_sin_f32 take revolutions: cos(pi f) = cos(2 pi (f/2))
+    __device__ static float cospi_(float f) { return __builtin_amdgcn_cosf(0.5f * f); }
+    __device__ static float sinpi_(float f) { return __builtin_amdgcn_sinf(0.5f * f); }
+};
+
+// Accumulate v into an 8-byte slot: fixed point (mixed) or double (FP64).
+template <typename R>
+__device__ __forceinline__ void acc_add(void *base, size_t idx, double v, double scale, int *status) {
+    if constexpr (Prec<R>::fixed) {
+        const double s = v * scale;
+        if (!(fabs(s) < kFixedLimit)) {
+            atomicOr(status, (s == s) ? kStatOverflow : kStatNonFinite);
+            return;
+        }
+        const long long iv = __double2ll_rn(s);
+        if (iv != 0) atomicAdd(reinterpret_cast<unsigned long long *>(base) + idx, (unsigned long long)iv);
+    } else {
+        if (!(fabs(v) < 1e300)) atomicOr(status, kStatNonFinite);
+        if (v != 0.0) unsafeAtomicAdd(reinterpret_cast<double *>(base) + idx, v);
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ double acc_read(const void *base, size_t idx, double scale) {
+    if constexpr (Prec<R>::fixed) {
+        return (double)reinterpret_cast<const long long *>(base)[idx] / scale;
+    } else {
+        return reinterpret_cast<const double *>(base)[idx];
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void acc_store(void *base, size_t idx, double v, double scale) {
+    if constexpr (Prec<R>::fixed) {
+        const double s = v * scale;
+        reinterpret_cast<long long *>(base)[idx] = (fabs(s) < kFixedLimit) ? __double2ll_rn(s) : 0;
+    } else {
+        reinterpret_cast<double *>(base)[idx] = v;
+    }
+}
+
+// Bond order of (il, jl), il > jl, local indices of instance inst
+// (workspace/neighbourlist.cpp:281-289 band rule, :162-203 off-diagonal patch).
+__device__ __forceinline__ int pair_bondorder(const DevState &S, int inst, int il, int jl, int flags) {
+    const int d = il - jl;
+    const int mid = S.inst_mid[inst];
+    if (d < mid) return S.band[S.inst_band_off[inst] + (long long)jl * mid + (d - 1)] & 7;
+    int bo = 7;
+    if (flags & kTileSparse) {
+        const int off = S.inst_sp_off[inst], n = S.inst_sp_n[inst];
+        for (int k = 0; k < n; ++k) {
+            const int4 e = S.sparse[off + k];
+            if (e.x == jl && e.y == il) bo = e.z & 7;
+        }
+    }
+    return bo;
+}
+
+// Persistent-grid chunk assignment. Consecutive chunks go to workgroups on
+// the same XCD (workgroups are dealt round-robin over 8 XCDs), so the j-atom
+// data of neighbouring tiles is shared through one L2.
+__device__ __forceinline__ int wave_global_id() {
+    const int nx = 8;
+    int b = blockIdx.x;
+    if ((gridDim.x % nx) == 0) b = (blockIdx.x % nx) * (gridDim.x / nx) + blockIdx.x / nx;
+    return b * kWavesPerGroup + threadIdx.x / 64;
+}
+
+// ------------------------------------------------------ per-instance setup
+__global__ void k_inst_clear(DevState S, int slot_begin, int slot_end) {
+    const int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= slot_end) return;
+    unsigned long long *e = reinterpret_cast<unsigned long long *>(S.inst_energy) + (size_t)s * kENum;
+    for (int k = 0; k < kENum; ++k) e[k] = 0;
+    for (int k = 0; k < kCNum; ++k) S.inst_count[(size_t)s * kCNum + k] = 0;
+    S.inst_status[s] = 0;
+}
+
+// --------------------------------------------------- neighbour-list build
+// Bounding box and reference centre of each block (one wave per block).
+__global__ void k_block_bounds(DevState S, int blk_begin, int blk_end) {
+    const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (b >= blk_end) return;
+    if (S.blk_inst[b] < 0) return;
+    const int l = lane_id();
+    const bool v = l < S.blk_n[b];
+    const size_t a = (size_t)b * kBlk + l;
+    double x = v ? S.px[a] : 0, y = v ? S.py[a] : 0, z = v ? S.pz[a] : 0;
+    double mnx = v ? x : 1e300, mny = v ? y : 1e300, mnz = v ? z : 1e300;
+    double mxx = v ? x : -1e300, mxy = v ? y : -1e300, mxz = v ? z : -1e300;
+    for (int o = 32; o > 0; o >>= 1) {
+        mnx = fmin(mnx, __shfl_xor(mnx, o, 64)); mxx = fmax(mxx, __shfl_xor(mxx, o, 64));
+        mny = fmin(mny, __shfl_xor(mny, o, 64)); mxy = fmax(mxy, __shfl_xor(mxy, o, 64));
+        mnz = fmin(mnz, __shfl_xor(mnz, o, 64)); mxz = fmax(mxz, __shfl_xor(mxz, o, 64));
+    }
+    if (l == 0) {
+        const double cx = 0.5 * (mnx + mxx), cy = 0.5 * (mny + mxy), cz = 0.5 * (mnz + mxz);
+        S.blk_box[b] = make_double4(cx, cy, cz, 0.0);
+        S.blk_ctr[b] = make_double4(cx, cy, cz, 0.0);
+        // half extents rounded up, plus a margin that covers float rounding
+        const float m = 1e-3f;
+        S.blk_half[b] = make_float4((float)(0.5 * (mxx - mnx)) + m, (float)(0.5 * (mxy - mny)) + m,
+                                    (float)(0.5 * (mxz - mnz)) + m, 0.f);
+    }
+}
+
+__device__ __forceinline__ bool boxes_within(const DevState &S, int I, int J, double r) {
+    const double4 ci = S.blk_box[I], cj = S.blk_box[J];
+    const float4 hi = S.blk_half[I], hj = S.blk_half[J];
+    const double dx = fmax(0.0, fabs(ci.x - cj.x) - (double)hi.x - (double)hj.x);
+    const double dy = fmax(0.0, fabs(ci.y - cj.y) - (double)hi.y - (double)hj.y);
+    const double dz = fmax(0.0, fabs(ci.z - cj.z) - (double)hi.z - (double)hj.z);
+    return dx * dx + dy * dy + dz * dz <= r * r;
+}
+
+__device__ __forceinline__ int tile_flags(const DevState &S, int inst, int I, int J) {
+    const int first = S.blk_first[I];
+    int f = 0;
+    if (I == J) f |= kTileDiag;
+    const int gap = (I - J) * kBlk - (kBlk - 1); // smallest il - jl over the tile
+    if (gap < S.inst_mid[inst]) f |= kTileNear;
+    const int n = S.inst_sp_n[inst];
+    if (n > 0) {
+        const int off = S.inst_sp_off[inst];
+        const int i0 = (I - first) * kBlk, j0 = (J - first) * kBlk;
+        for (int k = 0; k < n; ++k) {
+            const int4 e = S.sparse[off + k];
+            if (e.x >= j0 && e.x < j0 + kBlk && e.y >= i0 && e.y < i0 + kBlk) { f |= kTileSparse; break; }
+        }
+    }
+    return f;
+}
+
+// pass 0: count tiles per I row; pass 1: write them at row_off[I].
+template <int PASS>
+__global__ void k_tile_rows(DevState S, PairParams P, int blk_begin, int blk_end) {
+    const int I = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (I >= blk_end) return;
+    const int inst = S.blk_inst[I];
+    const int l = lane_id();
+    if (inst < 0) {
+        if (PASS == 0 && l == 0) S.row_count[I] = 0;
+        return;
+    }
+    const int first = S.blk_first[I];
+    int base = (PASS == 1) ? S.row_off[I] : 0;
+    int total = 0;
+    for (int J0 = first; J0 <= I; J0 += 64) {
+        const int J = J0 + l;
+        const bool hit = (J <= I) && boxes_within(S, I, J, P.list_r);
+        const unsigned long long m = __ballot(hit);
+        if (PASS == 1 && hit) {
+            const int pos = base + total + (int)__popcll(m & ((1ull << l) - 1ull));
+            S.tiles[pos] = make_int2(I, J | (tile_flags(S, inst, I, J) << 24));
+        }
+        total += (int)__popcll(m);
+    }
+    if (PASS == 0 && l == 0) S.row_count[I] = total;
+}
+
+// Exclusive scan of row_count over [0, n) into row_off, total at row_off[n].
+// One workgroup of 1024 threads; deterministic.
+__global__ void __launch_bounds__(1024) k_scan_rows(DevState S, int n) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b0 = t * per, b1 = min(n, b0 + per);
+    int s = 0;
+    for (int i = b0; i < b1; ++i) s += S.row_count[i];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int i = b0; i < b1; ++i) { S.row_off[i] = run; run += S.row_count[i]; }
+    if (t == 1023) S.row_off[n] = part[1023];
+}
+
+// ----------------------------------------------------- per-atom preparation
+// Mixed path: block-relative FP32 coordinates. Also clears the Born sums.
+template <typename R>
+__global__ void k_prepare(DevState S, int atom_begin, int atom_end) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= atom_end) return;
+    acc_store<R>(S.acc_g, a, 0.0, 1.0);
+    if constexpr (Prec<R>::fixed) {
+        const int b = a / kBlk;
+        const double4 c = S.blk_ctr[b];
+        S.xyzq32[a] = make_float4((float)(S.px[a] - c.x), (float)(S.py[a] - c.y), (float)(S.pz[a] - c.z),
+                                  (float)S.q[a]);
+    }
+}
+
+// Per-lane view of a tile's atoms.
+template <typename R> struct AtomXYZ { R x, y, z, q; };
+
+template <typename R>
+__device__ __forceinline__ AtomXYZ<R> load_xyzq(const DevState &S, size_t a, const double4 &shift) {
+    AtomXYZ<R> r;
+    if constexpr (Prec<R>::fixed) {
+        const float4 v = S.xyzq32[a];
+        r.x = v.x + (float)shift.x; r.y = v.y + (float)shift.y; r.z = v.z + (float)shift.z; r.q = v.w;
+    } else {
+        r.x = S.px[a]; r.y = S.py[a]; r.z = S.pz[a]; r.q = S.q[a];
+    }
+    return r;
+}
+
+// FP32 path: j coordinates are re-expressed relative to block I's centre.
+template <typename R>
+__device__ __forceinline__ double4 tile_shift(const DevState &S, int I, int J) {
+    if constexpr (Prec<R>::fixed) {
+        const double4 ci = S.blk_ctr[I], cj = S.blk_ctr[J];
+        return make_double4(cj.x - ci.x, cj.y - ci.y, cj.z - ci.z, 0.0);
+    } else {
+        return make_double4(0, 0, 0, 0);
+    }
+}
+
+template <typename R> __device__ __forceinline__ R ld_born(const DevState &S, size_t a) {
+    return reinterpret_cast<const R *>(S.born)[a];
+}
+
+// ------------------------------------------------------------ Born radii A4
+// Gpol_i += P4*V_j*CCF/r^4 over pairs with bond order > 2 and r <= Still
+// cutoff (forcefields/gbff.cpp:432-465), both directions per unordered pair.
+// Every accumulator is flushed at the end of each tile, so the sums an atom
+// receives do not depend on how tiles are spread over wavefronts.
+template <typename R>
+__global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
+    const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
+    const int nw = gridDim.x * kWavesPerGroup;
+    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    const int l = lane_id();
+    const R still2 = (R)P.still2, invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5;
+    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
+        const int cend = min(t_end, c + tpw);
+        for (int t = c; t < cend; ++t) {
+            const int2 tw = S.tiles[t];
+            const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
+            const int inst = S.blk_inst[I], first = S.blk_first[I];
+            const int il = (I - first) * kBlk + l;
+            const bool vi = l < S.blk_n[I];
+            const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
+            const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+            const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+            R ri, vji, rj, vjj;
+            if constexpr (Prec<R>::fixed) {
+                const float2 gi = S.gb32[a], gj = S.gb32[b];
+                ri = gi.x; vji = gi.y; rj = gj.x; vjj = gj.y;
+            } else {
+                const double2 gi = S.gb64[a], gj = S.gb64[b];
+                ri = gi.x; vji = gi.y; rj = gj.x; vjj = gj.y;
+            }
+            const R xi = pi.x, yi = pi.y, zi = pi.z;
+            R xj = pj.x, yj = pj.y, zj = pj.z;
+            int jl = (l < S.blk_n[J]) ? (J - first) * kBlk + l : -1;
+            R gi_acc = 0, gj = 0;
+            unsigned int npairs = 0;
+            const bool diag = fl & kTileDiag;
+            const bool lookup = fl & (kTileNear | kTileSparse);
+            for (int k = 0; k < kBlk; ++k) {
+                if (k > 0) {
+                    xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj); vjj = rot(vjj); jl = rot(jl); gj = rot(gj);
+                }
+                bool ok = vi && jl >= 0 && (!diag || jl < il);
+                if (ok && lookup) ok = pair_bondorder(S, inst, il, jl, fl) > 2;
+                if (!ok) continue;
+                const R dx = xi - xj, dy = yi - yj, dz = zi - zj;
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > still2) continue;
+                const R sR = ri + rj;
+                const R x = r2 * Prec<R>::rcp(sR * sR);
+                R ccf = (R)1;
+                if (x <= invP5) {
+                    const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(x * P5);
+                    ccf = h * h;
+                }
+                const R term = P4 * ccf * Prec<R>::rcp(r2 * r2);
+                gi_acc += term * vjj;
+                gj += term * vji;
+                npairs += 2;
+            }
+            gj = rot(gj); // 64th rotation: back on the lane that owns atom j
+            int *st = S.inst_status + inst;
+            if (vi) acc_add<R>(S.acc_g, a, gi_acc, kGpolScale, st);
+            if (l < S.blk_n[J]) acc_add<R>(S.acc_g, b, gj, kGpolScale, st);
+            const unsigned int tot = wave_sum(npairs);
+            if (l == 0 && tot) atomicAdd(S.inst_count + (size_t)inst * kCNum + kCBornPairs, (unsigned long long)tot);
+        }
+    }
+}
+
+// Born radius a_i = -halfeconv/Gpol + offset (gbff.cpp:463); self term
+// premul*q^2/a and its dE/da seed (gbff.cpp:1723-1728); clears force sums.
+// Launched over whole blocks: each wavefront covers exactly one block.
+template <typename R>
+__global__ void k_born_final(DevState S, PairParams P, int atom_begin, int atom_end) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = a / kBlk;
+    const bool in = a < atom_end;
+    const int inst = in ? S.blk_inst[b] : -1;
+    const bool v = inst >= 0 && (a % kBlk) < S.blk_n[b];
+    double eself = 0;
+    if (in) {
+        acc_store<R>(S.acc_fx, a, 0.0, 1.0);
+        acc_store<R>(S.acc_fy, a, 0.0, 1.0);
+        acc_store<R>(S.acc_fz, a, 0.0, 1.0);
+        acc_store<R>(S.acc_deda, a, 0.0, 1.0);
+    }
+    if (v) {
+        const double G = S.t123[a] + acc_read<R>(S.acc_g, a, kGpolScale);
+        const double ai = -P.halfeconv / G + P.born_off;
+        reinterpret_cast<R *>(S.born)[a] = (R)ai;
+        const double qi = S.q[a];
+        eself = P.premul * (qi * qi) / ai;
+        if (!isfinite(eself)) atomicOr(S.inst_status + inst, kStatNonFinite);
+        acc_store<R>(S.acc_deda, a, -eself / ai, kForceScale);
+    }
+    const double tot = wave_sum(eself);
+    if (lane_id() == 0 && inst >= 0 && tot != 0.0)
+        acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEPolSelf, tot, kEnergyScale, S.inst_status + inst);
+}
+
+// Energies of one lane over one tile.
+struct LaneEnergy {
+    double vdw = 0, elec = 0, pol = 0, vir = 0;
+    unsigned int pairs = 0;
+};
+
+template <typename R>
+__device__ __forceinline__ void flush_energy(const DevState &S, int inst, const LaneEnergy &e) {
+    const double v = wave_sum(e.vdw), el = wave_sum(e.elec), p = wave_sum(e.pol), vr = wave_sum(e.vir);
+    const unsigned int np = wave_sum(e.pairs);
+    if (lane_id() == 0) {
+        int *st = S.inst_status + inst;
+        if (v != 0.0) acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEVdw, v, kEnergyScale, st);
+        if (el != 0.0) acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEElec, el, kEnergyScale, st);
+        if (p != 0.0) acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEPolCross, p, kEnergyScale, st);
+        if (vr != 0.0) acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEVirial, vr, kEnergyScale, st);
+        if (np) atomicAdd(S.inst_count + (size_t)inst * kCNum + kCPairs, (unsigned long long)np);
+    }
+}
+
+// Switching function S and dS/dr of PD (forcefields/gbff.cpp:1605-1614).
+template <typename R>
+__device__ __forceinline__ void switch_fn(R r, R inner, R invSw, R &S, R &dS) {
+    const R d = r - inner;
+    const R s = (R)1 - (invSw * d) * (invSw * d);
+    dS = (R)(-4) * (invSw * invSw) * d * s;
+    S = s * s;
+}
+
+// --------------------------------------------------- fused pair pass A5
+// MODE 0: FastMode force pass (forcefields/gbff.cpp:1529-1729).
+// MODE 1: energy path -- FF_NonBonded with the context's switching modes
+//         (forcefields/nonbonded.cpp:670-873) plus the exact pairwise GB
+//         energy of calcBornEnergy (forcefields/gbff.cpp:520-579).
+template <typename R, int MODE>
+__global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
+    const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
+    const int nw = gridDim.x * kWavesPerGroup;
+    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    const int l = lane_id();
+    const R cut2 = (R)P.cut2, inner = (R)P.inner, invSw = (R)P.invSw;
+    const R vcut = (R)P.vcut, vinner = (R)P.vinner, vinvSw = (R)P.vinvSw;
+    const R premul = (R)P.premul, thresh = (R)P.exp_thresh;
+    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
+        const int cend = min(t_end, c + tpw);
+        for (int t = c; t < cend; ++t) {
+            const int2 tw = S.tiles[t];
+            const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
+            const int inst = S.blk_inst[I], first = S.blk_first[I];
+            const int il = (I - first) * kBlk + l;
+            const bool vi = l < S.blk_n[I];
+            const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
+            const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+            const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+            R Ri, ei, Rj, ej;
+            if constexpr (Prec<R>::fixed) {
+                const float2 u = S.lj32[a], v = S.lj32[b];
+                Ri = u.x; ei = u.y; Rj = v.x; ej = v.y;
+            } else {
+                const double2 u = S.lj64[a], v = S.lj64[b];
+                Ri = u.x; ei = u.y; Rj = v.x; ej = v.y;
+            }
+            const R xi = pi.x, yi = pi.y, zi = pi.z, qi = pi.q;
+            const R ai = ld_born<R>(S, a);
+            R xj = pj.x, yj = pj.y, zj = pj.z, qj = pj.q;
+            R aj = ld_born<R>(S, b);
+            int jl = (l < S.blk_n[J]) ? (J - first) * kBlk + l : -1;
+            R fxi = 0, fyi = 0, fzi = 0, dai = 0;
+            R fxj = 0, fyj = 0, fzj = 0, daj = 0;
+            LaneEnergy E;
+            const bool diag = fl & kTileDiag;
+            const bool lookup = fl & (kTileNear | kTileSparse);
+            for (int k = 0; k < kBlk; ++k) {
+                if (k > 0) {
+                    xj = rot(xj); yj = rot(yj); zj = rot(zj); qj = rot(qj); Rj = rot(Rj); ej = rot(ej); aj = rot(aj);
+                    jl = rot(jl);
+                    fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
+                    if (MODE == 0) daj = rot(daj);
+                }
+                if (!(vi && jl >= 0 && (!diag || jl < il))) continue;
+                const R dx = xi - xj, dy = yi - yj, dz = zi - zj;
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > cut2) continue;
+                const int bo = lookup ? pair_bondorder(S, inst, il, jl, fl) : 7;
+                const R rinv = Prec<R>::rsqrt(r2);
+                const R r = r2 * rinv;
+                const R qiqj = qi * qj;
+                R g = 0; // (dE/dr)/r, multiplies (xi - xj)
+                E.pairs++;
+                if (MODE == 0) {
+                    const R aiaj = ai * aj;
+                    const R u = (R)(-0.25) * r2 * Prec<R>::rcp(aiaj);
+                    const R s14e = (bo == 3) ? (R)P.s14e : (R)1;
+                    R elec_pot = 0, elec_force = 0;
+                    if (u > thresh) {
+                        const R tg = Prec<R>::expo(u);
+                        const R fgb = Prec<R>::rsqrt(r2 + tg * aiaj);
+                        const R L = fgb * fgb;
+                        R epol = premul * qiqj * fgb;
+                        R dedr = -epol * (r - (R)0.25 * r * tg) * L;
+                        R dedal = -epol * tg * ((R)1 - u) * L;
+                        if (bo >= 3) {
+                            elec_pot = (R)P.econv_in * s14e * qiqj * rinv;
+                            elec_force = -elec_pot * rinv;
+                        }
+                        if (r >= inner) {
+                            R Sw, dS;
+                            switch_fn<R>(r, inner, invSw, Sw, dS);
+                            elec_force = Sw * elec_force + elec_pot * dS;
+                            elec_pot *= Sw;
+                            dedr = Sw * dedr + dS * epol;
+                            dedal *= Sw;
+                            epol *= Sw;
+                        }
+                        g = dedr * (R)2 * rinv;
+                        E.pol += 2.0 * (double)epol;
+                        dai += dedal * aj;
+                        daj += dedal * ai;
+                    } else if (bo >= 3) {
+                        // far pair: GB dropped, solvent-dielectric Coulomb instead
+                        elec_pot = (R)P.econv_out * s14e * qiqj * rinv;
+                        elec_force = -elec_pot * rinv;
+                        if (r >= inner) {
+                            R Sw, dS;
+                            switch_fn<R>(r, inner, invSw, Sw, dS);
+                            elec_force = Sw * elec_force + elec_pot * dS;
+                            elec_pot *= Sw;
+                        }
+                    }
+                    if (bo >= 3) {
+                        R vdw_force = 0;
+                        if (r < vcut) {
+                            const R s14v = (bo == 3) ? (R)P.s14v : (R)1;
+                            const R rr = (Ri + Rj) * rinv;
+                            const R rr2 = rr * rr;
+                            const R B = rr2 * rr2 * rr2;
+                            const R A = B * B;
+                            const R eps = s14v * ei * ej;
+                            R vdw_pot = ((R)2 * eps) * ((R)0.5 * A - B);
+                            vdw_force = -((R)12 * eps * rinv) * (A - B);
+                            if (r > vinner) {
+                                R vS, vd;
+                                switch_fn<R>(r, vinner, vinvSw, vS, vd);
+                                vdw_force = vS * vdw_force + vdw_pot * vd;
+                                vdw_pot *= vS;
+                            }
+                            E.vdw += (double)vdw_pot;
+                        }
+                        E.elec += (double)elec_pot;
+                        g += (vdw_force + elec_force) * rinv;
+                    }
+                } else {
+                    // FF_NonBonded energy path, bond-order scale tables {0,0,0,s14,1,1,1,1}
+                    const R sv = (bo <= 2) ? (R)0 : (bo == 3 ? (R)P.s14v : (R)1);
+                    const R se = (bo <= 2) ? (R)0 : (bo == 3 ? (R)P.s14e : (R)1);
+                    R vdw_pot = 0, vdw_force = 0, elec_pot = 0, elec_force = 0;
+                    if (P.vdw_mode > 0 && r < vcut) {
+                        const R rr = (Ri + Rj) * rinv;
+                        const R rr2 = rr * rr;
+                        const R B = rr2 * rr2 * rr2;
+                        const R A = B * B;
+                        const R eps = sv * ei * ej;
+                        vdw_pot = ((R)2 * eps) * ((R)0.5 * A - B);
+                        vdw_force = -((R)12 * eps * rinv) * (A - B);
+                        if (P.vdw_mode == 2 && r > vinner) {
+                            R vS, vd;
+                            switch_fn<R>(r, vinner, vinvSw, vS, vd);
+                            vdw_force = vS * vdw_force + vdw_pot * vd;
+                            vdw_pot *= vS;
+                        }
+                    }
+                    if (P.elec_mode == 1 || P.elec_mode == 3) {
+                        elec_pot = (R)P.econv_in * se * qiqj * rinv;
+                        elec_force = -elec_pot * rinv;
+                        if (P.elec_mode == 3 && r > inner) {
+                            R Sw, dS;
+                            switch_fn<R>(r, inner, invSw, Sw, dS);
+                            elec_force = Sw * elec_force + elec_pot * dS;
+                            elec_pot *= Sw;
+                        }
+                    } else if (P.elec_mode == 4) {
+                        elec_pot = (R)P.econv_in * se * qiqj;
+                        if (r > inner) {
+                            const R c2r = cut2 - r2;
+                            elec_force = -elec_pot * (R)P.fs_sA * (rinv * rinv) * (c2r * c2r) *
+                                         (cut2 - (R)3 * (R)P.inner2 + (R)2 * r2);
+                            elec_pot *= -((R)P.fs_sA * r * ((R)P.fs_sB * (rinv * rinv) + (R)P.fs_sC +
+                                                            (R)P.fs_sD * r2 + (R)P.fs_sE * (r2 * r2)) -
+                                          (R)P.fs_cutV);
+                        } else {
+                            elec_force = -elec_pot * (rinv * rinv);
+                            elec_pot *= (rinv - (R)P.fs_eshift);
+                        }
+                    }
+                    E.vdw += (double)vdw_pot;
+                    E.elec += (double)elec_pot;
+                    const R fm = vdw_force + elec_force;
+                    E.vir += (double)(r * fm);
+                    g = fm * rinv;
+                    // exact pairwise GB energy, no threshold branch
+                    if (r <= (R)P.cut) {
+                        R Sw = (R)1, dS;
+                        if (r >= inner) switch_fn<R>(r, inner, invSw, Sw, dS);
+                        const R s2 = r * r;
+                        const R aiaj = ai * aj;
+                        const R fgb = Prec<R>::rsqrt(s2 + aiaj * Prec<R>::expo(-s2 * (R)0.25 * Prec<R>::rcp(aiaj)));
+                        E.pol += (double)(premul * (R)2 * Sw * qiqj * fgb);
+                    }
+                }
+                const R gx = g * dx, gy = g * dy, gz = g * dz;
+                fxi += gx; fyi += gy; fzi += gz;
+                fxj -= gx; fyj -= gy; fzj -= gz;
+            }
+            fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
+            if (MODE == 0) daj = rot(daj);
+            int *st = S.inst_status + inst;
+            if (vi) {
+                acc_add<R>(S.acc_fx, a, fxi, kForceScale, st);
+                acc_add<R>(S.acc_fy, a, fyi, kForceScale, st);
+                acc_add<R>(S.acc_fz, a, fzi, kForceScale, st);
+                if (MODE == 0) acc_add<R>(S.acc_deda, a, dai, kForceScale, st);
+            }
+            if (l < S.blk_n[J]) {
+                acc_add<R>(S.acc_fx, b, fxj, kForceScale, st);
+                acc_add<R>(S.acc_fy, b, fyj, kForceScale, st);
+                acc_add<R>(S.acc_fz, b, fzj, kForceScale, st);
+                if (MODE == 0) acc_add<R>(S.acc_deda, b, daj, kForceScale, st);
+            }
+            flush_energy<R>(S, inst, E);
+        }
+    }
+}
+
+// ---------------------------------------------------------- chain rule A6
+// dE/dx through the Born radii (forcefields/gbff.cpp:1751-1808): for each
+// unordered pair, both ordered terms share CCF; w_i = a_i^2/166 * dE/da_i
+// (the reference's literal 166.0, gbff.cpp:1755).
+template <typename R>
+__device__ __forceinline__ R chain_weight(const DevState &S, size_t a) {
+    const double ai = (double)ld_born<R>(S, a);
+    return (R)(ai * ai / 166.0 * acc_read<R>(S.acc_deda, a, kForceScale));
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
+    const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
+    const int nw = gridDim.x * kWavesPerGroup;
+    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    const int l = lane_id();
+    const R still2 = (R)P.still2, invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5;
+    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
+        const int cend = min(t_end, c + tpw);
+        for (int t = c; t < cend; ++t) {
+            const int2 tw = S.tiles[t];
+            const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
+            const int inst = S.blk_inst[I], first = S.blk_first[I];
+            const int il = (I - first) * kBlk + l;
+            const bool vi = l < S.blk_n[I];
+            const bool vj_ok = l < S.blk_n[J];
+            const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
+            const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+            const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+            R ri, vji, rj, vjj;
+            if constexpr (Prec<R>::fixed) {
+                const float2 gi = S.gb32[a], gj = S.gb32[b];
+                ri = gi.x; vji = gi.y; rj = gj.x; vjj = gj.y;
+            } else {
+                const double2 gi = S.gb64[a], gj = S.gb64[b];
+                ri = gi.x; vji = gi.y; rj = gj.x; vjj = gj.y;
+            }
+            const R xi = pi.x, yi = pi.y, zi = pi.z;
+            const R wi = vi ? chain_weight<R>(S, a) : (R)0;
+            R xj = pj.x, yj = pj.y, zj = pj.z;
+            R wj = vj_ok ? chain_weight<R>(S, b) : (R)0;
+            int jl = vj_ok ? (J - first) * kBlk + l : -1;
+            R fxi = 0, fyi = 0, fzi = 0, fxj = 0, fyj = 0, fzj = 0;
+            const bool diag = fl & kTileDiag;
+            const bool lookup = fl & (kTileNear | kTileSparse);
+            for (int k = 0; k < kBlk; ++k) {
+                if (k > 0) {
+                    xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj); vjj = rot(vjj); wj = rot(wj); jl = rot(jl);
+                    fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
+                }
+                bool ok = vi && jl >= 0 && (!diag || jl < il);
+                if (ok && lookup) ok = pair_bondorder(S, inst, il, jl, fl) > 2;
+                if (!ok) continue;
+                const R dx = xj - xi, dy = yj - yi, dz = zj - zi;
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 > still2) continue;
+                const R sR = ri + rj;
+                const R x = r2 * Prec<R>::rcp(sR * sR);
+                R ccf = (R)1, dccf = (R)0;
+                if (x <= invP5) {
+                    const R f = x * P5;
+                    const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(f);
+                    ccf = h * h;
+                    dccf = (R)2 * h * Prec<R>::sinpi_(f) * (f * (R)3.14159265358979323846);
+                }
+                const R r6 = r2 * r2 * r2;
+                const R coef = P4 * ((R)4 * ccf - dccf) * Prec<R>::rcp(r6) * (wi * vjj + wj * vji);
+                const R gx = coef * dx, gy = coef * dy, gz = coef * dz;
+                fxi += gx; fyi += gy; fzi += gz;
+                fxj -= gx; fyj -= gy; fzj -= gz;
+            }
+            fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
+            int *st = S.inst_status + inst;
+            if (vi) {
+                acc_add<R>(S.acc_fx, a, fxi, kForceScale, st);
+                acc_add<R>(S.acc_fy, a, fyi, kForceScale, st);
+                acc_add<R>(S.acc_fz, a, fzi, kForceScale, st);
+            }
+            if (vj_ok) {
+                acc_add<R>(S.acc_fx, b, fxj, kForceScale, st);
+                acc_add<R>(S.acc_fy, b, fyj, kForceScale, st);
+                acc_add<R>(S.acc_fz, b, fzj, kForceScale, st);
+            }
+        }
+    }
+}
+
+// Forces f = -dE/dx in kcal/mol/A (double), dE/da for export.
+template <typename R>
+__global__ void k_finalize(DevState S, int atom_begin, int atom_end) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= atom_end) return;
+    const int b = a / kBlk;
+    const int inst = S.blk_inst[b];
+    if (inst < 0 || (a % kBlk) >= S.blk_n[b]) return;
+    const double fx = -acc_read<R>(S.acc_fx, a, kForceScale);
+    const double fy = -acc_read<R>(S.acc_fy, a, kForceScale);
+    const double fz = -acc_read<R>(S.acc_fz, a, kForceScale);
+    S.fx[a] = fx; S.fy[a] = fy; S.fz[a] = fz;
+    S.deda_out[a] = acc_read<R>(S.acc_deda, a, kForceScale);
+    if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicOr(S.inst_status + inst, kStatNonFinite);
+}
+
+// ------------------------------------------------------ PD-format export
+// Packed list of one instance exactly as calcNewList_InfiniteSpace writes
+// it (workspace/neighbourlist.cpp:263-315): j ascending, r^2 in FP64 with the
+// reference's operation order (no contraction), r^2 <= lim included.
+template <int PASS>
+__global__ void k_export(DevState S, PairParams P, int blk0, int nblk, int shadow,
+                         int *counts, const long long *offs, uint32_t *entries) {
+#pragma clang fp contract(off)
+    const int I = blk0 + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (I >= blk0 + nblk) return;
+    const int l = lane_id();
+    const int inst = S.blk_inst[I];
+    const int il = (I - blk0) * kBlk + l;
+    const bool vi = l < S.blk_n[I];
+    const size_t a = (size_t)I * kBlk + l;
+    const double xi = vi ? S.px[a] : 0, yi = vi ? S.py[a] : 0, zi = vi ? S.pz[a] : 0;
+    long long pos = (PASS == 1 && vi) ? offs[il] : 0;
+    int cnt = 0;
+    const int Jend = shadow ? blk0 + nblk - 1 : I;
+    for (int J = blk0; J <= Jend; ++J) {
+        if (!boxes_within(S, I, J, P.list_r)) continue;
+        const int nj = S.blk_n[J];
+        for (int k = 0; k < nj; ++k) {
+            const int jl = (J - blk0) * kBlk + k;
+            if (!shadow && jl >= il) break;
+            const size_t bj = (size_t)J * kBlk + k;
+            const double dx = S.px[bj] - xi;
+            const double dy = S.py[bj] - yi;
+            const double dz = S.pz[bj] - zi;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (!vi || d2 > P.list_lim2 || jl == il) continue;
+            if (PASS == 0) {
+                ++cnt;
+            } else {
+                const int lo = min(il, jl), hi = max(il, jl);
+                const int bo = (hi - lo < S.inst_mid[inst])
+                                   ? pair_bondorder(S, inst, hi, lo, 0)
+                                   : pair_bondorder(S, inst, hi, lo, kTileSparse);
+                entries[pos++] = ((uint32_t)jl & 0x00FFFFFFu) | ((uint32_t)(bo & 7) << 24);
+            }
+        }
+    }
+    if (PASS == 0 && vi) counts[il] = cnt;
+}
+
+} // namespace pdgpu

@@ -1,0 +1,168 @@
+// pdgpu_integrate.h -- device side of the step drivers: PD's steepest
+// descent minimiser (protocols/minimise.cpp:181-235) and a velocity Verlet
+// integrator, both resident on the GPU for many instances at once.
+#pragma once
+#include "pdgpu_device.h"
+
+namespace pdgpu {
+
+// Per-instance integrator state.
+struct InstMD {
+    double mult;     // SD step multiplier
+    double old_e;    // SD reference energy, J/molecule
+    double epot;     // last total energy, J/molecule
+    double f2;       // sum of |f|^2 over atoms, (kcal/mol/A)^2
+    double ekin;     // kinetic energy, J
+    int action;      // 0 none, 1 move (after saving), 2 restore
+    int done;        // instance finished (converged or failed)
+    int steps_done;  // steps run, -1 on failure
+    int pad;
+};
+
+enum { kActNone = 0, kActMove = 1, kActRestore = 2, kActSave = 3 };
+
+// Total energy of an instance in J/molecule from the device record.
+template <typename R>
+__device__ __forceinline__ double inst_epot_kcal(const DevState &S, int s) {
+    double e = 0;
+    for (int k = kEVdw; k <= kEPolSelf; ++k) e += acc_read<R>(S.inst_energy, (size_t)s * kENum + k, kEnergyScale);
+    return e;
+}
+
+__global__ void k_md_reset(InstMD *md, int s0, int s1) {
+    const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= s1) return;
+    md[s].mult = 1.0;
+    md[s].old_e = 0;
+    md[s].epot = 0;
+    md[s].f2 = 0;
+    md[s].ekin = 0;
+    md[s].action = kActNone;
+    md[s].done = 0;
+    md[s].steps_done = 0;
+}
+
+// sum of |f|^2 per instance (one wave per block)
+__global__ void k_force_sq(DevState S, InstMD *md, int blk_begin, int blk_end) {
+    const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (b >= blk_end) return;
+    const int inst = S.blk_inst[b];
+    if (inst < 0) return;
+    const int l = lane_id();
+    const size_t a = (size_t)b * kBlk + l;
+    double f2 = 0;
+    if (l < S.blk_n[b]) f2 = S.fx[a] * S.fx[a] + S.fy[a] * S.fy[a] + S.fz[a] * S.fz[a];
+    f2 = wave_sum(f2);
+    if (l == 0) unsafeAtomicAdd(&md[inst].f2, f2);
+}
+
+// PD's accept / reject rule, one thread per instance.
+template <typename R>
+__global__ void k_sd_decide(DevState S, InstMD *md, const int *natoms, int s0, int s1, int step, double jpk,
+                            double grms_tol) {
+    const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= s1) return;
+    InstMD &m = md[s];
+    m.action = kActNone;
+    if (natoms[s] <= 0 || m.done) { m.f2 = 0; return; }
+    const double E = inst_epot_kcal<R>(S, s) * jpk;
+    const double grms = sqrt(m.f2 / (3.0 * natoms[s]));
+    m.f2 = 0;
+    m.epot = E;
+    if (S.inst_status[s] != 0 || !isfinite(E)) {
+        m.done = 1;
+        m.steps_done = -1;
+        return;
+    }
+    bool accept;
+    if (step == 0) {
+        accept = true;
+    } else if (E <= m.old_e) {
+        m.mult *= 1.2;
+        accept = true;
+    } else {
+        m.mult *= 0.5 / 1.2;
+        m.old_e += 1; // PD adds 1 J/molecule: the next step is always accepted
+        m.action = kActRestore;
+        m.steps_done = step + 1;
+        return;
+    }
+    if (accept) {
+        m.old_e = E;
+        m.steps_done = step + 1;
+        if (grms_tol > 0 && grms < grms_tol) {
+            m.done = 1;
+            m.action = kActSave; // converged: keep this structure
+            return;
+        }
+        m.action = kActMove;
+    }
+}
+
+// Per-atom SD update: save and move, restore, or save only (converged).
+__global__ void k_sd_update(DevState S, InstMD *md, double *opx, double *opy, double *opz, double *px, double *py,
+                            double *pz, int atom_begin, int atom_end, double step_size, double f_to_n) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= atom_end) return;
+    const int b = a / kBlk;
+    const int inst = S.blk_inst[b];
+    if (inst < 0 || (a % kBlk) >= S.blk_n[b]) return;
+    const int act = md[inst].action;
+    if (act == kActMove || act == kActSave) {
+        opx[a] = px[a]; opy[a] = py[a]; opz[a] = pz[a];
+        if (act == kActMove) {
+            const double s = f_to_n * (step_size * md[inst].mult);
+            px[a] += S.fx[a] * s;
+            py[a] += S.fy[a] * s;
+            pz[a] += S.fz[a] * s;
+        }
+    } else if (act == kActRestore) {
+        px[a] = opx[a]; py[a] = opy[a]; pz[a] = opz[a];
+    }
+}
+
+// cur = old at the end of a minimisation (protocols/minimise.cpp:173)
+__global__ void k_copy_back(DevState S, const InstMD *md, const double *opx, const double *opy, const double *opz, double *px,
+                            double *py, double *pz, int atom_begin, int atom_end) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= atom_end) return;
+    const int b = a / kBlk;
+    const int inst = S.blk_inst[b];
+    if (inst < 0 || (a % kBlk) >= S.blk_n[b]) return;
+    if (md && md[inst].steps_done < 0) return; // failed instances keep their positions
+    px[a] = opx[a]; py[a] = opy[a]; pz[a] = opz[a];
+}
+
+// Velocity Verlet half kick (and drift when drift != 0). Forces are
+// kcal/mol/A; f_to_n converts to N, minv is 1/kg, positions in A, v in m/s.
+__global__ void k_vv(DevState S, InstMD *md, const double *minv, double *vx, double *vy, double *vz, double *px,
+                     double *py, double *pz, int atom_begin, int atom_end, double dt, double f_to_n, int drift,
+                     int want_ekin) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = a / kBlk;
+    const bool in = a < atom_end;
+    const int inst = in ? S.blk_inst[b] : -1;
+    const bool v = in && inst >= 0 && (a % kBlk) < S.blk_n[b];
+    double ek = 0;
+    if (v) {
+        const double h = 0.5 * dt * minv[a] * f_to_n;
+        double ux = vx[a] + S.fx[a] * h, uy = vy[a] + S.fy[a] * h, uz = vz[a] + S.fz[a] * h;
+        vx[a] = ux; vy[a] = uy; vz[a] = uz;
+        if (drift) {
+            const double s = dt / 1e-10;
+            px[a] += ux * s; py[a] += uy * s; pz[a] += uz * s;
+        }
+        if (minv[a] > 0) ek = 0.5 / minv[a] * (ux * ux + uy * uy + uz * uz);
+    }
+    if (want_ekin) {
+        ek = wave_sum(ek);
+        if (lane_id() == 0 && in && inst >= 0 && ek != 0.0) unsafeAtomicAdd(&md[inst].ekin, ek);
+    }
+}
+
+__global__ void k_ekin_clear(InstMD *md, int s0, int s1) {
+    const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < s1) md[s].ekin = 0;
+}
+
+} // namespace pdgpu

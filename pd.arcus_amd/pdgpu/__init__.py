@@ -1,0 +1,340 @@
+"""Python binding of libpdgpu.so, the MI355X implementation of PD's
+Generalized Born / nonbonded forcefield evaluation (C ABI: include/pdgpu.h).
+
+The binding is a thin ctypes layer: every call goes through the C ABI.
+There is no fallback; if the shared library is missing, importing the
+module raises.
+
+Units follow PD (forcefields/forcefield.cpp:18-38): Angstrom, e,
+sqrt(J/molecule) for epsilon, J/molecule, Newton, kg.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libpdgpu.so")
+
+PDGPU_OK = 0
+PDGPU_ERR_INVALID = -1
+PDGPU_ERR_NOMEM = -2
+PDGPU_ERR_NONFINITE = -3
+PDGPU_ERR_HIP = -4
+PDGPU_ERR_UNSUPPORTED = -5
+PDGPU_ERR_NODEVICE = -6
+PDGPU_ALL = -1
+FP64 = 0
+MIXED = 1
+
+KCAL2J = 4184.0
+NA = 6.0221418e23
+J_PER_KCALMOL = KCAL2J / NA
+N_PER_KCALMOL_A = J_PER_KCALMOL / 1e-10
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint32)
+_bp = C.POINTER(C.c_int8)
+
+# every entry point declared in include/pdgpu.h
+EXPORTS = (
+    "pdgpu_options_default", "pdgpu_ctx_create", "pdgpu_ctx_destroy", "pdgpu_last_error",
+    "pdgpu_instance_add", "pdgpu_instance_remove", "pdgpu_instance_natoms",
+    "pdgpu_set_positions", "pdgpu_get_positions", "pdgpu_nlist_build", "pdgpu_nlist_export",
+    "pdgpu_calc_forces", "pdgpu_calc_energies", "pdgpu_get_born_radii", "pdgpu_get_forces",
+    "pdgpu_minimise_sd", "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_md_vv",
+    "pdgpu_bondorder_build", "pdgpu_fixed_terms", "pdgpu_instance_bytes", "pdgpu_bytes_free",
+)
+
+
+class PdgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pdgpu error {code}: {msg}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("device", C.c_int32), ("fast_mode", C.c_int32),
+                ("elec_mode", C.c_int32), ("vdw_mode", C.c_int32), ("tiles_per_wave", C.c_int32)] + \
+               [(n, C.c_double) for n in (
+                   "cutoff", "inner_cutoff", "vdw_cutoff", "vdw_inner_cutoff", "vdw14_scaling",
+                   "elec14_scaling", "dielectric_solute", "dielectric_solvent", "gbsa_still_cutoff",
+                   "exp_approx_threshold", "dielectric_offset", "born_radius_offset",
+                   "p1", "p2", "p3", "p4", "p5", "nlist_padding")] + \
+               [("device_memory_budget", C.c_int64), ("max_instances", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Topology(C.Structure):
+    _fields_ = [("natoms", C.c_int32), ("max_index_delta", C.c_int32),
+                ("charge", _dp), ("radius", _dp), ("epsilon", _dp), ("gb_radius", _dp),
+                ("vj", _dp), ("terms123", _dp), ("mass", _dp), ("band", _bp),
+                ("nsparse", C.c_int32), ("reserved", C.c_int32),
+                ("sparse_i", _ip), ("sparse_j", _ip), ("sparse_order", _ip)]
+
+
+class Energies(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "epot", "epot_vdw", "epot_elec", "epot_pol", "epot_pol_self", "epot_pol_cross", "virial")] + \
+               [("pairs", C.c_int64), ("born_pairs", C.c_int64), ("status", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libpdgpu.so not built ({LIB_PATH}); run pd.arcus_amd/build.py")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    sig = {
+        "pdgpu_options_default": (None, [P(Options)]),
+        "pdgpu_ctx_create": (C.c_int, [P(Options), P(C.c_void_p)]),
+        "pdgpu_ctx_destroy": (None, [C.c_void_p]),
+        "pdgpu_last_error": (C.c_char_p, [C.c_void_p]),
+        "pdgpu_instance_add": (C.c_int, [C.c_void_p, P(Topology), _ip]),
+        "pdgpu_instance_remove": (C.c_int, [C.c_void_p, C.c_int32]),
+        "pdgpu_instance_natoms": (C.c_int, [C.c_void_p, C.c_int32]),
+        "pdgpu_set_positions": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+        "pdgpu_get_positions": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+        "pdgpu_set_velocities": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+        "pdgpu_get_velocities": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+        "pdgpu_nlist_build": (C.c_int, [C.c_void_p, C.c_int32]),
+        "pdgpu_nlist_export": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, _up, C.c_int64, P(C.c_int64)]),
+        "pdgpu_calc_forces": (C.c_int, [C.c_void_p, C.c_int32, P(Energies), _dp]),
+        "pdgpu_calc_energies": (C.c_int, [C.c_void_p, C.c_int32, P(Energies), _dp]),
+        "pdgpu_get_born_radii": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+        "pdgpu_get_forces": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+        "pdgpu_minimise_sd": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double, _ip]),
+        "pdgpu_md_vv": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, P(Energies), _dp]),
+        "pdgpu_bondorder_build": (C.c_int, [C.c_int32, _ip, C.c_int32, _ip, _ip, C.c_int32, _ip, _ip, C.c_int32,
+                                            _ip, _ip, _ip, _bp, _ip, _ip, _ip, _ip]),
+        "pdgpu_fixed_terms": (C.c_int, [C.c_int32, _dp, C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _ip, _dp,
+                                        P(Options), _dp, _dp]),
+        "pdgpu_instance_bytes": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+        "pdgpu_bytes_free": (C.c_int64, [C.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def default_options(**kw):
+    o = Options()
+    lib.pdgpu_options_default(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+def bondorder_build(resid, a12, b12, a13, b13, a14, b14):
+    """PD BondOrder band + sparse list from undirected pair lists."""
+    resid = _i32(resid)
+    n = len(resid)
+    arrs = [_i32(x) for x in (a12, b12, a13, b13, a14, b14)]
+    mid = np.zeros(1, np.int32)
+    args = [n, _p(resid, _ip), len(arrs[0]), _p(arrs[0], _ip), _p(arrs[1], _ip), len(arrs[2]), _p(arrs[2], _ip),
+            _p(arrs[3], _ip), len(arrs[4]), _p(arrs[4], _ip), _p(arrs[5], _ip), _p(mid, _ip)]
+    rc = lib.pdgpu_bondorder_build(*args, None, None, None, None, None)
+    if rc != 0:
+        raise PdgpuError(rc, "bondorder_build")
+    m = int(mid[0])
+    band = np.zeros(max(0, m * (n - 1)), np.int8)
+    cap = len(arrs[0]) + len(arrs[2]) + len(arrs[4]) + 1
+    si, sj, so = (np.zeros(cap, np.int32) for _ in range(3))
+    ns = np.zeros(1, np.int32)
+    rc = lib.pdgpu_bondorder_build(*args, _p(band, _bp), _p(ns, _ip), _p(si, _ip), _p(sj, _ip), _p(so, _ip))
+    if rc != 0:
+        raise PdgpuError(rc, "bondorder_build")
+    k = int(ns[0])
+    return m, band, si[:k].copy(), sj[:k].copy(), so[:k].copy()
+
+
+def fixed_terms(gb_radius, a12, b12, d12, a13, b13, d13, options=None):
+    """Still V_j and terms123 (calcFixedBornRadiiTerms)."""
+    o = options if options is not None else default_options()
+    g = _f64(gb_radius)
+    n = len(g)
+    a12, b12, a13, b13 = (_i32(x) for x in (a12, b12, a13, b13))
+    d12, d13 = _f64(d12), _f64(d13)
+    vj = np.zeros(n)
+    t = np.zeros(n)
+    rc = lib.pdgpu_fixed_terms(n, _p(g, _dp), len(a12), _p(a12, _ip), _p(b12, _ip), _p(d12, _dp), len(a13),
+                               _p(a13, _ip), _p(b13, _ip), _p(d13, _dp), C.byref(o), _p(vj, _dp), _p(t, _dp))
+    if rc != 0:
+        raise PdgpuError(rc, "fixed_terms")
+    return vj, t
+
+
+class Context:
+    """One pdgpu context (one GPU stream) holding resident instances."""
+
+    def __init__(self, options=None, **kw):
+        self.opt = options if options is not None else default_options(**kw)
+        h = C.c_void_p()
+        rc = lib.pdgpu_ctx_create(C.byref(self.opt), C.byref(h))
+        if rc != 0:
+            raise PdgpuError(rc, "pdgpu_ctx_create failed")
+        self.h = h
+        self._natoms = {}
+
+    def close(self):
+        if self.h:
+            lib.pdgpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PdgpuError(rc, f"{what}: {lib.pdgpu_last_error(self.h).decode()}")
+
+    def add(self, charge, radius, epsilon, gb_radius, vj, terms123, mid, band, mass=None,
+            sparse_i=None, sparse_j=None, sparse_order=None, xyz=None):
+        keep = [_f64(charge), _f64(radius), _f64(epsilon), _f64(gb_radius), _f64(vj), _f64(terms123)]
+        n = len(keep[0])
+        massa = _f64(mass) if mass is not None else None
+        banda = np.ascontiguousarray(band, dtype=np.int8)
+        e = np.zeros(0, np.int32)
+        si = _i32(sparse_i if sparse_i is not None else e)
+        sj = _i32(sparse_j if sparse_j is not None else e)
+        so = _i32(sparse_order if sparse_order is not None else e)
+        t = Topology(n, int(mid), *[_p(a, _dp) for a in keep], _p(massa, _dp), _p(banda, _bp), len(si), 0,
+                     _p(si, _ip), _p(sj, _ip), _p(so, _ip))
+        i = C.c_int32()
+        self._check(lib.pdgpu_instance_add(self.h, C.byref(t), C.byref(i)), "instance_add")
+        self._natoms[i.value] = n
+        if xyz is not None:
+            self.set_positions(i.value, xyz)
+        return i.value
+
+    def add_system(self, s, xyz=None):
+        """s: mapping with PD per-atom arrays (see pdgpu.synth / golden fixtures)."""
+        return self.add(s["charge"], s["radius"], s["epsilon"], s["gb_radius"], s["vj"], s["terms123"],
+                        s["mid"], s["band"], s.get("mass"), s.get("sp_i"), s.get("sp_j"), s.get("sp_o"),
+                        xyz=s["xyz"] if xyz is None else xyz)
+
+    def remove(self, iid):
+        self._check(lib.pdgpu_instance_remove(self.h, iid), "instance_remove")
+        self._natoms.pop(iid, None)
+
+    def natoms(self, iid):
+        return self._natoms[iid]
+
+    def set_positions(self, iid, xyz):
+        x = _f64(xyz).reshape(-1)
+        assert len(x) == 3 * self._natoms[iid]
+        self._check(lib.pdgpu_set_positions(self.h, iid, _p(x, _dp)), "set_positions")
+
+    def get_positions(self, iid):
+        x = np.zeros(3 * self._natoms[iid])
+        self._check(lib.pdgpu_get_positions(self.h, iid, _p(x, _dp)), "get_positions")
+        return x.reshape(-1, 3)
+
+    def set_velocities(self, iid, v):
+        x = _f64(v).reshape(-1)
+        assert len(x) == 3 * self._natoms[iid]
+        self._check(lib.pdgpu_set_velocities(self.h, iid, _p(x, _dp)), "set_velocities")
+
+    def get_velocities(self, iid):
+        x = np.zeros(3 * self._natoms[iid])
+        self._check(lib.pdgpu_get_velocities(self.h, iid, _p(x, _dp)), "get_velocities")
+        return x.reshape(-1, 3)
+
+    def nlist_build(self, iid=PDGPU_ALL):
+        self._check(lib.pdgpu_nlist_build(self.h, iid), "nlist_build")
+
+    def nlist_export(self, iid, shadow=True):
+        n = self._natoms[iid]
+        rs = np.zeros(n + 1, np.int32)
+        cnt = C.c_int64()
+        rc = lib.pdgpu_nlist_export(self.h, iid, int(shadow), _p(rs, _ip), None, 0, C.byref(cnt))
+        if rc not in (0, PDGPU_ERR_NOMEM):
+            self._check(rc, "nlist_export")
+        ent = np.zeros(max(1, cnt.value), np.uint32)
+        self._check(lib.pdgpu_nlist_export(self.h, iid, int(shadow), _p(rs, _ip), _p(ent, _up), len(ent),
+                                           C.byref(cnt)), "nlist_export")
+        return rs, ent[:cnt.value]
+
+    def _calc(self, fn, iid, want_forces, check):
+        if iid == PDGPU_ALL:
+            n = self.opt.max_instances if self.opt.max_instances > 0 else 8192
+            arr = (Energies * n)()
+            rc = fn(self.h, iid, arr, None)
+            if check and rc not in (0, PDGPU_ERR_NONFINITE):
+                self._check(rc, "calc")
+            return {i: arr[i].as_dict() for i in self._natoms}
+        e = Energies()
+        f = np.zeros(3 * self._natoms[iid]) if want_forces else None
+        rc = fn(self.h, iid, C.byref(e), _p(f, _dp))
+        if check and rc not in (0, PDGPU_ERR_NONFINITE):
+            self._check(rc, "calc")
+        return e.as_dict(), (f.reshape(-1, 3) if want_forces else None)
+
+    def calc_forces(self, iid=PDGPU_ALL, forces=True, check=True):
+        return self._calc(lib.pdgpu_calc_forces, iid, forces, check)
+
+    def calc_energies(self, iid=PDGPU_ALL, forces=True, check=True):
+        return self._calc(lib.pdgpu_calc_energies, iid, forces, check)
+
+    def born_radii(self, iid):
+        b = np.zeros(self._natoms[iid])
+        self._check(lib.pdgpu_get_born_radii(self.h, iid, _p(b, _dp)), "get_born_radii")
+        return b
+
+    def forces(self, iid):
+        f = np.zeros(3 * self._natoms[iid])
+        self._check(lib.pdgpu_get_forces(self.h, iid, _p(f, _dp)), "get_forces")
+        return f.reshape(-1, 3)
+
+    def minimise_sd(self, iid, steps, step_size=2e5, update_nlist=1, grms_tol=0.0):
+        if iid == PDGPU_ALL:
+            n = self.opt.max_instances if self.opt.max_instances > 0 else 8192
+            out = np.zeros(n, np.int32)
+        else:
+            out = np.zeros(1, np.int32)
+        rc = lib.pdgpu_minimise_sd(self.h, iid, steps, step_size, update_nlist, grms_tol, _p(out, _ip))
+        if rc not in (0, PDGPU_ERR_NONFINITE):
+            self._check(rc, "minimise_sd")
+        return out if iid == PDGPU_ALL else int(out[0])
+
+    def md_vv(self, iid, steps, dt, update_nlist=10):
+        e = Energies()
+        ek = np.zeros(1)
+        self._check(lib.pdgpu_md_vv(self.h, iid, steps, dt, update_nlist, C.byref(e), _p(ek, _dp)), "md_vv")
+        return e.as_dict(), float(ek[0])
+
+    def bytes_free(self):
+        return lib.pdgpu_bytes_free(self.h)
+
+    def instance_bytes(self, natoms, mid, nsparse=0):
+        return lib.pdgpu_instance_bytes(self.h, natoms, mid, nsparse)

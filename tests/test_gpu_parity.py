@@ -1,0 +1,160 @@
+"""GPU parity: the HIP path through the C ABI against the compiled
+reference's golden vectors (and so against the oracle, which matches them
+to ~1e-15).
+
+Tolerances: FP64 path -- energy relative error <= 1e-10 per component,
+force RMS relative error <= 1e-9, Born radii <= 1e-10; neighbour lists
+bit-exact. Mixed path -- total energy relative error <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+import golden_io as G
+import pdgpu
+from gpu_helpers import context_for, rel, rms_rel
+
+pytestmark = pytest.mark.gpu
+CASES = G.case_names()
+E_KEYS = ("epot", "epot_vdw", "epot_elec", "epot_pol", "epot_pol_self", "epot_pol_cross")
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_force_path(name):
+    g = G.load(name)
+    with context_for(g) as ctx:
+        i = ctx.add_system(g)
+        e, f = ctx.calc_forces(i)
+        assert e["status"] == 0
+        for k in E_KEYS:
+            assert rel(e[k], g[k]) <= 1e-10, (k, e[k], g[k])
+        assert rms_rel(f, g["force"]) <= 1e-9
+        born = ctx.born_radii(i)
+        assert np.max(np.abs(born - g["born"]) / np.abs(g["born"])) <= 1e-10
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fp64_energy_path(name):
+    g = G.load(name)
+    with context_for(g) as ctx:
+        i = ctx.add_system(g)
+        e, f = ctx.calc_energies(i)
+        assert rel(e["epot"], g["en_epot"]) <= 1e-10
+        assert rel(e["epot_vdw"], g["en_epot_vdw"]) <= 1e-10
+        assert rel(e["epot_elec"], g["en_epot_elec"]) <= 1e-10
+        assert rel(e["epot_pol"], g["en_pol"]) <= 1e-10
+        assert rel(e["epot_pol_self"], g["en_pol_self"]) <= 1e-10
+        assert rel(e["virial"], g["en_virial"]) <= 1e-10
+        assert rms_rel(f, g["en_force"]) <= 1e-9
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("precision", [pdgpu.FP64, pdgpu.MIXED])
+def test_nlist_export_exact(name, precision):
+    g = G.load(name)
+    with context_for(g, precision) as ctx:
+        i = ctx.add_system(g)
+        ctx.nlist_build(i)
+        rs, ent = ctx.nlist_export(i, shadow=True)
+        assert np.array_equal(rs, g["nl_full_row_start"])
+        assert np.array_equal(ent, g["nl_full_entries"])
+        rs, ent = ctx.nlist_export(i, shadow=False)
+        assert np.array_equal(rs, g["nl_half_row_start"])
+        assert np.array_equal(ent, g["nl_half_entries"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_mixed_force_path(name):
+    g = G.load(name)
+    with context_for(g, pdgpu.MIXED) as ctx:
+        i = ctx.add_system(g)
+        e, f = ctx.calc_forces(i)
+        assert e["status"] == 0
+        assert rel(e["epot"], g["epot"]) <= 1e-5, (e["epot"], g["epot"])
+        scale = max(abs(g[k]) for k in E_KEYS)
+        for k in E_KEYS:
+            assert abs(e[k] - g[k]) <= 1e-5 * scale, k
+        assert rms_rel(f, g["force"]) <= 1e-4
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_mixed_energy_path(name):
+    g = G.load(name)
+    with context_for(g, pdgpu.MIXED) as ctx:
+        i = ctx.add_system(g)
+        e, f = ctx.calc_energies(i)
+        assert rel(e["epot"], g["en_epot"]) <= 1e-5
+
+
+BATCH = ["c1_seed1", "c1_seed2", "c1_seed3", "tri_ala", "gly_ala"]
+
+
+@pytest.mark.parametrize("precision", [pdgpu.FP64, pdgpu.MIXED])
+def test_batch_equals_alone(precision):
+    gs = [G.load(n) for n in BATCH]
+    alone = []
+    for g in gs:
+        with context_for(g, precision) as ctx:
+            i = ctx.add_system(g)
+            alone.append(ctx.calc_forces(i))
+    with context_for(gs[0], precision) as ctx:
+        ids = [ctx.add_system(g) for g in gs]
+        ctx.nlist_build()
+        res = ctx.calc_forces(pdgpu.PDGPU_ALL)
+        for k, (g, i) in enumerate(zip(gs, ids)):
+            e = res[i]
+            f = ctx.forces(i)
+            ea, fa = alone[k]
+            if precision == pdgpu.MIXED:
+                # fixed-point accumulation: identical bits alone and in the mix
+                assert e["epot"] == ea["epot"]
+                assert np.array_equal(f, fa)
+            else:
+                assert rel(e["epot"], ea["epot"]) <= 1e-12
+                assert rms_rel(f, fa) <= 1e-12
+            assert rel(e["epot"], g["epot"]) <= (1e-10 if precision == pdgpu.FP64 else 1e-5)
+
+
+def test_mixed_is_deterministic():
+    g = G.load("res60_seed4")
+    with context_for(g, pdgpu.MIXED) as ctx:
+        i = ctx.add_system(g)
+        e1, f1 = ctx.calc_forces(i)
+        e2, f2 = ctx.calc_forces(i)
+        assert e1["epot"] == e2["epot"] and np.array_equal(f1, f2)
+
+
+@pytest.mark.parametrize("name", [n for n in CASES if "sd_xyz" in np.load(G.GOLDEN_DIR + "/" + n + ".npz").files])
+def test_fp64_sd_minimiser(name):
+    g = G.load(name)
+    with context_for(g) as ctx:
+        i = ctx.add_system(g)
+        done = ctx.minimise_sd(i, int(g["sd_steps"]), g["sd_stepsize"], 1)
+        assert done == int(g["sd_done"])
+        xyz = ctx.get_positions(i)
+        assert np.max(np.abs(xyz - g["sd_xyz"])) <= 1e-7
+        e, f = ctx.calc_forces(i)
+        assert rel(e["epot"], g["sd_epot"]) <= 1e-9
+
+
+def test_nonfinite_instance_is_isolated():
+    g = G.load("tri_ala")
+    with context_for(g, pdgpu.MIXED) as ctx:
+        good = ctx.add_system(g)
+        xyz = g["xyz"].copy()
+        xyz[5] = xyz[12]  # two non-bonded atoms on top of each other
+        bad = ctx.add_system(g, xyz=xyz)
+        res = ctx.calc_forces(pdgpu.PDGPU_ALL)
+        assert res[bad]["status"] != 0 and np.isnan(res[bad]["epot"])
+        assert res[good]["status"] == 0
+        assert rel(res[good]["epot"], g["epot"]) <= 1e-5
+
+
+def test_remove_and_reuse_slots():
+    gs = [G.load(n) for n in ("tri_ala", "gly_ala", "c1_seed2")]
+    with context_for(gs[0]) as ctx:
+        ids = [ctx.add_system(g) for g in gs]
+        ctx.remove(ids[1])
+        j = ctx.add_system(gs[1])
+        res = ctx.calc_forces(pdgpu.PDGPU_ALL)
+        for g, i in ((gs[0], ids[0]), (gs[1], j), (gs[2], ids[2])):
+            assert rel(res[i]["epot"], g["epot"]) <= 1e-10
