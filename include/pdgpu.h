@@ -179,6 +179,23 @@ int pdgpu_get_velocities(pdgpu_ctx *ctx, int32_t id, double *vel);
 int pdgpu_md_vv(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
                 pdgpu_energies *energies, double *ekin);
 
+/* Optional timing of the kernel groups below with HIP events recorded on
+ * the context's stream around each launch (no reference counterpart; used
+ * by the benchmark). pdgpu_profile(ctx, 1) clears and starts recording,
+ * (ctx, 0) clears and stops; pdgpu_profile_read sums milliseconds and
+ * launch counts per group since the last clear. */
+enum pdgpu_kernel_group {
+    PDGPU_KERNEL_NLIST = 0,     /* bounding boxes, tile rows, row scan      */
+    PDGPU_KERNEL_ATOM = 1,      /* per-atom prepare / Born finalise / forces */
+    PDGPU_KERNEL_BORN = 2,      /* Born-radius pair sum                      */
+    PDGPU_KERNEL_PAIR = 3,      /* fused LJ + Coulomb + GB pair pass         */
+    PDGPU_KERNEL_CHAIN = 4,     /* chain rule through the Born radii         */
+    PDGPU_KERNEL_INTEGRATE = 5, /* integrator / minimiser updates            */
+    PDGPU_KERNEL_NGROUPS = 6
+};
+int pdgpu_profile(pdgpu_ctx *ctx, int32_t enable);
+int pdgpu_profile_read(pdgpu_ctx *ctx, double *ms, int64_t *launches, int32_t n);
+
 /* Host-side setup helpers (no device work).
  * BondOrder::reinit (workspace/bondorder.cpp:170-195) from undirected 1-2,
  * 1-3, 1-4 pair lists and residue numbers. Call with band == NULL to obtain

@@ -220,6 +220,7 @@ void orc_fixed_terms(int n, const double *gb_radius,
         double Ri = ORC_RADJ(gb_radius[i]);
         terms123[i] = -166.0 / (Ri + p->dielectric_offset + p->P1);
     }
+    if (n <= 0) return;
     double *t2 = (double *)calloc((size_t)n, sizeof(double));
     double *t3 = (double *)calloc((size_t)n, sizeof(double));
     for (int k = 0; k < n12; ++k) {

@@ -123,6 +123,12 @@ struct pdgpu_ctx {
     // host mirrors of per-block metadata
     std::vector<int> h_blk_inst, h_blk_n, h_blk_first;
 
+    // optional per-kernel timing with HIP events on the context stream
+    bool prof = false;
+    struct ProfRec { int kind; hipEvent_t a, b; };
+    std::vector<ProfRec> prof_recs;
+    std::vector<hipEvent_t> prof_pool;
+
     DevState S() const {
         DevState s;
         s.px = (const double *)px.p; s.py = (const double *)py.p; s.pz = (const double *)pz.p;
@@ -295,6 +301,39 @@ Span span_of(const pdgpu_ctx *c, int id) {
 
 int waves_cap(const pdgpu_ctx *c) { return std::max(64, c->num_cus * 16); }
 
+hipEvent_t prof_event(pdgpu_ctx *c) {
+    if (!c->prof_pool.empty()) {
+        hipEvent_t e = c->prof_pool.back();
+        c->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one kernel launch with events when profiling is on.
+struct ProfScope {
+    pdgpu_ctx *c;
+    int kind;
+    hipEvent_t a = nullptr;
+    ProfScope(pdgpu_ctx *c_, int k) : c(c_), kind(k) {
+        if (c->prof) {
+            a = prof_event(c);
+            if (a) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~ProfScope() {
+        if (c->prof && a) {
+            hipEvent_t b = prof_event(c);
+            if (b) {
+                (void)hipEventRecord(b, c->stream);
+                c->prof_recs.push_back({kind, a, b});
+            }
+        }
+    }
+};
+
 void pair_grid(const pdgpu_ctx *c, int64_t tiles, int *groups, int *tpw) {
     const int wcap = waves_cap(c);
     int t = c->opt.tiles_per_wave;
@@ -312,6 +351,7 @@ int launch_build(pdgpu_ctx *c, int id) {
     const DevState S = c->S();
     const Span sp = span_of(c, id);
     const int nb = sp.b1 - sp.b0;
+    ProfScope ps(c, PDGPU_KERNEL_NLIST);
     if (nb > 0) {
         hipLaunchKernelGGL(k_block_bounds, dim3((nb + 3) / 4), dim3(256), 0, c->stream, S, sp.b0, sp.b1);
         hipLaunchKernelGGL(k_tile_rows<0>, dim3((nb + 3) / 4), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1);
@@ -360,17 +400,34 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
     const int ns = sp.s1 - sp.s0;
     int groups, tpw;
     pair_grid(c, span_tiles(c, id), &groups, &tpw);
-    hipLaunchKernelGGL(k_inst_clear, dim3((ns + 255) / 256), dim3(256), 0, c->stream, S, sp.s0, sp.s1);
-    hipLaunchKernelGGL(k_prepare<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
-    hipLaunchKernelGGL(k_born<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
-    hipLaunchKernelGGL(k_born_final<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, c->P, a0, a1);
+    {
+        ProfScope ps(c, PDGPU_KERNEL_ATOM);
+        hipLaunchKernelGGL(k_inst_clear, dim3((ns + 255) / 256), dim3(256), 0, c->stream, S, sp.s0, sp.s1);
+        hipLaunchKernelGGL(k_prepare<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+    }
+    {
+        ProfScope ps(c, PDGPU_KERNEL_BORN);
+        hipLaunchKernelGGL(k_born<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+    }
+    {
+        ProfScope ps(c, PDGPU_KERNEL_ATOM);
+        hipLaunchKernelGGL(k_born_final<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, c->P, a0, a1);
+    }
     if (mode == 0) {
-        hipLaunchKernelGGL((k_pair<R, 0>), dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+        {
+            ProfScope ps(c, PDGPU_KERNEL_PAIR);
+            hipLaunchKernelGGL((k_pair<R, 0>), dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+        }
+        ProfScope ps(c, PDGPU_KERNEL_CHAIN);
         hipLaunchKernelGGL(k_chain<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
     } else {
+        ProfScope ps(c, PDGPU_KERNEL_PAIR);
         hipLaunchKernelGGL((k_pair<R, 1>), dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
     }
-    hipLaunchKernelGGL(k_finalize<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+    {
+        ProfScope ps(c, PDGPU_KERNEL_ATOM);
+        hipLaunchKernelGGL(k_finalize<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+    }
     HIPCHECK(c, hipGetLastError());
     return PDGPU_OK;
 }
@@ -540,6 +597,8 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->tiles, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse};
     for (DevBuf *b : all) release(*b);
+    for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -836,6 +895,32 @@ extern "C" int pdgpu_get_born_radii(pdgpu_ctx *c, int32_t id, double *out) {
     } else {
         HIPCHECK(c, hipMemcpyAsync(out, (double *)c->born.p + a0, I.natoms * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
+    return PDGPU_OK;
+}
+
+extern "C" int pdgpu_profile(pdgpu_ctx *c, int32_t enable) {
+    if (!c) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (auto &r : c->prof_recs) {
+        c->prof_pool.push_back(r.a);
+        c->prof_pool.push_back(r.b);
+    }
+    c->prof_recs.clear();
+    c->prof = enable != 0;
+    return PDGPU_OK;
+}
+
+extern "C" int pdgpu_profile_read(pdgpu_ctx *c, double *ms, int64_t *launches, int32_t n) {
+    if (!c || !ms || !launches || n <= 0) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n; ++k) { ms[k] = 0; launches[k] = 0; }
+    for (auto &r : c->prof_recs) {
+        float t = 0;
+        HIPCHECK(c, hipEventElapsedTime(&t, r.a, r.b));
+        if (r.kind < n) { ms[r.kind] += t; launches[r.kind] += 1; }
     }
     return PDGPU_OK;
 }

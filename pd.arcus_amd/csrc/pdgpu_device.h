@@ -285,15 +285,16 @@ __global__ void k_tile_rows(DevState S, PairParams P, int blk_begin, int blk_end
         return;
     }
     const int first = S.blk_first[I];
-    int base = (PASS == 1) ? S.row_off[I] : 0;
+    const int base = (PASS == 1) ? S.row_off[I] : 0;
+    const int reserved = (PASS == 1) ? S.row_count[I] : 0; // never write past the counted row
     int total = 0;
     for (int J0 = first; J0 <= I; J0 += 64) {
         const int J = J0 + l;
         const bool hit = (J <= I) && boxes_within(S, I, J, P.list_r);
         const unsigned long long m = __ballot(hit);
         if (PASS == 1 && hit) {
-            const int pos = base + total + (int)__popcll(m & ((1ull << l) - 1ull));
-            S.tiles[pos] = make_int2(I, J | (tile_flags(S, inst, I, J) << 24));
+            const int local = total + (int)__popcll(m & ((1ull << l) - 1ull));
+            if (local < reserved) S.tiles[base + local] = make_int2(I, J | (tile_flags(S, inst, I, J) << 24));
         }
         total += (int)__popcll(m);
     }

@@ -45,7 +45,9 @@ EXPORTS = (
     "pdgpu_calc_forces", "pdgpu_calc_energies", "pdgpu_get_born_radii", "pdgpu_get_forces",
     "pdgpu_minimise_sd", "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_md_vv",
     "pdgpu_bondorder_build", "pdgpu_fixed_terms", "pdgpu_instance_bytes", "pdgpu_bytes_free",
+    "pdgpu_profile", "pdgpu_profile_read",
 )
+KERNEL_GROUPS = ("nlist", "atom", "born", "pair", "chain", "integrate")
 
 
 class PdgpuError(RuntimeError):
@@ -113,6 +115,8 @@ def _load():
                                         P(Options), _dp, _dp]),
         "pdgpu_instance_bytes": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
         "pdgpu_bytes_free": (C.c_int64, [C.c_void_p]),
+        "pdgpu_profile": (C.c_int, [C.c_void_p, C.c_int32]),
+        "pdgpu_profile_read": (C.c_int, [C.c_void_p, _dp, P(C.c_int64), C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -332,6 +336,17 @@ class Context:
         ek = np.zeros(1)
         self._check(lib.pdgpu_md_vv(self.h, iid, steps, dt, update_nlist, C.byref(e), _p(ek, _dp)), "md_vv")
         return e.as_dict(), float(ek[0])
+
+    def profile(self, enable=True):
+        self._check(lib.pdgpu_profile(self.h, int(enable)), "profile")
+
+    def profile_read(self):
+        n = len(KERNEL_GROUPS)
+        ms = np.zeros(n)
+        cnt = np.zeros(n, np.int64)
+        self._check(lib.pdgpu_profile_read(self.h, _p(ms, _dp), cnt.ctypes.data_as(C.POINTER(C.c_int64)), n),
+                    "profile_read")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_GROUPS)}
 
     def bytes_free(self):
         return lib.pdgpu_bytes_free(self.h)
