@@ -171,6 +171,21 @@ int pdgpu_get_forces(pdgpu_ctx *ctx, int32_t id, double *forces);
 int pdgpu_minimise_sd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double step_size,
                       int32_t update_nlist, double grms_tol, int32_t *steps_done);
 
+/* The same minimiser, driven step by step for continuous batching: a
+ * server starts a minimisation on an instance as soon as it is added
+ * (pdgpu_sd_start), advances every started minimisation together
+ * (pdgpu_sd_advance), and collects each one as it finishes
+ * (pdgpu_sd_finish: cur = old, final energies and forces; the instance can
+ * then be read and removed). Each instance keeps its own step counter,
+ * multiplier and list-refresh schedule, so its result does not depend on
+ * which other instances share the launches. done_by_id / steps_by_id (may
+ * be NULL) are indexed by instance id; *running counts unfinished ones. */
+int pdgpu_sd_start(pdgpu_ctx *ctx, int32_t id, int32_t steps, double step_size, int32_t update_nlist,
+                   double grms_tol);
+int pdgpu_sd_advance(pdgpu_ctx *ctx, int32_t nsteps, int32_t *done_by_id, int32_t *steps_by_id,
+                     int32_t *running);
+int pdgpu_sd_finish(pdgpu_ctx *ctx, int32_t id, pdgpu_energies *energies, int32_t *steps_done);
+
 /* Velocity Verlet, textbook form (half kick, drift, force, half kick), on the
  * device. vel in/out in m/s (natoms*3, single id; NULL keeps device state),
  * dt in seconds. The neighbour list is rebuilt every update_nlist steps. */

@@ -87,6 +87,7 @@ struct Instance {
     int64_t band_off = 0, band_len = 0, sp_off = 0;
     bool listed = false;  // tile rows built
     bool has_vel = false;
+    bool sd = false;      // a minimisation was started and not finished
     std::vector<double> mass;
 };
 
@@ -117,7 +118,7 @@ struct pdgpu_ctx {
     // per block
     DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, row_count, row_off, tiles;
     // per instance
-    DevBuf inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
+    DevBuf inst_need, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf band, sparse;
 
     // host mirrors of per-block metadata
@@ -148,6 +149,7 @@ struct pdgpu_ctx {
         s.band = (const int8_t *)band.p; s.sparse = (const int4 *)sparse.p;
         s.inst_energy = inst_energy.p; s.inst_count = (unsigned long long *)inst_count.p;
         s.inst_status = (int *)inst_status.p;
+        s.inst_need = nullptr;
         return s;
     }
 };
@@ -346,10 +348,14 @@ void pair_grid(const pdgpu_ctx *c, int64_t tiles, int *groups, int *tpw) {
     *tpw = t;
 }
 
+// Rebuilds the tile rows of instance id (or all). With flagged, only the
+// instances whose device flag inst_need is set are rebuilt (the flags are
+// written by a kernel, so no host round trip is needed).
 template <typename R>
-int launch_build(pdgpu_ctx *c, int id) {
-    const DevState S = c->S();
-    const Span sp = span_of(c, id);
+int launch_build(pdgpu_ctx *c, int id, bool flagged = false) {
+    DevState S = c->S();
+    if (flagged) S.inst_need = (const int *)c->inst_need.p;
+    const Span sp = span_of(c, flagged ? PDGPU_ALL : id);
     const int nb = sp.b1 - sp.b0;
     ProfScope ps(c, PDGPU_KERNEL_NLIST);
     if (nb > 0) {
@@ -363,6 +369,7 @@ int launch_build(pdgpu_ctx *c, int id) {
     // no host round trip: kernels read the tile count from row_off; the
     // host keeps the upper bound for launch sizing
     c->ntiles = c->tri_sum;
+    if (flagged) return PDGPU_OK; // flagged builds only run on instances listed before
     if (id == PDGPU_ALL) {
         for (auto &I : c->inst)
             if (I.active) I.listed = true;
@@ -372,16 +379,24 @@ int launch_build(pdgpu_ctx *c, int id) {
     return PDGPU_OK;
 }
 
+// Builds the lists of instances that have none yet, leaving the lists of
+// the others as they are (a list is only rebuilt when its owner asks).
 int ensure_listed(pdgpu_ctx *c, int id) {
-    bool need = false;
-    if (id == PDGPU_ALL) {
-        for (auto &I : c->inst)
-            if (I.active && !I.listed) need = true;
-    } else {
-        need = !c->inst[id].listed;
+    if (id != PDGPU_ALL) {
+        if (c->inst[id].listed) return PDGPU_OK;
+        return c->mixed ? launch_build<float>(c, id) : launch_build<double>(c, id);
     }
-    if (!need) return PDGPU_OK;
-    return c->mixed ? launch_build<float>(c, id) : launch_build<double>(c, id);
+    std::vector<int> need(c->max_inst, 0);
+    int n = 0;
+    for (int s = 0; s < c->max_inst; ++s)
+        if (c->inst[s].active && !c->inst[s].listed) { need[s] = 1; ++n; }
+    if (n == 0) return PDGPU_OK;
+    HIPCHECK(c, hipMemcpyAsync(c->inst_need.p, need.data(), need.size() * 4, hipMemcpyHostToDevice, c->stream));
+    TRY(c->mixed ? launch_build<float>(c, PDGPU_ALL, true) : launch_build<double>(c, PDGPU_ALL, true));
+    HIPCHECK(c, hipStreamSynchronize(c->stream)); // the host flag vector goes out of scope
+    for (int s = 0; s < c->max_inst; ++s)
+        if (need[s]) c->inst[s].listed = true;
+    return PDGPU_OK;
 }
 
 int64_t span_tiles(const pdgpu_ctx *c, int id) {
@@ -561,6 +576,7 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     int rc = PDGPU_OK;
     const int ms = c->max_inst;
     if (rc == PDGPU_OK) rc = resize(c, c->inst_nat, ms, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_need, ms, 4);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_mid, ms, 4);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_band_off, ms, 8);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_sp_off, ms, 4);
@@ -594,7 +610,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->born, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy, &c->acc_fz, &c->fx, &c->fy, &c->fz,
                      &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
                      &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->row_count, &c->row_off,
-                     &c->tiles, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
+                     &c->tiles, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse};
     for (DevBuf *b : all) release(*b);
     for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -775,9 +791,14 @@ extern "C" int pdgpu_instance_remove(pdgpu_ctx *c, int32_t id) {
     c->spr.release(I.sp_off, I.nsparse);
     c->tri_sum -= (int64_t)I.nblocks * (I.nblocks + 1) / 2;
     I = Instance();
-    // the tile list of the remaining instances is rewritten at the next build
-    for (auto &J : c->inst)
-        if (J.active) J.listed = false;
+    // re-pack the remaining rows (their bounds are kept) so no tile of the
+    // removed instance is reachable
+    const DevState S = c->S();
+    const int all = (int)c->cap_blocks;
+    hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, c->stream, S, all);
+    hipLaunchKernelGGL(k_tile_rows<1>, dim3((all + 3) / 4), dim3(256), 0, c->stream, S, c->P, 0, all);
+    HIPCHECK(c, hipGetLastError());
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     return PDGPU_OK;
 }
 

@@ -99,6 +99,7 @@ struct DevState {
     const int *inst_sp_off, *inst_sp_n;
     const int8_t *band;
     const int4 *sparse;           // (i_local, j_local, order, 0), i<j
+    const int *inst_need;         // optional per-instance rebuild flags (null: all)
     void *inst_energy;            // [slot*kENum]
     unsigned long long *inst_count; // [slot*kCNum]
     int *inst_status;
@@ -224,6 +225,7 @@ __global__ void k_block_bounds(DevState S, int blk_begin, int blk_end) {
     const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     if (b >= blk_end) return;
     if (S.blk_inst[b] < 0) return;
+    if (S.inst_need && !S.inst_need[S.blk_inst[b]]) return;
     const int l = lane_id();
     const bool v = l < S.blk_n[b];
     const size_t a = (size_t)b * kBlk + l;
@@ -284,6 +286,7 @@ __global__ void k_tile_rows(DevState S, PairParams P, int blk_begin, int blk_end
         if (PASS == 0 && l == 0) S.row_count[I] = 0;
         return;
     }
+    if (PASS == 0 && S.inst_need && !S.inst_need[inst]) return; // keep this instance's rows
     const int first = S.blk_first[I];
     const int base = (PASS == 1) ? S.row_off[I] : 0;
     const int reserved = (PASS == 1) ? S.row_count[I] : 0; // never write past the counted row
@@ -386,6 +389,7 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
             const int inst = S.blk_inst[I], first = S.blk_first[I];
+            if (inst < 0) continue; // rows of a removed instance (uniform per wave)
             const int il = (I - first) * kBlk + l;
             const bool vi = l < S.blk_n[I];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
@@ -518,6 +522,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
             const int inst = S.blk_inst[I], first = S.blk_first[I];
+            if (inst < 0) continue; // rows of a removed instance (uniform per wave)
             const int il = (I - first) * kBlk + l;
             const bool vi = l < S.blk_n[I];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
@@ -725,6 +730,7 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
             const int inst = S.blk_inst[I], first = S.blk_first[I];
+            if (inst < 0) continue; // rows of a removed instance (uniform per wave)
             const int il = (I - first) * kBlk + l;
             const bool vi = l < S.blk_n[I];
             const bool vj_ok = l < S.blk_n[J];

@@ -1,6 +1,8 @@
 // pdgpu_integrate.h -- device side of the step drivers: PD's steepest
-// descent minimiser (protocols/minimise.cpp:181-235) and a velocity Verlet
-// integrator, both resident on the GPU for many instances at once.
+// descent minimiser (protocols/minimise.cpp:99-235) and a velocity Verlet
+// integrator, resident on the GPU for many instances at once. Each instance
+// carries its own step counter, so minimisations can start and finish at
+// different times while sharing launches (continuous batching).
 #pragma once
 #include "pdgpu_device.h"
 
@@ -8,20 +10,26 @@ namespace pdgpu {
 
 // Per-instance integrator state.
 struct InstMD {
-    double mult;     // SD step multiplier
-    double old_e;    // SD reference energy, J/molecule
-    double epot;     // last total energy, J/molecule
-    double f2;       // sum of |f|^2 over atoms, (kcal/mol/A)^2
-    double ekin;     // kinetic energy, J
-    int action;      // 0 none, 1 move (after saving), 2 restore
-    int done;        // instance finished (converged or failed)
-    int steps_done;  // steps run, -1 on failure
+    double mult;      // SD step multiplier (m_StepMultiplier)
+    double old_e;     // SD reference energy, J/molecule (m_OldEnergy)
+    double epot;      // last total energy, J/molecule
+    double f2;        // sum of |f|^2 over atoms, (kcal/mol/A)^2
+    double ekin;      // kinetic energy, J
+    double step_size; // StepSize (A per N)
+    double grms_tol;  // force-RMS stop, kcal/mol/A (0: off)
+    int action;       // see below
+    int running;      // minimisation started and not finished
+    int done;         // finished (converged, out of steps, or failed)
+    int steps_done;   // SD iterations run, -1 on failure
+    int step;         // next iteration number
+    int max_steps;    // Steps
+    int update_nlist; // UpdateNList
     int pad;
 };
 
 enum { kActNone = 0, kActMove = 1, kActRestore = 2, kActSave = 3 };
 
-// Total energy of an instance in J/molecule from the device record.
+// Total energy of an instance in kcal/mol from the device record.
 template <typename R>
 __device__ __forceinline__ double inst_epot_kcal(const DevState &S, int s) {
     double e = 0;
@@ -29,17 +37,32 @@ __device__ __forceinline__ double inst_epot_kcal(const DevState &S, int s) {
     return e;
 }
 
-__global__ void k_md_reset(InstMD *md, int s0, int s1) {
+__global__ void k_sd_start(InstMD *md, int s, int max_steps, double step_size, double grms_tol, int update_nlist) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    InstMD &m = md[s];
+    m.mult = 1.0;
+    m.old_e = 0;
+    m.epot = 0;
+    m.f2 = 0;
+    m.ekin = 0;
+    m.step_size = step_size;
+    m.grms_tol = grms_tol;
+    m.action = kActNone;
+    m.running = 1;
+    m.done = max_steps <= 0 ? 1 : 0;
+    m.steps_done = 0;
+    m.step = 0;
+    m.max_steps = max_steps;
+    m.update_nlist = update_nlist;
+}
+
+// Rebuild flags of running minimisations: PD refreshes the list when
+// step % UpdateNList == 0 (protocols/protocolbase.cpp:113-122).
+__global__ void k_sd_need(const InstMD *md, int *need, int s0, int s1) {
     const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= s1) return;
-    md[s].mult = 1.0;
-    md[s].old_e = 0;
-    md[s].epot = 0;
-    md[s].f2 = 0;
-    md[s].ekin = 0;
-    md[s].action = kActNone;
-    md[s].done = 0;
-    md[s].steps_done = 0;
+    const InstMD &m = md[s];
+    need[s] = (m.running && !m.done && m.update_nlist > 0 && (m.step % m.update_nlist) == 0) ? 1 : 0;
 }
 
 // sum of |f|^2 per instance (one wave per block)
@@ -53,55 +76,50 @@ __global__ void k_force_sq(DevState S, InstMD *md, int blk_begin, int blk_end) {
     double f2 = 0;
     if (l < S.blk_n[b]) f2 = S.fx[a] * S.fx[a] + S.fy[a] * S.fy[a] + S.fz[a] * S.fz[a];
     f2 = wave_sum(f2);
-    if (l == 0) unsafeAtomicAdd(&md[inst].f2, f2);
+    if (l == 0 && md[inst].running && !md[inst].done) unsafeAtomicAdd(&md[inst].f2, f2);
 }
 
-// PD's accept / reject rule, one thread per instance.
+// PD's accept / reject rule (protocols/minimise.cpp:181-235), one thread per
+// instance, on the energy just evaluated.
 template <typename R>
-__global__ void k_sd_decide(DevState S, InstMD *md, const int *natoms, int s0, int s1, int step, double jpk,
-                            double grms_tol) {
+__global__ void k_sd_decide(DevState S, InstMD *md, const int *natoms, int s0, int s1, double jpk) {
     const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= s1) return;
     InstMD &m = md[s];
     m.action = kActNone;
-    if (natoms[s] <= 0 || m.done) { m.f2 = 0; return; }
+    if (natoms[s] <= 0 || !m.running || m.done) { m.f2 = 0; return; }
     const double E = inst_epot_kcal<R>(S, s) * jpk;
     const double grms = sqrt(m.f2 / (3.0 * natoms[s]));
     m.f2 = 0;
     m.epot = E;
     if (S.inst_status[s] != 0 || !isfinite(E)) {
         m.done = 1;
-        m.steps_done = -1;
+        m.steps_done = -1; // Minimisation::runcore returns -1 (minimise.cpp:123-127)
         return;
     }
-    bool accept;
-    if (step == 0) {
-        accept = true;
-    } else if (E <= m.old_e) {
-        m.mult *= 1.2;
-        accept = true;
-    } else {
+    const int step = m.step++;
+    m.steps_done = m.step;
+    if (step != 0 && !(E <= m.old_e)) {
+        // uphill: restore the saved structure, shrink the step
         m.mult *= 0.5 / 1.2;
         m.old_e += 1; // PD adds 1 J/molecule: the next step is always accepted
         m.action = kActRestore;
-        m.steps_done = step + 1;
-        return;
-    }
-    if (accept) {
+    } else {
+        if (step != 0) m.mult *= 1.2;
         m.old_e = E;
-        m.steps_done = step + 1;
-        if (grms_tol > 0 && grms < grms_tol) {
-            m.done = 1;
+        if (m.grms_tol > 0 && grms < m.grms_tol) {
             m.action = kActSave; // converged: keep this structure
+            m.done = 1;
             return;
         }
         m.action = kActMove;
     }
+    if (m.step >= m.max_steps) m.done = 1;
 }
 
 // Per-atom SD update: save and move, restore, or save only (converged).
-__global__ void k_sd_update(DevState S, InstMD *md, double *opx, double *opy, double *opz, double *px, double *py,
-                            double *pz, int atom_begin, int atom_end, double step_size, double f_to_n) {
+__global__ void k_sd_update(DevState S, const InstMD *md, double *opx, double *opy, double *opz, double *px,
+                            double *py, double *pz, int atom_begin, int atom_end, double f_to_n) {
     const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= atom_end) return;
     const int b = a / kBlk;
@@ -111,7 +129,7 @@ __global__ void k_sd_update(DevState S, InstMD *md, double *opx, double *opy, do
     if (act == kActMove || act == kActSave) {
         opx[a] = px[a]; opy[a] = py[a]; opz[a] = pz[a];
         if (act == kActMove) {
-            const double s = f_to_n * (step_size * md[inst].mult);
+            const double s = f_to_n * (md[inst].step_size * md[inst].mult);
             px[a] += S.fx[a] * s;
             py[a] += S.fy[a] * s;
             pz[a] += S.fz[a] * s;
@@ -121,15 +139,16 @@ __global__ void k_sd_update(DevState S, InstMD *md, double *opx, double *opy, do
     }
 }
 
-// cur = old at the end of a minimisation (protocols/minimise.cpp:173)
-__global__ void k_copy_back(DevState S, const InstMD *md, const double *opx, const double *opy, const double *opz, double *px,
-                            double *py, double *pz, int atom_begin, int atom_end) {
+// cur = old at the end of a minimisation (protocols/minimise.cpp:173);
+// failed instances keep their positions, as the reference returns early.
+__global__ void k_copy_back(DevState S, const InstMD *md, const double *opx, const double *opy, const double *opz,
+                            double *px, double *py, double *pz, int atom_begin, int atom_end) {
     const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= atom_end) return;
     const int b = a / kBlk;
     const int inst = S.blk_inst[b];
     if (inst < 0 || (a % kBlk) >= S.blk_n[b]) return;
-    if (md && md[inst].steps_done < 0) return; // failed instances keep their positions
+    if (md[inst].steps_done < 0) return;
     px[a] = opx[a]; py[a] = opy[a]; pz[a] = opz[a];
 }
 
@@ -146,7 +165,7 @@ __global__ void k_vv(DevState S, InstMD *md, const double *minv, double *vx, dou
     double ek = 0;
     if (v) {
         const double h = 0.5 * dt * minv[a] * f_to_n;
-        double ux = vx[a] + S.fx[a] * h, uy = vy[a] + S.fy[a] * h, uz = vz[a] + S.fz[a] * h;
+        const double ux = vx[a] + S.fx[a] * h, uy = vy[a] + S.fy[a] * h, uz = vz[a] + S.fz[a] * h;
         vx[a] = ux; vy[a] = uy; vz[a] = uz;
         if (drift) {
             const double s = dt / 1e-10;

@@ -45,7 +45,7 @@ EXPORTS = (
     "pdgpu_calc_forces", "pdgpu_calc_energies", "pdgpu_get_born_radii", "pdgpu_get_forces",
     "pdgpu_minimise_sd", "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_md_vv",
     "pdgpu_bondorder_build", "pdgpu_fixed_terms", "pdgpu_instance_bytes", "pdgpu_bytes_free",
-    "pdgpu_profile", "pdgpu_profile_read",
+    "pdgpu_profile", "pdgpu_profile_read", "pdgpu_sd_start", "pdgpu_sd_advance", "pdgpu_sd_finish",
 )
 KERNEL_GROUPS = ("nlist", "atom", "born", "pair", "chain", "integrate")
 
@@ -116,6 +116,9 @@ def _load():
         "pdgpu_instance_bytes": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
         "pdgpu_bytes_free": (C.c_int64, [C.c_void_p]),
         "pdgpu_profile": (C.c_int, [C.c_void_p, C.c_int32]),
+        "pdgpu_sd_start": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double]),
+        "pdgpu_sd_advance": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip]),
+        "pdgpu_sd_finish": (C.c_int, [C.c_void_p, C.c_int32, P(Energies), _ip]),
         "pdgpu_profile_read": (C.c_int, [C.c_void_p, _dp, P(C.c_int64), C.c_int32]),
     }
     for name, (res, args) in sig.items():
@@ -330,6 +333,26 @@ class Context:
         if rc not in (0, PDGPU_ERR_NONFINITE):
             self._check(rc, "minimise_sd")
         return out if iid == PDGPU_ALL else int(out[0])
+
+    def sd_start(self, iid, steps, step_size=2e5, update_nlist=1, grms_tol=0.0):
+        self._check(lib.pdgpu_sd_start(self.h, iid, steps, step_size, update_nlist, grms_tol), "sd_start")
+
+    def sd_advance(self, nsteps):
+        n = self.opt.max_instances if self.opt.max_instances > 0 else 8192
+        done = np.zeros(n, np.int32)
+        steps = np.zeros(n, np.int32)
+        running = C.c_int32()
+        self._check(lib.pdgpu_sd_advance(self.h, nsteps, _p(done, _ip), _p(steps, _ip), C.byref(running)),
+                    "sd_advance")
+        return done, steps, running.value
+
+    def sd_finish(self, iid):
+        e = Energies()
+        st = C.c_int32()
+        rc = lib.pdgpu_sd_finish(self.h, iid, C.byref(e), C.byref(st))
+        if rc not in (0, PDGPU_ERR_NONFINITE):
+            self._check(rc, "sd_finish")
+        return e.as_dict(), st.value
 
     def md_vv(self, iid, steps, dt, update_nlist=10):
         e = Energies()
