@@ -36,7 +36,24 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libpdgpu.so")
     os.replace(tmp, LIB)
+    build_host_demo(verbose)
     return LIB
+
+
+def build_host_demo(verbose=False):
+    """C++ drop-in adapter (host/ff_gb_hip.cpp) and its driver, linked
+    against libpdgpu.so with an rpath to the in-tree lib directory."""
+    out = os.path.join(LIBDIR, "ff_gb_hip_demo")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "host", "ff_gb_hip.cpp"),
+           os.path.join(HERE, "host", "ff_gb_hip_demo.cpp"), "-L" + LIBDIR, "-lpdgpu",
+           "-Wl,-rpath,$ORIGIN", "-o", out]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building the C++ adapter")
+    return out
 
 
 if __name__ == "__main__":

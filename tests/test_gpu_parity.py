@@ -11,7 +11,7 @@ import pytest
 
 import golden_io as G
 import pdgpu
-from gpu_helpers import context_for, rel, rms_rel
+from gpu_helpers import J2KCAL, context_for, rel, rms_rel
 
 pytestmark = pytest.mark.gpu
 CASES = G.case_names()
@@ -133,7 +133,20 @@ def test_fp64_sd_minimiser(name):
         xyz = ctx.get_positions(i)
         assert np.max(np.abs(xyz - g["sd_xyz"])) <= 1e-7
         e, f = ctx.calc_forces(i)
-        assert rel(e["epot"], g["sd_epot"]) <= 1e-9
+        if abs(g["sd_epot"]) * J2KCAL < 1e5:
+            assert rel(e["epot"], g["sd_epot"]) <= 1e-9
+        else:
+            # PD's 200-step run collapses bonded atoms onto each other
+            # (GB-only forcefield, no bond terms): the energy there moves by
+            # percents for 1e-9 A, so the final evaluation is checked against
+            # the oracle at the positions the GPU reached
+            import oracle_lib as O
+            p = O.default_params(**g["param_dict"])
+            s = O.OSystem(g["charge"], g["radius"], g["epsilon"], g["gb_radius"], g["vj"], g["terms123"],
+                          g["mass"], g["mid"], g["band"], g["sp_i"], g["sp_j"], g["sp_o"])
+            rs, ent = ctx.nlist_export(i, shadow=True)
+            eo, fo, _, _ = O.gb_calc_forces(s, xyz, rs, ent, p)
+            assert rel(e["epot"], eo["epot"]) <= 1e-9
 
 
 def test_nonfinite_instance_is_isolated():
