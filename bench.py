@@ -96,6 +96,32 @@ def cpu_baseline_md(mol, xyz, params_kw, pairs_per_eval, budget_s=12.0):
                       f"{len(mol['charge'])}-atom md structure, {dt:.1f} s, oracle C port, 1 thread"}
 
 
+def cpu_baseline_batch(mols, pairs, budget_s=12.0):
+    """The oracle (scalar C port, 1 thread) on a bounded sample of the batch:
+    conformers in order, each a list build + Born + pair pass + chain rule,
+    until ~budget_s seconds have passed."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    p = O.default_params(cutoff=10.0, inner_cutoff=7.0, vdw_cutoff=10.0, vdw_inner_cutoff=7.0, vdw14_scaling=0.5,
+                         elec14_scaling=0.83333, dielectric_solvent=78.5, padding=2.0)
+    done = 0
+    npairs = 0
+    t0 = time.perf_counter()
+    for m, pc in zip(mols, pairs):
+        s = O.OSystem(m["charge"], m["radius"], m["epsilon"], m["gb_radius"], m["vj"], m["terms123"], m["mass"],
+                      m["mid"], m["band"], m["sp_i"], m["sp_j"], m["sp_o"])
+        rs, ent = O.nlist_build(s, m["xyz"], 10.0, 2.0, True)
+        O.gb_calc_forces(s, m["xyz"], rs, ent, p)
+        done += 1
+        npairs += pc
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": npairs / dt, "unit": "atom-pair evals/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} of {len(mols)} conformers, one evaluation each (list build + Born + pair "
+                      f"pass + chain rule), {dt:.1f} s, oracle C port, 1 thread"}
+
+
 def roofline(prof, pairs, born_pairs, precision):
     """Dominant kernel group and its achieved algorithmic FLOP rate."""
     flops = {"pair": FLOPS_PAIR * pairs, "born": FLOPS_BORN * born_pairs, "chain": FLOPS_CHAIN * born_pairs}
@@ -175,14 +201,18 @@ def run_batch(args, pdgpu):
                                              dielectric_solvent=78.5, nlist_padding=2.0,
                                              max_instances=args.conformers)
     log(f"building {args.conformers} conformers")
-    mols = [synth.peptide(seed=1000 + k, residues=int(L)) for k, L in enumerate(lengths)]
+    t0 = time.perf_counter()
+    mols = synth.many([1000 + k for k in range(args.conformers)], [int(L) for L in lengths])
+    log(f"built in {time.perf_counter() - t0:.1f} s")
     natoms = sum(m["n"] for m in mols)
     res = {}
     for name, prec in (("mixed", pdgpu.MIXED), ("fp64", pdgpu.FP64)):
         ctx = pdgpu.Context(opt(prec))
+        t0 = time.perf_counter()
         for m in mols:
             ctx.add_system(m)
         ctx.nlist_build()
+        log(f"{name}: {len(mols)} instances resident in {time.perf_counter() - t0:.1f} s")
         for _ in range(args.warmup):
             ctx.calc_forces(pdgpu.PDGPU_ALL)
         t0 = time.perf_counter()
@@ -201,6 +231,10 @@ def run_batch(args, pdgpu):
     dt, pairs, born, prof, r = res["mixed"]
     dt64, pairs64, born64, prof64, r64 = res["fp64"]
     errs = [abs(r[k]["epot"] - r64[k]["epot"]) / abs(r64[k]["epot"]) for k in r]
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle port)")
+        cpu = cpu_baseline_batch(mols, [r64[k]["pairs"] for k in sorted(r64)])
     rl = roofline(prof, pairs, born, "mixed")
     rl["traffic"] = load_traffic(rl["kernel"], "batch")
     return {
@@ -216,6 +250,7 @@ def run_batch(args, pdgpu):
         "mixed_over_fp64": (pairs * args.steps / dt) / (pairs64 * args.steps / dt64),
         "max_energy_rel_err_mixed_vs_fp64": max(errs),
         "roofline": rl,
+        **({"cpu_baseline": cpu} if cpu else {}),
     }
 
 

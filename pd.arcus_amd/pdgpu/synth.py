@@ -213,6 +213,8 @@ def build(seq, phi=None, psi=None, seed=0, mode="extended", min_dist=2.0, tries=
     clashes with a soft-core push-apart relaxation (relax_clashes): the
     stand-in for PD's soft-core clash-relief minimisation."""
     L = len(seq)
+    if mode == "extended" and phi is None:
+        return _build_extended(seq)
     rng = np.random.default_rng(seed)
     if phi is None:
         if mode == "extended":
@@ -287,6 +289,72 @@ def build(seq, phi=None, psi=None, seed=0, mode="extended", min_dist=2.0, tries=
     if mode != "extended" and relax_iters > 0:
         xyz = relax_clashes(xyz, bonds, min_dist, relax_iters)
     return _finish(xyz, el, np.array(q, dtype=float), np.array(res_of, dtype=np.int32), bonds)
+
+
+_EXT = {}
+
+
+def _extended_frames():
+    """Residue-frame step of the phi = psi = omega = 180 chain: the (N, CA, C)
+    frame of residue i+1 is a fixed rigid transform of residue i's."""
+    if "step" not in _EXT:
+        N0 = np.array([0.0, 0.0, 0.0])
+        CA0 = np.array([B_NCA, 0.0, 0.0])
+        C0 = _place(np.array([0.0, 1.0, 0.0]), N0, CA0, B_CAC, A_NCAC, 180.0)
+        N1 = _place(N0, CA0, C0, B_CN, A_CACN, 180.0)
+        CA1 = _place(CA0, C0, N1, B_NCA, A_CNCA, 180.0)
+        C1 = _place(C0, N1, CA1, B_CAC, A_NCAC, 180.0)
+        R0 = _frame(N0[None], CA0[None], C0[None])[0]
+        R1 = _frame(N1[None], CA1[None], C1[None])[0]
+        # world = R @ local + CA; residue 1 = step applied to residue 0
+        _EXT["R0"], _EXT["t0"] = R0, CA0
+        _EXT["dR"] = R1 @ R0.T
+        _EXT["dt"] = CA1 - _EXT["dR"] @ CA0
+    return _EXT
+
+
+def _ext_local(letter):
+    """All atoms of a residue type in its own frame for the extended chain."""
+    key = "local_" + letter
+    if key not in _EXT:
+        ex = _extended_frames()
+        t = template(letter)
+        N0 = np.array([0.0, 0.0, 0.0])
+        CA0 = np.array([B_NCA, 0.0, 0.0])
+        C0 = _place(np.array([0.0, 1.0, 0.0]), N0, CA0, B_CAC, A_NCAC, 180.0)
+        atoms = _residue_atoms(t, N0, CA0, C0, 180.0, 180.0, False)
+        _EXT[key] = (atoms - ex["t0"]) @ ex["R0"]
+    return _EXT[key]
+
+
+def _build_extended(seq):
+    ex = _extended_frames()
+    L = len(seq)
+    Rs = np.empty((L, 3, 3))
+    ts = np.empty((L, 3))
+    Rs[0], ts[0] = ex["R0"], ex["t0"]
+    for i in range(1, L):
+        Rs[i] = ex["dR"] @ Rs[i - 1]
+        ts[i] = ex["dR"] @ ts[i - 1] + ex["dt"]
+    xyz, el, q, bonds, res_of = [], [], [], [], []
+    prev_c = None
+    off = 0
+    for i, letter in enumerate(seq):
+        t = template(letter)
+        xyz.append(_ext_local(letter) @ Rs[i].T + ts[i])
+        c_i = off + 4 + t.nside
+        el += ["N", "HN", "C", "HC"] + t.elems + ["C", "O"]
+        ca_q = 0.03 - (0.06 if letter == "G" else 0.0)
+        q += [-0.42, 0.27, ca_q, 0.08] + list(t.charges) + [0.60, -0.56]
+        bonds += [(off, off + 1), (off, off + 2), (off + 2, off + 3), (off + 2, c_i), (c_i, c_i + 1)]
+        bonds += [(off + a, off + b) for a, b in t.bonds]
+        if prev_c is not None:
+            bonds.append((prev_c, off))
+        prev_c = c_i
+        res_of += [i] * t.natoms
+        off += t.natoms
+    return _finish(np.concatenate(xyz), el, np.array(q, dtype=float), np.array(res_of, dtype=np.int32),
+                   np.array(bonds, dtype=np.int64))
 
 
 def relax_clashes(xyz, bonds, min_dist=2.0, iters=100):
@@ -380,3 +448,21 @@ def peptide(seed, residues, mode="extended"):
     """One molecule of `residues` residues with a seeded random sequence."""
     seq = random_sequence(seed, residues)
     return build(seq, seed=seed, mode=mode)
+
+
+def _peptide_job(args):
+    seed, residues, mode = args
+    return peptide(seed, residues, mode)
+
+
+def many(seeds, residues, mode="extended", workers=None):
+    """Builds many molecules in parallel worker processes (call this before
+    any GPU context exists in the process: workers are forked)."""
+    import multiprocessing as mp
+    import os
+    jobs = list(zip(seeds, residues, [mode] * len(seeds)))
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(jobs) < 8:
+        return [_peptide_job(j) for j in jobs]
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_peptide_job, jobs, chunksize=max(1, len(jobs) // (4 * workers)))
