@@ -259,7 +259,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="md", choices=("md", "batch"))
+    ap.add_argument("--workload", default="batch", choices=("md", "batch"))
     ap.add_argument("--conformers", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
