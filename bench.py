@@ -124,8 +124,9 @@ def cpu_baseline_batch(mols, pairs, budget_s=12.0):
 
 def roofline(prof, pairs, born_pairs, precision):
     """Dominant kernel group and its achieved algorithmic FLOP rate."""
-    flops = {"pair": FLOPS_PAIR * pairs, "born": FLOPS_BORN * born_pairs, "chain": FLOPS_CHAIN * born_pairs}
-    name = max(("pair", "born", "chain"), key=lambda k: prof[k][0])
+    flops = {"pair": FLOPS_PAIR * pairs, "born": FLOPS_BORN * born_pairs, "chain": FLOPS_CHAIN * born_pairs,
+             "enum": 0}
+    name = max(("pair", "born", "chain", "enum"), key=lambda k: prof[k][0])
     ms, launches = prof[name]
     avg_s = ms / 1e3 / max(1, launches)
     achieved = flops[name] / avg_s / 1e12
@@ -231,6 +232,10 @@ def run_batch(args, pdgpu):
     dt, pairs, born, prof, r = res["mixed"]
     dt64, pairs64, born64, prof64, r64 = res["fp64"]
     errs = [abs(r[k]["epot"] - r64[k]["epot"]) / abs(r64[k]["epot"]) for k in r]
+    comp = lambda e: abs(e["epot_vdw"]) + abs(e["epot_elec"]) + abs(e["epot_pol_self"]) + abs(e["epot_pol_cross"])
+    errs_c = [abs(r[k]["epot"] - r64[k]["epot"]) / comp(r64[k]) for k in r]
+    worst = max(r, key=lambda k: errs[k] if False else abs(r[k]["epot"] - r64[k]["epot"]) / abs(r64[k]["epot"]))
+    tot_err = abs(sum(r[k]["epot"] for k in r) - sum(r64[k]["epot"] for k in r)) / abs(sum(r64[k]["epot"] for k in r))
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle port)")
@@ -249,6 +254,15 @@ def run_batch(args, pdgpu):
                  "roofline": roofline(prof64, pairs64, born64, "fp64")},
         "mixed_over_fp64": (pairs * args.steps / dt) / (pairs64 * args.steps / dt64),
         "max_energy_rel_err_mixed_vs_fp64": max(errs),
+        "median_energy_rel_err_mixed_vs_fp64": float(np.median(errs)),
+        "max_energy_err_rel_to_component_sum": max(errs_c),
+        "batch_total_energy_rel_err": tot_err,
+        "worst_instance": {"epot_fp64_kcal": r64[worst]["epot"] / pdgpu.J_PER_KCALMOL,
+                           "epot_mixed_kcal": r[worst]["epot"] / pdgpu.J_PER_KCALMOL,
+                           "components_fp64_kcal": {k: r64[worst][k] / pdgpu.J_PER_KCALMOL for k in
+                                                    ("epot_vdw", "epot_elec", "epot_pol_self", "epot_pol_cross")},
+                           "components_mixed_kcal": {k: r[worst][k] / pdgpu.J_PER_KCALMOL for k in
+                                                     ("epot_vdw", "epot_elec", "epot_pol_self", "epot_pol_cross")}},
         "roofline": rl,
         **({"cpu_baseline": cpu} if cpu else {}),
     }

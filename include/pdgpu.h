@@ -206,7 +206,8 @@ enum pdgpu_kernel_group {
     PDGPU_KERNEL_PAIR = 3,      /* fused LJ + Coulomb + GB pair pass         */
     PDGPU_KERNEL_CHAIN = 4,     /* chain rule through the Born radii         */
     PDGPU_KERNEL_INTEGRATE = 5, /* integrator / minimiser updates            */
-    PDGPU_KERNEL_NGROUPS = 6
+    PDGPU_KERNEL_ENUM = 6,      /* per-evaluation pair enumeration           */
+    PDGPU_KERNEL_NGROUPS = 7
 };
 int pdgpu_profile(pdgpu_ctx *ctx, int32_t enable);
 int pdgpu_profile_read(pdgpu_ctx *ctx, double *ms, int64_t *launches, int32_t n);

@@ -120,6 +120,8 @@ struct pdgpu_ctx {
     // per instance
     DevBuf inst_need, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf band, sparse;
+    DevBuf plist, tile_cnt, overflow;
+    int64_t plist_cap = 0; // tiles the pair-list buffer holds
 
     // host mirrors of per-block metadata
     std::vector<int> h_blk_inst, h_blk_n, h_blk_first;
@@ -150,6 +152,10 @@ struct pdgpu_ctx {
         s.inst_energy = inst_energy.p; s.inst_count = (unsigned long long *)inst_count.p;
         s.inst_status = (int *)inst_status.p;
         s.inst_need = nullptr;
+        s.plist = (uint16_t *)plist.p;
+        s.tile_cnt = (int2 *)tile_cnt.p;
+        s.plist_cap = (int)std::min<int64_t>(plist_cap, INT32_MAX);
+        s.overflow = (int *)overflow.p;
         return s;
     }
 };
@@ -348,6 +354,21 @@ void pair_grid(const pdgpu_ctx *c, int64_t tiles, int *groups, int *tpw) {
     *tpw = t;
 }
 
+// Grows the per-tile pair-list buffer (kTileCap 2-byte entries per tile);
+// its contents are rewritten by every evaluation, so nothing is copied.
+int ensure_plist(pdgpu_ctx *c, int64_t tiles) {
+    if (tiles <= c->plist_cap) return PDGPU_OK;
+    const int64_t cap = std::max<int64_t>(tiles, c->plist_cap * 5 / 4 + 64);
+    c->used -= (int64_t)(c->plist.bytes + c->tile_cnt.bytes);
+    release(c->plist);
+    release(c->tile_cnt);
+    c->plist_cap = 0;
+    TRY(resize(c, c->plist, (size_t)cap * kTileCap, 2));
+    TRY(resize(c, c->tile_cnt, (size_t)cap, 8));
+    c->plist_cap = cap;
+    return PDGPU_OK;
+}
+
 // Rebuilds the tile rows of instance id (or all). With flagged, only the
 // instances whose device flag inst_need is set are rebuilt (the flags are
 // written by a kernel, so no host round trip is needed).
@@ -366,9 +387,12 @@ int launch_build(pdgpu_ctx *c, int id, bool flagged = false) {
     hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, c->stream, S, all);
     hipLaunchKernelGGL(k_tile_rows<1>, dim3((all + 3) / 4), dim3(256), 0, c->stream, S, c->P, 0, all);
     HIPCHECK(c, hipGetLastError());
-    // no host round trip: kernels read the tile count from row_off; the
-    // host keeps the upper bound for launch sizing
-    c->ntiles = c->tri_sum;
+    // the pair-list buffer needs one slot per tile: read the new total
+    int total = 0;
+    HIPCHECK(c, hipMemcpyAsync(&total, (int *)c->row_off.p + all, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->ntiles = total;
+    TRY(ensure_plist(c, total));
     if (flagged) return PDGPU_OK; // flagged builds only run on instances listed before
     if (id == PDGPU_ALL) {
         for (auto &I : c->inst)
@@ -419,6 +443,10 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
         hipLaunchKernelGGL(k_inst_clear, dim3((ns + 255) / 256), dim3(256), 0, c->stream, S, sp.s0, sp.s1);
         hipLaunchKernelGGL(k_prepare<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+    }
+    {
+        ProfScope ps(c, PDGPU_KERNEL_ENUM);
+        hipLaunchKernelGGL(k_enum<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
     }
     {
         ProfScope ps(c, PDGPU_KERNEL_BORN);
@@ -589,6 +617,7 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     if (rc == PDGPU_OK) rc = resize(c, c->band, 4096, 1);
     if (rc == PDGPU_OK) rc = resize(c, c->sparse, 64, 16);
     if (rc == PDGPU_OK) rc = resize(c, c->tiles, 4096, 8);
+    if (rc == PDGPU_OK) rc = resize(c, c->overflow, 1, 4);
     if (rc != PDGPU_OK) {
         pdgpu_ctx_destroy(c);
         return rc;
@@ -611,7 +640,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
                      &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->row_count, &c->row_off,
                      &c->tiles, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
-                     &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse};
+                     &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow};
     for (DevBuf *b : all) release(*b);
     for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : c->prof_pool) (void)hipEventDestroy(e);

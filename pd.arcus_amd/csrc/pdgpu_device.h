@@ -100,6 +100,10 @@ struct DevState {
     const int8_t *band;
     const int4 *sparse;           // (i_local, j_local, order, 0), i<j
     const int *inst_need;         // optional per-instance rebuild flags (null: all)
+    uint16_t *plist;              // per-tile compacted pair lists, kTileCap entries each
+    int2 *tile_cnt;               // (born-side count, other count) per tile
+    int plist_cap;                // tiles the list buffer holds
+    int *overflow;                // set when a tile index exceeds plist_cap
     void *inst_energy;            // [slot*kENum]
     unsigned long long *inst_count; // [slot*kCNum]
     int *inst_status;
@@ -371,73 +375,153 @@ template <typename R> __device__ __forceinline__ R ld_born(const DevState &S, si
     return reinterpret_cast<const R *>(S.born)[a];
 }
 
-// ------------------------------------------------------------ Born radii A4
-// Gpol_i += P4*V_j*CCF/r^4 over pairs with bond order > 2 and r <= Still
-// cutoff (forcefields/gbff.cpp:432-465), both directions per unordered pair.
-// Every accumulator is flushed at the end of each tile, so the sums an atom
-// receives do not depend on how tiles are spread over wavefronts.
+// ------------------------------------------------------ pair enumeration
+// Each evaluation first walks every tile once: lane = i-atom, the 64 j-atoms
+// rotate through the lanes (DPP wave_ror:1), and only the distance and
+// bond-order tests are done. Pairs inside the cutoffs are compacted, with
+// ballot + mbcnt, into the tile's slot of a 2-byte-per-pair list:
+//   entry = i_lane | j_lane << 6 | bond_order << 12
+// Born-radius / chain-rule pairs (bond order > 2, r <= Still cutoff) are
+// packed from the front of the slot, the remaining pair-pass pairs from the
+// back. The compute kernels then work through these lists 64 pairs at a
+// time with every lane busy, instead of paying the full pair arithmetic for
+// every slot of the 64x64 tile.
+constexpr int kTileCap = 4096;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ unsigned long long lanes_below() {
+    const int l = lane_id();
+    return l ? (~0ull >> (64 - l)) : 0ull;
+}
+
 template <typename R>
-__global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
+__global__ void __launch_bounds__(256) k_enum(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
     const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
     const int l = lane_id();
-    const R still2 = (R)P.still2, invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5;
+    const R cut2 = (R)P.cut2, still2 = (R)P.still2;
+    const unsigned long long below = lanes_below();
     for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
         const int cend = min(t_end, c + tpw);
         for (int t = c; t < cend; ++t) {
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
             const int inst = S.blk_inst[I], first = S.blk_first[I];
-            if (inst < 0) continue; // rows of a removed instance (uniform per wave)
+            if (inst < 0 || t >= S.plist_cap) {
+                if (l == 0 && t < S.plist_cap) S.tile_cnt[t] = make_int2(0, 0);
+                if (l == 0 && t >= S.plist_cap) atomicOr(S.overflow, 1);
+                continue;
+            }
             const int il = (I - first) * kBlk + l;
             const bool vi = l < S.blk_n[I];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
-            R ri, vji, rj, vjj;
-            if constexpr (Prec<R>::fixed) {
-                const float2 gi = S.gb32[a], gj = S.gb32[b];
-                ri = gi.x; vji = gi.y; rj = gj.x; vjj = gj.y;
-            } else {
-                const double2 gi = S.gb64[a], gj = S.gb64[b];
-                ri = gi.x; vji = gi.y; rj = gj.x; vjj = gj.y;
-            }
-            const R xi = pi.x, yi = pi.y, zi = pi.z;
             R xj = pj.x, yj = pj.y, zj = pj.z;
             int jl = (l < S.blk_n[J]) ? (J - first) * kBlk + l : -1;
-            R gi_acc = 0, gj = 0;
-            unsigned int npairs = 0;
+            int jlane = l;
             const bool diag = fl & kTileDiag;
             const bool lookup = fl & (kTileNear | kTileSparse);
+            uint16_t *L = S.plist + (size_t)t * kTileCap;
+            int nb = 0, no = 0;
             for (int k = 0; k < kBlk; ++k) {
-                if (k > 0) {
-                    xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj); vjj = rot(vjj); jl = rot(jl); gj = rot(gj);
-                }
-                bool ok = vi && jl >= 0 && (!diag || jl < il);
-                if (ok && lookup) ok = pair_bondorder(S, inst, il, jl, fl) > 2;
-                if (!ok) continue;
-                const R dx = xi - xj, dy = yi - yj, dz = zi - zj;
+                if (k > 0) { xj = rot(xj); yj = rot(yj); zj = rot(zj); jl = rot(jl); jlane = rot(jlane); }
+                const R dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
                 const R r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > still2) continue;
-                const R sR = ri + rj;
-                const R x = r2 * Prec<R>::rcp(sR * sR);
-                R ccf = (R)1;
-                if (x <= invP5) {
-                    const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(x * P5);
-                    ccf = h * h;
-                }
-                const R term = P4 * ccf * Prec<R>::rcp(r2 * r2);
-                gi_acc += term * vjj;
-                gj += term * vji;
-                npairs += 2;
+                const bool ok = vi && jl >= 0 && (!diag || jl < il) && (r2 <= cut2 || r2 <= still2);
+                int bo = 7;
+                if (ok && lookup) bo = pair_bondorder(S, inst, il, jl, fl);
+                const bool born = ok && bo > 2 && r2 <= still2;
+                const bool oth = ok && !born && r2 <= cut2;
+                const unsigned long long mb = __ballot(born), mo = __ballot(oth);
+                const uint16_t e = (uint16_t)(l | (jlane << 6) | (bo << 12));
+                if (born) L[nb + (int)__popcll(mb & below)] = e;
+                if (oth) L[kTileCap - 1 - (no + (int)__popcll(mo & below))] = e;
+                nb += (int)__popcll(mb);
+                no += (int)__popcll(mo);
             }
-            gj = rot(gj); // 64th rotation: back on the lane that owns atom j
+            if (l == 0) S.tile_cnt[t] = make_int2(nb, no);
+        }
+    }
+}
+
+// Position of the q-th entry of a tile's list: born-side entries from the
+// front, the others from the back.
+__device__ __forceinline__ int entry_pos(int q, int nb) { return q < nb ? q : kTileCap - 1 - (q - nb); }
+
+// ------------------------------------------------------------ Born radii A4
+// Gpol_i += P4*V_j*CCF/r^4 over pairs with bond order > 2 and r <= Still
+// cutoff (forcefields/gbff.cpp:432-465), both directions per unordered pair.
+// Every accumulator is flushed at the end of each tile, so the sums an atom
+// receives do not depend on how tiles are spread over wavefronts.
+template <typename R> struct BornAtom { R x, y, z, rad, vj; };
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
+    __shared__ BornAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
+    __shared__ R gi_acc[kWavesPerGroup][kBlk], gj_acc[kWavesPerGroup][kBlk];
+    const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
+    const int nw = gridDim.x * kWavesPerGroup;
+    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    const int l = lane_id(), w = threadIdx.x / 64;
+    const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5;
+    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
+        const int cend = min(t_end, c + tpw);
+        for (int t = c; t < cend; ++t) {
+            const int2 tw = S.tiles[t];
+            const int I = tw.x, J = tw.y & 0xFFFFFF;
+            const int inst = S.blk_inst[I];
+            if (inst < 0) continue;
+            const int nb = S.tile_cnt[t].x;
+            if (nb == 0) continue;
+            const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
+            const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+            const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+            if constexpr (Prec<R>::fixed) {
+                const float2 gi = S.gb32[a], gj = S.gb32[b];
+                si[w][l] = {pi.x, pi.y, pi.z, gi.x, gi.y};
+                sj[w][l] = {pj.x, pj.y, pj.z, gj.x, gj.y};
+            } else {
+                const double2 gi = S.gb64[a], gj = S.gb64[b];
+                si[w][l] = {pi.x, pi.y, pi.z, gi.x, gi.y};
+                sj[w][l] = {pj.x, pj.y, pj.z, gj.x, gj.y};
+            }
+            gi_acc[w][l] = 0;
+            gj_acc[w][l] = 0;
+            wave_lds_sync();
+            const uint16_t *L = S.plist + (size_t)t * kTileCap;
+            for (int q0 = 0; q0 < nb; q0 += kBlk) {
+                const int q = q0 + l;
+                if (q < nb) {
+                    const int e = L[q];
+                    const int ii = e & 63, jj = (e >> 6) & 63;
+                    const BornAtom<R> A = si[w][ii], B = sj[w][jj];
+                    const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    const R sR = A.rad + B.rad;
+                    const R x = r2 * Prec<R>::rcp(sR * sR);
+                    R ccf = (R)1;
+                    if (x <= invP5) {
+                        const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(x * P5);
+                        ccf = h * h;
+                    }
+                    const R term = P4 * ccf * Prec<R>::rcp(r2 * r2);
+                    atomicAdd(&gi_acc[w][ii], term * B.vj);
+                    atomicAdd(&gj_acc[w][jj], term * A.vj);
+                }
+            }
+            wave_lds_sync();
             int *st = S.inst_status + inst;
-            if (vi) acc_add<R>(S.acc_g, a, gi_acc, kGpolScale, st);
-            if (l < S.blk_n[J]) acc_add<R>(S.acc_g, b, gj, kGpolScale, st);
-            const unsigned int tot = wave_sum(npairs);
-            if (l == 0 && tot) atomicAdd(S.inst_count + (size_t)inst * kCNum + kCBornPairs, (unsigned long long)tot);
+            if (l < S.blk_n[I]) acc_add<R>(S.acc_g, a, gi_acc[w][l], kGpolScale, st);
+            if (l < S.blk_n[J]) acc_add<R>(S.acc_g, b, gj_acc[w][l], kGpolScale, st);
+            if (l == 0) atomicAdd(S.inst_count + (size_t)inst * kCNum + kCBornPairs, (unsigned long long)(2 * nb));
+            wave_lds_sync();
         }
     }
 }
@@ -502,6 +586,145 @@ __device__ __forceinline__ void switch_fn(R r, R inner, R invSw, R &S, R &dS) {
     S = s * s;
 }
 
+template <typename R> struct PairAtom { R x, y, z, q, rad, eps, a; };
+
+// One pair of the fused pass. Returns (dE/dr)/r (multiplies xi - xj) and
+// adds dE/d(a) contributions to dai / daj (MODE 0) and energies to E.
+template <typename R, int MODE>
+__device__ __forceinline__ R pair_term(const PairParams &P, const PairAtom<R> &A, const PairAtom<R> &B, int bo,
+                                       R r2, LaneEnergy &E, R &dai, R &daj) {
+    const R inner = (R)P.inner, invSw = (R)P.invSw;
+    const R vcut = (R)P.vcut, vinner = (R)P.vinner, vinvSw = (R)P.vinvSw;
+    const R premul = (R)P.premul;
+    const R rinv = Prec<R>::rsqrt(r2);
+    const R r = r2 * rinv;
+    const R qiqj = A.q * B.q;
+    R g = 0;
+    E.pairs++;
+    if (MODE == 0) {
+        const R aiaj = A.a * B.a;
+        const R u = (R)(-0.25) * r2 * Prec<R>::rcp(aiaj);
+        const R s14e = (bo == 3) ? (R)P.s14e : (R)1;
+        R elec_pot = 0, elec_force = 0;
+        if (u > (R)P.exp_thresh) {
+            const R tg = Prec<R>::expo(u);
+            const R fgb = Prec<R>::rsqrt(r2 + tg * aiaj);
+            const R L = fgb * fgb;
+            R epol = premul * qiqj * fgb;
+            R dedr = -epol * (r - (R)0.25 * r * tg) * L;
+            R dedal = -epol * tg * ((R)1 - u) * L;
+            if (bo >= 3) {
+                elec_pot = (R)P.econv_in * s14e * qiqj * rinv;
+                elec_force = -elec_pot * rinv;
+            }
+            if (r >= inner) {
+                R Sw, dS;
+                switch_fn<R>(r, inner, invSw, Sw, dS);
+                elec_force = Sw * elec_force + elec_pot * dS;
+                elec_pot *= Sw;
+                dedr = Sw * dedr + dS * epol;
+                dedal *= Sw;
+                epol *= Sw;
+            }
+            g = dedr * (R)2 * rinv;
+            E.pol += 2.0 * (double)epol;
+            dai = dedal * B.a;
+            daj = dedal * A.a;
+        } else if (bo >= 3) {
+            // far pair: GB dropped, solvent-dielectric Coulomb instead
+            elec_pot = (R)P.econv_out * s14e * qiqj * rinv;
+            elec_force = -elec_pot * rinv;
+            if (r >= inner) {
+                R Sw, dS;
+                switch_fn<R>(r, inner, invSw, Sw, dS);
+                elec_force = Sw * elec_force + elec_pot * dS;
+                elec_pot *= Sw;
+            }
+        }
+        if (bo >= 3) {
+            R vdw_force = 0;
+            if (r < vcut) {
+                const R s14v = (bo == 3) ? (R)P.s14v : (R)1;
+                const R rr = (A.rad + B.rad) * rinv;
+                const R rr2 = rr * rr;
+                const R Bp = rr2 * rr2 * rr2;
+                const R Ap = Bp * Bp;
+                const R eps = s14v * A.eps * B.eps;
+                R vdw_pot = ((R)2 * eps) * ((R)0.5 * Ap - Bp);
+                vdw_force = -((R)12 * eps * rinv) * (Ap - Bp);
+                if (r > vinner) {
+                    R vS, vd;
+                    switch_fn<R>(r, vinner, vinvSw, vS, vd);
+                    vdw_force = vS * vdw_force + vdw_pot * vd;
+                    vdw_pot *= vS;
+                }
+                E.vdw += (double)vdw_pot;
+            }
+            E.elec += (double)elec_pot;
+            g += (vdw_force + elec_force) * rinv;
+        }
+    } else {
+        // FF_NonBonded energy path, bond-order scale tables {0,0,0,s14,1,1,1,1}
+        const R sv = (bo <= 2) ? (R)0 : (bo == 3 ? (R)P.s14v : (R)1);
+        const R se = (bo <= 2) ? (R)0 : (bo == 3 ? (R)P.s14e : (R)1);
+        R vdw_pot = 0, vdw_force = 0, elec_pot = 0, elec_force = 0;
+        if (P.vdw_mode > 0 && r < vcut) {
+            const R rr = (A.rad + B.rad) * rinv;
+            const R rr2 = rr * rr;
+            const R Bp = rr2 * rr2 * rr2;
+            const R Ap = Bp * Bp;
+            const R eps = sv * A.eps * B.eps;
+            vdw_pot = ((R)2 * eps) * ((R)0.5 * Ap - Bp);
+            vdw_force = -((R)12 * eps * rinv) * (Ap - Bp);
+            if (P.vdw_mode == 2 && r > vinner) {
+                R vS, vd;
+                switch_fn<R>(r, vinner, vinvSw, vS, vd);
+                vdw_force = vS * vdw_force + vdw_pot * vd;
+                vdw_pot *= vS;
+            }
+        }
+        const R cut2 = (R)P.cut2;
+        if (P.elec_mode == 1 || P.elec_mode == 3) {
+            elec_pot = (R)P.econv_in * se * qiqj * rinv;
+            elec_force = -elec_pot * rinv;
+            if (P.elec_mode == 3 && r > inner) {
+                R Sw, dS;
+                switch_fn<R>(r, inner, invSw, Sw, dS);
+                elec_force = Sw * elec_force + elec_pot * dS;
+                elec_pot *= Sw;
+            }
+        } else if (P.elec_mode == 4) {
+            elec_pot = (R)P.econv_in * se * qiqj;
+            if (r > inner) {
+                const R c2r = cut2 - r2;
+                elec_force = -elec_pot * (R)P.fs_sA * (rinv * rinv) * (c2r * c2r) *
+                             (cut2 - (R)3 * (R)P.inner2 + (R)2 * r2);
+                elec_pot *= -((R)P.fs_sA * r * ((R)P.fs_sB * (rinv * rinv) + (R)P.fs_sC + (R)P.fs_sD * r2 +
+                                                (R)P.fs_sE * (r2 * r2)) -
+                              (R)P.fs_cutV);
+            } else {
+                elec_force = -elec_pot * (rinv * rinv);
+                elec_pot *= (rinv - (R)P.fs_eshift);
+            }
+        }
+        E.vdw += (double)vdw_pot;
+        E.elec += (double)elec_pot;
+        const R fm = vdw_force + elec_force;
+        E.vir += (double)(r * fm);
+        g = fm * rinv;
+        // exact pairwise GB energy, no threshold branch
+        if (r <= (R)P.cut) {
+            R Sw = (R)1, dS;
+            if (r >= inner) switch_fn<R>(r, inner, invSw, Sw, dS);
+            const R s2 = r * r;
+            const R aiaj = A.a * B.a;
+            const R fgb = Prec<R>::rsqrt(s2 + aiaj * Prec<R>::expo(-s2 * (R)0.25 * Prec<R>::rcp(aiaj)));
+            E.pol += (double)(premul * (R)2 * Sw * qiqj * fgb);
+        }
+    }
+    return g;
+}
+
 // --------------------------------------------------- fused pair pass A5
 // MODE 0: FastMode force pass (forcefields/gbff.cpp:1529-1729).
 // MODE 1: energy path -- FF_NonBonded with the context's switching modes
@@ -509,22 +732,23 @@ __device__ __forceinline__ void switch_fn(R r, R inner, R invSw, R &S, R &dS) {
 //         energy of calcBornEnergy (forcefields/gbff.cpp:520-579).
 template <typename R, int MODE>
 __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
-    const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
+    __shared__ PairAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
+    __shared__ R fi[kWavesPerGroup][4][kBlk], fj[kWavesPerGroup][4][kBlk];
+    const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
-    const int l = lane_id();
-    const R cut2 = (R)P.cut2, inner = (R)P.inner, invSw = (R)P.invSw;
-    const R vcut = (R)P.vcut, vinner = (R)P.vinner, vinvSw = (R)P.vinvSw;
-    const R premul = (R)P.premul, thresh = (R)P.exp_thresh;
+    const int l = lane_id(), w = threadIdx.x / 64;
+    const R cut2 = (R)P.cut2;
     for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
         const int cend = min(t_end, c + tpw);
         for (int t = c; t < cend; ++t) {
             const int2 tw = S.tiles[t];
-            const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
-            const int inst = S.blk_inst[I], first = S.blk_first[I];
-            if (inst < 0) continue; // rows of a removed instance (uniform per wave)
-            const int il = (I - first) * kBlk + l;
-            const bool vi = l < S.blk_n[I];
+            const int I = tw.x, J = tw.y & 0xFFFFFF;
+            const int inst = S.blk_inst[I];
+            if (inst < 0) continue;
+            const int2 cnt = S.tile_cnt[t];
+            const int ntot = cnt.x + cnt.y;
+            if (ntot == 0) continue;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -536,173 +760,53 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                 const double2 u = S.lj64[a], v = S.lj64[b];
                 Ri = u.x; ei = u.y; Rj = v.x; ej = v.y;
             }
-            const R xi = pi.x, yi = pi.y, zi = pi.z, qi = pi.q;
-            const R ai = ld_born<R>(S, a);
-            R xj = pj.x, yj = pj.y, zj = pj.z, qj = pj.q;
-            R aj = ld_born<R>(S, b);
-            int jl = (l < S.blk_n[J]) ? (J - first) * kBlk + l : -1;
-            R fxi = 0, fyi = 0, fzi = 0, dai = 0;
-            R fxj = 0, fyj = 0, fzj = 0, daj = 0;
+            si[w][l] = {pi.x, pi.y, pi.z, pi.q, Ri, ei, ld_born<R>(S, a)};
+            sj[w][l] = {pj.x, pj.y, pj.z, pj.q, Rj, ej, ld_born<R>(S, b)};
+            for (int k = 0; k < 4; ++k) { fi[w][k][l] = 0; fj[w][k][l] = 0; }
+            wave_lds_sync();
+            const uint16_t *L = S.plist + (size_t)t * kTileCap;
             LaneEnergy E;
-            const bool diag = fl & kTileDiag;
-            const bool lookup = fl & (kTileNear | kTileSparse);
-            for (int k = 0; k < kBlk; ++k) {
-                if (k > 0) {
-                    xj = rot(xj); yj = rot(yj); zj = rot(zj); qj = rot(qj); Rj = rot(Rj); ej = rot(ej); aj = rot(aj);
-                    jl = rot(jl);
-                    fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
-                    if (MODE == 0) daj = rot(daj);
-                }
-                if (!(vi && jl >= 0 && (!diag || jl < il))) continue;
-                const R dx = xi - xj, dy = yi - yj, dz = zi - zj;
-                const R r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > cut2) continue;
-                const int bo = lookup ? pair_bondorder(S, inst, il, jl, fl) : 7;
-                const R rinv = Prec<R>::rsqrt(r2);
-                const R r = r2 * rinv;
-                const R qiqj = qi * qj;
-                R g = 0; // (dE/dr)/r, multiplies (xi - xj)
-                E.pairs++;
-                if (MODE == 0) {
-                    const R aiaj = ai * aj;
-                    const R u = (R)(-0.25) * r2 * Prec<R>::rcp(aiaj);
-                    const R s14e = (bo == 3) ? (R)P.s14e : (R)1;
-                    R elec_pot = 0, elec_force = 0;
-                    if (u > thresh) {
-                        const R tg = Prec<R>::expo(u);
-                        const R fgb = Prec<R>::rsqrt(r2 + tg * aiaj);
-                        const R L = fgb * fgb;
-                        R epol = premul * qiqj * fgb;
-                        R dedr = -epol * (r - (R)0.25 * r * tg) * L;
-                        R dedal = -epol * tg * ((R)1 - u) * L;
-                        if (bo >= 3) {
-                            elec_pot = (R)P.econv_in * s14e * qiqj * rinv;
-                            elec_force = -elec_pot * rinv;
+            for (int q0 = 0; q0 < ntot; q0 += kBlk) {
+                const int q = q0 + l;
+                if (q < ntot) {
+                    const int e = L[entry_pos(q, cnt.x)];
+                    const int ii = e & 63, jj = (e >> 6) & 63, bo = (e >> 12) & 7;
+                    const PairAtom<R> A = si[w][ii], B = sj[w][jj];
+                    const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 <= cut2) { // born-side entries may reach past Cutoff when GbsaStillCutoff is larger
+                        R dai = 0, daj = 0;
+                        const R g = pair_term<R, MODE>(P, A, B, bo, r2, E, dai, daj);
+                        const R gx = g * dx, gy = g * dy, gz = g * dz;
+                        atomicAdd(&fi[w][0][ii], gx);
+                        atomicAdd(&fi[w][1][ii], gy);
+                        atomicAdd(&fi[w][2][ii], gz);
+                        atomicAdd(&fj[w][0][jj], -gx);
+                        atomicAdd(&fj[w][1][jj], -gy);
+                        atomicAdd(&fj[w][2][jj], -gz);
+                        if (MODE == 0) {
+                            atomicAdd(&fi[w][3][ii], dai);
+                            atomicAdd(&fj[w][3][jj], daj);
                         }
-                        if (r >= inner) {
-                            R Sw, dS;
-                            switch_fn<R>(r, inner, invSw, Sw, dS);
-                            elec_force = Sw * elec_force + elec_pot * dS;
-                            elec_pot *= Sw;
-                            dedr = Sw * dedr + dS * epol;
-                            dedal *= Sw;
-                            epol *= Sw;
-                        }
-                        g = dedr * (R)2 * rinv;
-                        E.pol += 2.0 * (double)epol;
-                        dai += dedal * aj;
-                        daj += dedal * ai;
-                    } else if (bo >= 3) {
-                        // far pair: GB dropped, solvent-dielectric Coulomb instead
-                        elec_pot = (R)P.econv_out * s14e * qiqj * rinv;
-                        elec_force = -elec_pot * rinv;
-                        if (r >= inner) {
-                            R Sw, dS;
-                            switch_fn<R>(r, inner, invSw, Sw, dS);
-                            elec_force = Sw * elec_force + elec_pot * dS;
-                            elec_pot *= Sw;
-                        }
-                    }
-                    if (bo >= 3) {
-                        R vdw_force = 0;
-                        if (r < vcut) {
-                            const R s14v = (bo == 3) ? (R)P.s14v : (R)1;
-                            const R rr = (Ri + Rj) * rinv;
-                            const R rr2 = rr * rr;
-                            const R B = rr2 * rr2 * rr2;
-                            const R A = B * B;
-                            const R eps = s14v * ei * ej;
-                            R vdw_pot = ((R)2 * eps) * ((R)0.5 * A - B);
-                            vdw_force = -((R)12 * eps * rinv) * (A - B);
-                            if (r > vinner) {
-                                R vS, vd;
-                                switch_fn<R>(r, vinner, vinvSw, vS, vd);
-                                vdw_force = vS * vdw_force + vdw_pot * vd;
-                                vdw_pot *= vS;
-                            }
-                            E.vdw += (double)vdw_pot;
-                        }
-                        E.elec += (double)elec_pot;
-                        g += (vdw_force + elec_force) * rinv;
-                    }
-                } else {
-                    // FF_NonBonded energy path, bond-order scale tables {0,0,0,s14,1,1,1,1}
-                    const R sv = (bo <= 2) ? (R)0 : (bo == 3 ? (R)P.s14v : (R)1);
-                    const R se = (bo <= 2) ? (R)0 : (bo == 3 ? (R)P.s14e : (R)1);
-                    R vdw_pot = 0, vdw_force = 0, elec_pot = 0, elec_force = 0;
-                    if (P.vdw_mode > 0 && r < vcut) {
-                        const R rr = (Ri + Rj) * rinv;
-                        const R rr2 = rr * rr;
-                        const R B = rr2 * rr2 * rr2;
-                        const R A = B * B;
-                        const R eps = sv * ei * ej;
-                        vdw_pot = ((R)2 * eps) * ((R)0.5 * A - B);
-                        vdw_force = -((R)12 * eps * rinv) * (A - B);
-                        if (P.vdw_mode == 2 && r > vinner) {
-                            R vS, vd;
-                            switch_fn<R>(r, vinner, vinvSw, vS, vd);
-                            vdw_force = vS * vdw_force + vdw_pot * vd;
-                            vdw_pot *= vS;
-                        }
-                    }
-                    if (P.elec_mode == 1 || P.elec_mode == 3) {
-                        elec_pot = (R)P.econv_in * se * qiqj * rinv;
-                        elec_force = -elec_pot * rinv;
-                        if (P.elec_mode == 3 && r > inner) {
-                            R Sw, dS;
-                            switch_fn<R>(r, inner, invSw, Sw, dS);
-                            elec_force = Sw * elec_force + elec_pot * dS;
-                            elec_pot *= Sw;
-                        }
-                    } else if (P.elec_mode == 4) {
-                        elec_pot = (R)P.econv_in * se * qiqj;
-                        if (r > inner) {
-                            const R c2r = cut2 - r2;
-                            elec_force = -elec_pot * (R)P.fs_sA * (rinv * rinv) * (c2r * c2r) *
-                                         (cut2 - (R)3 * (R)P.inner2 + (R)2 * r2);
-                            elec_pot *= -((R)P.fs_sA * r * ((R)P.fs_sB * (rinv * rinv) + (R)P.fs_sC +
-                                                            (R)P.fs_sD * r2 + (R)P.fs_sE * (r2 * r2)) -
-                                          (R)P.fs_cutV);
-                        } else {
-                            elec_force = -elec_pot * (rinv * rinv);
-                            elec_pot *= (rinv - (R)P.fs_eshift);
-                        }
-                    }
-                    E.vdw += (double)vdw_pot;
-                    E.elec += (double)elec_pot;
-                    const R fm = vdw_force + elec_force;
-                    E.vir += (double)(r * fm);
-                    g = fm * rinv;
-                    // exact pairwise GB energy, no threshold branch
-                    if (r <= (R)P.cut) {
-                        R Sw = (R)1, dS;
-                        if (r >= inner) switch_fn<R>(r, inner, invSw, Sw, dS);
-                        const R s2 = r * r;
-                        const R aiaj = ai * aj;
-                        const R fgb = Prec<R>::rsqrt(s2 + aiaj * Prec<R>::expo(-s2 * (R)0.25 * Prec<R>::rcp(aiaj)));
-                        E.pol += (double)(premul * (R)2 * Sw * qiqj * fgb);
                     }
                 }
-                const R gx = g * dx, gy = g * dy, gz = g * dz;
-                fxi += gx; fyi += gy; fzi += gz;
-                fxj -= gx; fyj -= gy; fzj -= gz;
             }
-            fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
-            if (MODE == 0) daj = rot(daj);
+            wave_lds_sync();
             int *st = S.inst_status + inst;
-            if (vi) {
-                acc_add<R>(S.acc_fx, a, fxi, kForceScale, st);
-                acc_add<R>(S.acc_fy, a, fyi, kForceScale, st);
-                acc_add<R>(S.acc_fz, a, fzi, kForceScale, st);
-                if (MODE == 0) acc_add<R>(S.acc_deda, a, dai, kForceScale, st);
+            if (l < S.blk_n[I]) {
+                acc_add<R>(S.acc_fx, a, fi[w][0][l], kForceScale, st);
+                acc_add<R>(S.acc_fy, a, fi[w][1][l], kForceScale, st);
+                acc_add<R>(S.acc_fz, a, fi[w][2][l], kForceScale, st);
+                if (MODE == 0) acc_add<R>(S.acc_deda, a, fi[w][3][l], kForceScale, st);
             }
             if (l < S.blk_n[J]) {
-                acc_add<R>(S.acc_fx, b, fxj, kForceScale, st);
-                acc_add<R>(S.acc_fy, b, fyj, kForceScale, st);
-                acc_add<R>(S.acc_fz, b, fzj, kForceScale, st);
-                if (MODE == 0) acc_add<R>(S.acc_deda, b, daj, kForceScale, st);
+                acc_add<R>(S.acc_fx, b, fj[w][0][l], kForceScale, st);
+                acc_add<R>(S.acc_fy, b, fj[w][1][l], kForceScale, st);
+                acc_add<R>(S.acc_fz, b, fj[w][2][l], kForceScale, st);
+                if (MODE == 0) acc_add<R>(S.acc_deda, b, fj[w][3][l], kForceScale, st);
             }
             flush_energy<R>(S, inst, E);
+            wave_lds_sync();
         }
     }
 }
@@ -717,23 +821,27 @@ __device__ __forceinline__ R chain_weight(const DevState &S, size_t a) {
     return (R)(ai * ai / 166.0 * acc_read<R>(S.acc_deda, a, kForceScale));
 }
 
+template <typename R> struct ChainAtom { R x, y, z, rad, vj, wt; };
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
-    const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
+    __shared__ ChainAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
+    __shared__ R fi[kWavesPerGroup][3][kBlk], fj[kWavesPerGroup][3][kBlk];
+    const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
-    const int l = lane_id();
-    const R still2 = (R)P.still2, invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5;
+    const int l = lane_id(), w = threadIdx.x / 64;
+    const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5;
     for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
         const int cend = min(t_end, c + tpw);
         for (int t = c; t < cend; ++t) {
             const int2 tw = S.tiles[t];
-            const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
-            const int inst = S.blk_inst[I], first = S.blk_first[I];
-            if (inst < 0) continue; // rows of a removed instance (uniform per wave)
-            const int il = (I - first) * kBlk + l;
-            const bool vi = l < S.blk_n[I];
-            const bool vj_ok = l < S.blk_n[J];
+            const int I = tw.x, J = tw.y & 0xFFFFFF;
+            const int inst = S.blk_inst[I];
+            if (inst < 0) continue;
+            const int nb = S.tile_cnt[t].x;
+            if (nb == 0) continue;
+            const bool vi = l < S.blk_n[I], vj = l < S.blk_n[J];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -745,52 +853,52 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
                 const double2 gi = S.gb64[a], gj = S.gb64[b];
                 ri = gi.x; vji = gi.y; rj = gj.x; vjj = gj.y;
             }
-            const R xi = pi.x, yi = pi.y, zi = pi.z;
-            const R wi = vi ? chain_weight<R>(S, a) : (R)0;
-            R xj = pj.x, yj = pj.y, zj = pj.z;
-            R wj = vj_ok ? chain_weight<R>(S, b) : (R)0;
-            int jl = vj_ok ? (J - first) * kBlk + l : -1;
-            R fxi = 0, fyi = 0, fzi = 0, fxj = 0, fyj = 0, fzj = 0;
-            const bool diag = fl & kTileDiag;
-            const bool lookup = fl & (kTileNear | kTileSparse);
-            for (int k = 0; k < kBlk; ++k) {
-                if (k > 0) {
-                    xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj); vjj = rot(vjj); wj = rot(wj); jl = rot(jl);
-                    fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
+            si[w][l] = {pi.x, pi.y, pi.z, ri, vji, vi ? chain_weight<R>(S, a) : (R)0};
+            sj[w][l] = {pj.x, pj.y, pj.z, rj, vjj, vj ? chain_weight<R>(S, b) : (R)0};
+            for (int k = 0; k < 3; ++k) { fi[w][k][l] = 0; fj[w][k][l] = 0; }
+            wave_lds_sync();
+            const uint16_t *L = S.plist + (size_t)t * kTileCap;
+            for (int q0 = 0; q0 < nb; q0 += kBlk) {
+                const int q = q0 + l;
+                if (q < nb) {
+                    const int e = L[q];
+                    const int ii = e & 63, jj = (e >> 6) & 63;
+                    const ChainAtom<R> A = si[w][ii], B = sj[w][jj];
+                    const R dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    const R sR = A.rad + B.rad;
+                    const R x = r2 * Prec<R>::rcp(sR * sR);
+                    R ccf = (R)1, dccf = (R)0;
+                    if (x <= invP5) {
+                        const R f = x * P5;
+                        const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(f);
+                        ccf = h * h;
+                        dccf = (R)2 * h * Prec<R>::sinpi_(f) * (f * (R)3.14159265358979323846);
+                    }
+                    const R r6 = r2 * r2 * r2;
+                    const R coef = P4 * ((R)4 * ccf - dccf) * Prec<R>::rcp(r6) * (A.wt * B.vj + B.wt * A.vj);
+                    const R gx = coef * dx, gy = coef * dy, gz = coef * dz;
+                    atomicAdd(&fi[w][0][ii], gx);
+                    atomicAdd(&fi[w][1][ii], gy);
+                    atomicAdd(&fi[w][2][ii], gz);
+                    atomicAdd(&fj[w][0][jj], -gx);
+                    atomicAdd(&fj[w][1][jj], -gy);
+                    atomicAdd(&fj[w][2][jj], -gz);
                 }
-                bool ok = vi && jl >= 0 && (!diag || jl < il);
-                if (ok && lookup) ok = pair_bondorder(S, inst, il, jl, fl) > 2;
-                if (!ok) continue;
-                const R dx = xj - xi, dy = yj - yi, dz = zj - zi;
-                const R r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 > still2) continue;
-                const R sR = ri + rj;
-                const R x = r2 * Prec<R>::rcp(sR * sR);
-                R ccf = (R)1, dccf = (R)0;
-                if (x <= invP5) {
-                    const R f = x * P5;
-                    const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(f);
-                    ccf = h * h;
-                    dccf = (R)2 * h * Prec<R>::sinpi_(f) * (f * (R)3.14159265358979323846);
-                }
-                const R r6 = r2 * r2 * r2;
-                const R coef = P4 * ((R)4 * ccf - dccf) * Prec<R>::rcp(r6) * (wi * vjj + wj * vji);
-                const R gx = coef * dx, gy = coef * dy, gz = coef * dz;
-                fxi += gx; fyi += gy; fzi += gz;
-                fxj -= gx; fyj -= gy; fzj -= gz;
             }
-            fxj = rot(fxj); fyj = rot(fyj); fzj = rot(fzj);
+            wave_lds_sync();
             int *st = S.inst_status + inst;
             if (vi) {
-                acc_add<R>(S.acc_fx, a, fxi, kForceScale, st);
-                acc_add<R>(S.acc_fy, a, fyi, kForceScale, st);
-                acc_add<R>(S.acc_fz, a, fzi, kForceScale, st);
+                acc_add<R>(S.acc_fx, a, fi[w][0][l], kForceScale, st);
+                acc_add<R>(S.acc_fy, a, fi[w][1][l], kForceScale, st);
+                acc_add<R>(S.acc_fz, a, fi[w][2][l], kForceScale, st);
             }
-            if (vj_ok) {
-                acc_add<R>(S.acc_fx, b, fxj, kForceScale, st);
-                acc_add<R>(S.acc_fy, b, fyj, kForceScale, st);
-                acc_add<R>(S.acc_fz, b, fzj, kForceScale, st);
+            if (vj) {
+                acc_add<R>(S.acc_fx, b, fj[w][0][l], kForceScale, st);
+                acc_add<R>(S.acc_fy, b, fj[w][1][l], kForceScale, st);
+                acc_add<R>(S.acc_fz, b, fj[w][2][l], kForceScale, st);
             }
+            wave_lds_sync();
         }
     }
 }

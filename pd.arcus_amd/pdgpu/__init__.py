@@ -47,7 +47,7 @@ EXPORTS = (
     "pdgpu_bondorder_build", "pdgpu_fixed_terms", "pdgpu_instance_bytes", "pdgpu_bytes_free",
     "pdgpu_profile", "pdgpu_profile_read", "pdgpu_sd_start", "pdgpu_sd_advance", "pdgpu_sd_finish",
 )
-KERNEL_GROUPS = ("nlist", "atom", "born", "pair", "chain", "integrate")
+KERNEL_GROUPS = ("nlist", "atom", "born", "pair", "chain", "integrate", "enum")
 
 
 class PdgpuError(RuntimeError):
