@@ -215,15 +215,16 @@ def run_batch(args, pdgpu):
         ctx.nlist_build()
         log(f"{name}: {len(mols)} instances resident in {time.perf_counter() - t0:.1f} s")
         for _ in range(args.warmup):
-            ctx.calc_forces(pdgpu.PDGPU_ALL)
+            ctx.calc_forces(pdgpu.PDGPU_ALL, raw=True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            r = ctx.calc_forces(pdgpu.PDGPU_ALL)
+            ctx.calc_forces(pdgpu.PDGPU_ALL, raw=True)
         dt = time.perf_counter() - t0
         ctx.profile(True)
         for _ in range(args.steps):
-            ctx.calc_forces(pdgpu.PDGPU_ALL)
+            ctx.calc_forces(pdgpu.PDGPU_ALL, raw=True)
         prof = ctx.profile_read()
+        r = ctx.calc_forces(pdgpu.PDGPU_ALL)
         ctx.close()
         pairs = sum(e["pairs"] for e in r.values())
         born = sum(e["born_pairs"] for e in r.values())

@@ -588,6 +588,164 @@ void orc_gb_calc_energies(const orc_system *s, const double *xyz, const orc_nlis
     free(a);
 }
 
+/* ---------------- FF_Bonded: forcefields/ffbonded.cpp:809-1590 ---------------- */
+typedef struct { double x, y, z; } v3;
+static inline v3 v3_sub(const double *a, const double *b) { v3 r = {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; return r; }
+static inline v3 v3_cross(v3 a, v3 b) {
+    v3 r = {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    return r;
+}
+static inline double v3_dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 v3_scale(v3 a, double s) { v3 r = {a.x * s, a.y * s, a.z * s}; return r; }
+static inline v3 v3_axpy(v3 a, double s, v3 b) { v3 r = {a.x * s - b.x, a.y * s - b.y, a.z * s - b.z}; return r; }
+static inline void f_add(double *f, int i, v3 v, double s) {
+    if (!f) return;
+    f[3 * i] += s * v.x; f[3 * i + 1] += s * v.y; f[3 * i + 2] += s * v.z;
+}
+
+/* calcBondForces (:885-940): E = k(d-l)^2, force 2k(d-l)/Angstrom along the
+ * bond, virial d * force * Angstrom. */
+static void bonded_bonds(const orc_bonded *b, const double *xyz, double *f, orc_energies *e) {
+    for (int t = 0; t < b->nbonds; ++t) {
+        const int i = b->bond_ij[2 * t], j = b->bond_ij[2 * t + 1];
+        const double l = b->bond_lk[2 * t], k = b->bond_lk[2 * t + 1];
+        const v3 v = v3_sub(xyz + 3 * i, xyz + 3 * j);
+        const double d = sqrt(v3_dot(v, v));
+        const double fm = 2.0 * k * (d - l) / kAngstrom;
+        e->virial += d * fm * kAngstrom;
+        e->epot_bond += k * sqr(d - l);
+        f_add(f, i, v, -fm / d);
+        f_add(f, j, v, fm / d);
+    }
+}
+
+/* calcAngleForces (:994-1120): theta from the unit bond vectors at the apex,
+ * E = k(theta-theta0)^2; terms with k == 0 are skipped. */
+static void bonded_angles(const orc_bonded *b, const double *xyz, double *f, orc_energies *e) {
+    for (int t = 0; t < b->nangles; ++t) {
+        const double k = b->angle_tk[2 * t + 1];
+        if (k == 0) continue;
+        const int i = b->angle_iaj[3 * t], a = b->angle_iaj[3 * t + 1], j = b->angle_iaj[3 * t + 2];
+        const v3 vi = v3_sub(xyz + 3 * i, xyz + 3 * a), vj = v3_sub(xyz + 3 * j, xyz + 3 * a);
+        const double ri = 1.0 / sqrt(v3_dot(vi, vi)), rj = 1.0 / sqrt(v3_dot(vj, vj));
+        const v3 ui = v3_scale(vi, ri), uj = v3_scale(vj, rj);
+        const double c = v3_dot(ui, uj);
+        const double theta = acos(c), sn = sin(theta);
+        const double dth = theta - b->angle_tk[2 * t];
+        e->epot_angle += k * dth * dth;
+        const double fm = 2.0 * k * dth / kAngstrom;
+        /* dtheta/dx_i = (u_i cos - u_j) / (|v_i| sin); the apex takes the rest */
+        const v3 gi = v3_scale(v3_axpy(ui, c, uj), fm * ri / sn);
+        const v3 gj = v3_scale(v3_axpy(uj, c, ui), fm * rj / sn);
+        f_add(f, i, gi, -1.0);
+        f_add(f, a, gi, 1.0);
+        f_add(f, j, gj, -1.0);
+        f_add(f, a, gj, 1.0);
+    }
+}
+
+/* calcDihedralForces (:1199-1440). With v1 = i-a, v2 = a-b, v3 = b-j,
+ * m = v1 x v2, n = v2 x v3, o = v2 x m: cos phi from m.n, sin from o.n,
+ * phi = -atan2(sin, cos) (the IUPAC dihedral). The gradient goes through
+ * d(cos)/d(m,n) when |sin phi| > 0.1 and through d(sin)/d(o,n) otherwise,
+ * as the reference does. Returns the energy of the dihedral. */
+static double bonded_dihedral(const double *xyz, const int32_t *q, int terms, const double *vn,
+                              const double *nn, const double *gam, double *f) {
+    const int i = q[0], a = q[1], bb = q[2], j = q[3];
+    const v3 v1 = v3_sub(xyz + 3 * i, xyz + 3 * a);
+    const v3 v2 = v3_sub(xyz + 3 * a, xyz + 3 * bb);
+    const v3 v3_ = v3_sub(xyz + 3 * bb, xyz + 3 * j);
+    v3 m = v3_cross(v1, v2), n = v3_cross(v2, v3_), o = v3_cross(v2, m);
+    const double im = 1.0 / sqrt(v3_dot(m, m)), in = 1.0 / sqrt(v3_dot(n, n)), io = 1.0 / sqrt(v3_dot(o, o));
+    const double cph = v3_dot(m, n) * im * in;
+    const double sph = v3_dot(o, n) * io * in;
+    n = v3_scale(n, in);
+    const double phi = -atan2(sph, cph);
+    const int use_cos = fabs(sph) > 0.1;
+    v3 g1, g2; /* d(cos)/dm, d(cos)/dn   or   d(sin)/do, d(sin)/dn */
+    if (use_cos) {
+        m = v3_scale(m, im);
+        g1 = v3_scale(v3_axpy(m, cph, n), im);
+        g2 = v3_scale(v3_axpy(n, cph, m), in);
+    } else {
+        o = v3_scale(o, io);
+        g1 = v3_scale(v3_axpy(o, sph, n), io);
+        g2 = v3_scale(v3_axpy(n, sph, o), in);
+    }
+    double etot = 0;
+    v3 fi = {0, 0, 0}, fab = {0, 0, 0}, fj = {0, 0, 0};
+    for (int t = 0; t < terms; ++t) {
+        double en, dedphi;
+        if (nn[t] > 0) {
+            en = vn[t] * (1.0 + cos(nn[t] * phi + gam[t]));
+            dedphi = -nn[t] * vn[t] * sin(nn[t] * phi + gam[t]);
+        } else {
+            double d = phi - gam[t];
+            if (d < -kPI) d += 2.0 * kPI;
+            else if (d > kPI) d -= 2.0 * kPI;
+            en = vn[t] * d * d;
+            dedphi = 2.0 * vn[t] * d;
+        }
+        etot += en;
+        if (use_cos) {
+            const double s = dedphi / sph;
+            /* m = v1 x v2 and n = v2 x v3 carried back to the atoms */
+            const v3 ci = v3_cross(v2, g1);
+            const v3 cj = v3_cross(g2, v2);
+            const v3 cab1 = v3_cross(g1, v1), cab2 = v3_cross(v3_, g2);
+            fi.x += s * ci.x; fi.y += s * ci.y; fi.z += s * ci.z;
+            fj.x += s * cj.x; fj.y += s * cj.y; fj.z += s * cj.z;
+            fab.x += s * (cab1.x + cab2.x); fab.y += s * (cab1.y + cab2.y); fab.z += s * (cab1.z + cab2.z);
+        } else {
+            const double s = -dedphi / cph;
+            /* o = v2 x (v1 x v2): d(o)/d(v1) = |v2|^2 I - v2 v2^T, and the
+             * v2-dependence of o, m and n for the middle atoms */
+            const double b2 = v3_dot(v2, v2);
+            const double gv2 = v3_dot(g1, v2);
+            const v3 ti = {b2 * g1.x - v2.x * gv2, b2 * g1.y - v2.y * gv2, b2 * g1.z - v2.z * gv2};
+            const v3 cj = v3_cross(g2, v2);
+            /* d(o)/d(v2) acting on g1: 2 v2 (v1.g1) ... written per component */
+            const double v1g = v3_dot(v1, g1), v1v2 = v3_dot(v1, v2);
+            const v3 tab = {
+                -v1v2 * g1.x + 2.0 * v2.x * v1g - v1.x * gv2,
+                -v1v2 * g1.y + 2.0 * v2.y * v1g - v1.y * gv2,
+                -v1v2 * g1.z + 2.0 * v2.z * v1g - v1.z * gv2};
+            const v3 cab2 = v3_cross(v3_, g2);
+            fi.x += s * ti.x; fi.y += s * ti.y; fi.z += s * ti.z;
+            fj.x += s * cj.x; fj.y += s * cj.y; fj.z += s * cj.z;
+            fab.x += s * (tab.x + cab2.x); fab.y += s * (tab.y + cab2.y); fab.z += s * (tab.z + cab2.z);
+        }
+    }
+    if (f) {
+        const double u = 1.0 / kAngstrom;
+        f[3 * i] += fi.x * u; f[3 * i + 1] += fi.y * u; f[3 * i + 2] += fi.z * u;
+        f[3 * a] += (fab.x - fi.x) * u; f[3 * a + 1] += (fab.y - fi.y) * u; f[3 * a + 2] += (fab.z - fi.z) * u;
+        f[3 * bb] += (fj.x - fab.x) * u; f[3 * bb + 1] += (fj.y - fab.y) * u; f[3 * bb + 2] += (fj.z - fab.z) * u;
+        f[3 * j] -= fj.x * u; f[3 * j + 1] -= fj.y * u; f[3 * j + 2] -= fj.z * u;
+    }
+    return etot;
+}
+
+void orc_bonded_forces(const orc_bonded *b, const double *xyz, double *force, orc_energies *e) {
+    orc_energies z;
+    memset(&z, 0, sizeof(z));
+    bonded_bonds(b, xyz, force, &z);
+    bonded_angles(b, xyz, force, &z);
+    for (int t = 0; t < b->ntorsions; ++t)
+        z.epot_torsion += bonded_dihedral(xyz, b->tors_iabj + 4 * t, b->tors_terms[t], b->tors_vn + 4 * t,
+                                          b->tors_n + 4 * t, b->tors_gamma + 4 * t, force);
+    for (int t = 0; t < b->nimpropers; ++t)
+        z.epot_improper += bonded_dihedral(xyz, b->impr_iabj + 4 * t, b->impr_terms[t], b->impr_vn + 4 * t,
+                                           b->impr_n + 4 * t, b->impr_gamma + 4 * t, force);
+    /* calcImproperForces adds impropers into the Hamiltonian's epot_torsion (:1556) */
+    e->epot_bond += z.epot_bond;
+    e->epot_angle += z.epot_angle;
+    e->epot_torsion += z.epot_torsion + z.epot_improper;
+    e->epot_improper += z.epot_improper;
+    e->virial += z.virial;
+    e->epot += z.epot_bond + z.epot_angle + z.epot_torsion + z.epot_improper;
+}
+
 /* ---------------- A11: protocols/minimise.cpp:99-235 ---------------- */
 static int build_list(const orc_system *s, const double *xyz, const orc_params *p, orc_nlist *nl) {
     const double lc = p->cutoff > p->vdw_cutoff ? p->cutoff : p->vdw_cutoff;
@@ -605,6 +763,9 @@ static void eval_forces(const orc_system *s, const double *xyz, const orc_nlist 
                         const orc_params *p, double *f, orc_energies *e) {
     memset(e, 0, sizeof(*e));
     memset(f, 0, sizeof(double) * 3 * (size_t)s->n);
+    /* Forcefield::calcForces runs its components in the order they were
+     * added (forcefields/forcefield.cpp:220-227): FF_Bonded, then GB */
+    if (s->bonded) orc_bonded_forces(s->bonded, xyz, f, e);
     orc_gb_calc_forces(s, xyz, nl, p, f, e, NULL, NULL);
 }
 

@@ -42,8 +42,30 @@ typedef struct orc_params {
 /* Components of PD's Hamiltonian touched by the path (workspace/hamiltonian.h). */
 typedef struct orc_energies {
     double epot, epot_vdw, epot_elec, epot_pol, epot_pol_self, epot_pol_cross;
-    double virial; /* InternalVirial (energy path only) */
+    double virial; /* InternalVirial */
+    double epot_bond, epot_angle;
+    double epot_torsion;  /* torsions + impropers, as PD's Hamiltonian holds them */
+    double epot_improper; /* FF_Bonded::getEImproper                              */
 } orc_energies;
+
+/* FF_Bonded's assembled term lists (forcefields/ffbonded.h:22-72), atom
+ * indices local to the system. Bond: E = k(d-l)^2, k in J/molecule/A^2.
+ * Angle: E = k(theta-theta0)^2, k in J/molecule/rad^2. Dihedral i-a-b-j,
+ * up to 4 terms: E = Vn(1+cos(n phi+gamma)) for n > 0, Vn (phi-gamma)^2
+ * (difference wrapped to [-pi,pi]) for n == 0; phi is the IUPAC dihedral. */
+typedef struct orc_bonded {
+    int nbonds, nangles, ntorsions, nimpropers;
+    const int32_t *bond_ij;      /* 2 per bond        */
+    const double *bond_lk;       /* 2 per bond: l, k  */
+    const int32_t *angle_iaj;    /* 3 per angle, apex a in the middle */
+    const double *angle_tk;      /* 2 per angle: theta0, k */
+    const int32_t *tors_iabj;    /* 4 per torsion     */
+    const int32_t *tors_terms;   /* 1..4              */
+    const double *tors_vn, *tors_n, *tors_gamma; /* 4 per torsion */
+    const int32_t *impr_iabj;
+    const int32_t *impr_terms;
+    const double *impr_vn, *impr_n, *impr_gamma;
+} orc_bonded;
 
 /* Per-instance topology and parameters. */
 typedef struct orc_system {
@@ -59,6 +81,8 @@ typedef struct orc_system {
     const int8_t *band;      /* mid*(n-1) band of bond orders          */
     int nsparse;             /* off-diagonal bond orders               */
     const int32_t *sp_i, *sp_j, *sp_o;
+    const orc_bonded *bonded; /* FF_Bonded evaluated beside GB by the step
+                                 drivers (orc_minimise_sd, orc_md_vv); NULL: GB only */
 } orc_system;
 
 /* Packed neighbour list: row_start[n+1], entries j | (bondorder&7)<<24. */
@@ -119,6 +143,13 @@ void orc_nb_calc_forces(const orc_system *s, const double *xyz, const orc_nlist 
 void orc_gb_calc_energies(const orc_system *s, const double *xyz, const orc_nlist *nl,
                           const orc_params *p, double *force, orc_energies *e,
                           double *born);
+
+/* FF_Bonded::calcForces (forcefields/ffbonded.cpp:809-824): bonds, angles,
+ * torsions, impropers. Forces (N) are ADDED to force (may be NULL); the
+ * four component energies (J/molecule) are ADDED to e->epot_bond, ...,
+ * e->epot_torsion (torsions and impropers), e->epot_improper, their sum to
+ * e->epot, and the bond virial to e->virial. */
+void orc_bonded_forces(const orc_bonded *b, const double *xyz, double *force, orc_energies *e);
 
 /* A11 Minimisation::runcore with SteepestDescent, UpdateNList=update_nlist.
  * xyz is updated in place; returns steps done or -1 on NaN. energies/forces

@@ -65,6 +65,70 @@ public:
     double p(int k) const { const double v[5] = {P1, P2, P3, P4, P5}; return v[k]; }
 };
 
+// Gives the driver read access to FF_Bonded's assembled term lists
+// (forcefields/ffbonded.h:204-207).
+class BondedProbe : public FF_Bonded {
+public:
+    explicit BondedProbe(WorkSpace &ws) : FF_Bonded(ws) {}
+    const std::vector<Bond> &bonds() const { return bond; }
+    const std::vector<Angle> &angles() const { return angle; }
+    const std::vector<Torsion> &torsions() const { return torsion; }
+    const std::vector<Torsion> &impropers() const { return improper; }
+};
+
+// Uniform deviate in [-1, 1) from a SplitMix64 state.
+double splitmix_unit(uint64_t &x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+void dump_dihedrals(Rec &r, const std::vector<Torsion> &t, const std::string &tag) {
+    std::vector<int32_t> idx, terms;
+    std::vector<double> vn, nn, gamma;
+    for (size_t k = 0; k < t.size(); ++k) {
+        idx.push_back(t[k].i); idx.push_back(t[k].a); idx.push_back(t[k].b); idx.push_back(t[k].j);
+        terms.push_back(t[k].terms);
+        for (int m = 0; m < 4; ++m) {
+            const bool used = m < t[k].terms;
+            vn.push_back(used ? t[k].Vn[m] : 0.0);
+            nn.push_back(used ? t[k].n[m] : 0.0);
+            gamma.push_back(used ? t[k].gamma[m] : 0.0);
+        }
+    }
+    r.i(tag + "_idx", idx); r.i(tag + "_terms", terms);
+    r.d(tag + "_vn", vn); r.d(tag + "_n", nn); r.d(tag + "_gamma", gamma);
+}
+
+void dump_bonded_terms(Rec &r, const BondedProbe &b) {
+    std::vector<int32_t> bi, ai;
+    std::vector<double> bp, ap;
+    for (size_t k = 0; k < b.bonds().size(); ++k) {
+        const Bond &x = b.bonds()[k];
+        bi.push_back(x.i); bi.push_back(x.j); bp.push_back(x.l); bp.push_back(x.k);
+    }
+    for (size_t k = 0; k < b.angles().size(); ++k) {
+        const Angle &x = b.angles()[k];
+        ai.push_back(x.i); ai.push_back(x.a); ai.push_back(x.j); ap.push_back(x.theta0); ap.push_back(x.k);
+    }
+    r.i("bd_idx", bi); r.d("bd_lk", bp);
+    r.i("an_idx", ai); r.d("an_tk", ap);
+    dump_dihedrals(r, b.torsions(), "to");
+    dump_dihedrals(r, b.impropers(), "im");
+}
+
+void dump_forces(Rec &r, WorkSpace &ws, const std::string &tag);
+
+void dump_bonded_eval(Rec &r, WorkSpace &ws, const BondedProbe &b, const std::string &tag) {
+    dump_forces(r, ws, tag + "_force");
+    r.s(tag + "_epot", ws.ene.epot);
+    r.s(tag + "_bond", b.getEBond()); r.s(tag + "_angle", b.getEAngle());
+    r.s(tag + "_torsion", b.getETorsion()); r.s(tag + "_improper", b.getEImproper());
+}
+
 // SplitMix64 over the 20 standard residues.
 std::string random_sequence(uint64_t seed, int len) {
     static const char *aa = "ACDEFGHIKLMNPQRSTVWY";
@@ -246,6 +310,64 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
         np.push_back(nb.VdwInnerCutoff); np.push_back(nb.Vdw14Scaling); np.push_back(nb.Elec14Scaling);
         np.push_back(nb.Dielectric);
         r.d("nb_params", np);
+    }
+    {
+        // FF_Bonded alone: term lists, then forces at the built geometry and
+        // at a perturbed one (every coordinate moved by up to 0.2 A)
+        System sys(ffps);
+        sys.add(NewProtein(ffps, pd_grammar(c.seq)));
+        WorkSpace ws(sys);
+        Forcefield ffb(ws);
+        BondedProbe bonds(ws);
+        ffb.add(bonds);
+        ffb.setup();
+        dump_bonded_terms(r, bonds);
+        ffb.calcForces();
+        dump_bonded_eval(r, ws, bonds, "bf");
+        uint64_t x = 0x5EEDULL + c.name.size() * 7919ULL + (uint64_t)c.seq.size();
+        std::vector<double> xyz;
+        for (int i = 0; i < (int)ws.atom.size(); ++i) {
+            ws.cur.atom[i].p.x += 0.2 * splitmix_unit(x);
+            ws.cur.atom[i].p.y += 0.2 * splitmix_unit(x);
+            ws.cur.atom[i].p.z += 0.2 * splitmix_unit(x);
+            xyz.push_back(ws.cur.atom[i].p.x); xyz.push_back(ws.cur.atom[i].p.y); xyz.push_back(ws.cur.atom[i].p.z);
+        }
+        r.d("bfp_xyz", xyz);
+        ffb.calcForces();
+        dump_bonded_eval(r, ws, bonds, "bfp");
+    }
+    if (c.sd_steps > 0) {
+        // FF_Bonded + FF_GeneralizedBorn_Still in one Forcefield: energies at
+        // the built geometry, then steepest descent
+        System sys(ffps);
+        sys.add(NewProtein(ffps, pd_grammar(c.seq)));
+        WorkSpace ws(sys);
+        ws.nlist().setPadding(c.padding);
+        Forcefield ff(ws);
+        BondedProbe bonds(ws);
+        GBProbe gb(ws);
+        configure(gb, c);
+        ff.add(bonds);
+        ff.add(gb);
+        ff.printEnergySummary();
+        dump_forces(r, ws, "ff_force");
+        r.s("ff_epot", ws.ene.epot); r.s("ff_bond", ws.ene.epot_bond); r.s("ff_angle", ws.ene.epot_angle);
+        r.s("ff_torsion", ws.ene.epot_torsion); r.s("ff_vdw", ws.ene.epot_vdw); r.s("ff_elec", ws.ene.epot_elec);
+        r.s("ff_pol", ws.ene.epot_pol);
+        Protocol::Minimisation mini(ff);
+        mini.Algorithm = Protocol::Minimisation::SteepestDescent;
+        mini.Steps = c.sd_steps;
+        mini.UpdateScr = 0;
+        mini.OutputLevel = Verbosity::Silent;
+        int done = mini.run();
+        std::vector<double> xyz;
+        for (int i = 0; i < (int)ws.atom.size(); ++i) {
+            xyz.push_back(ws.cur.atom[i].p.x); xyz.push_back(ws.cur.atom[i].p.y); xyz.push_back(ws.cur.atom[i].p.z);
+        }
+        r.d("sdf_xyz", xyz);
+        dump_forces(r, ws, "sdf_force");
+        r.s("sdf_epot", ws.ene.epot);
+        r.s("sdf_done", (double)done);
     }
     if (c.sd_steps > 0) {
         System sys(ffps);

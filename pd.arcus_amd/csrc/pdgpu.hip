@@ -653,7 +653,7 @@ extern "C" int64_t pdgpu_instance_bytes(const pdgpu_ctx *c, int32_t natoms, int3
     const int64_t nb = (natoms + kBlk - 1) / kBlk;
     const int64_t per_atom = 8 * 21 + 16 * 2 + (c->mixed ? 4 + 16 + 16 : 8);
     const int64_t per_block = 4 * 3 + 32 * 2 + 16 + 4 * 2;
-    return nb * kBlk * per_atom + nb * per_block + (int64_t)mid * std::max(0, natoms - 1) + nsparse * 16 +
+    return nb * kBlk * per_atom + nb * per_block + (int64_t)mid * nb * kBlk + nsparse * 16 +
            nb * (nb + 1) / 2 * 8;
 }
 
@@ -685,8 +685,10 @@ extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t
         b0 = c->blocks.alloc(nb);
         if (b0 < 0) return fail(c, PDGPU_ERR_NOMEM, "block allocation failed");
     }
-    // band
-    const int64_t blen = (int64_t)t->max_index_delta * (n - 1);
+    // band: mid*(n-1) entries are uploaded; the range is allocated for every
+    // padded atom so that the enumeration can stage 64 whole rows per block
+    const int64_t bcopy = (int64_t)t->max_index_delta * (n - 1);
+    const int64_t blen = (int64_t)t->max_index_delta * nb * kBlk;
     int64_t boff = c->bandr.alloc(blen);
     if (boff < 0) {
         const int64_t need = c->cap_band - c->bandr.tail_free() + blen;
@@ -764,8 +766,8 @@ extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t
         c->h_blk_first[b0 + k] = (int)b0;
     }
     // bond orders
-    if (blen > 0)
-        HIPCHECK(c, hipMemcpyAsync((int8_t *)c->band.p + boff, t->band, blen, hipMemcpyHostToDevice, s));
+    if (bcopy > 0)
+        HIPCHECK(c, hipMemcpyAsync((int8_t *)c->band.p + boff, t->band, bcopy, hipMemcpyHostToDevice, s));
     std::vector<int4> sp(t->nsparse);
     for (int k = 0; k < t->nsparse; ++k) {
         int i = t->sparse_i[k], j = t->sparse_j[k];

@@ -203,6 +203,7 @@ class Context:
             raise PdgpuError(rc, "pdgpu_ctx_create failed")
         self.h = h
         self._natoms = {}
+        self._earr = None
 
     def close(self):
         if self.h:
@@ -292,13 +293,17 @@ class Context:
                                            C.byref(cnt)), "nlist_export")
         return rs, ent[:cnt.value]
 
-    def _calc(self, fn, iid, want_forces, check):
+    def _calc(self, fn, iid, want_forces, check, raw=False):
         if iid == PDGPU_ALL:
-            n = self.opt.max_instances if self.opt.max_instances > 0 else 8192
-            arr = (Energies * n)()
+            if self._earr is None:
+                n = self.opt.max_instances if self.opt.max_instances > 0 else 8192
+                self._earr = (Energies * n)()
+            arr = self._earr
             rc = fn(self.h, iid, arr, None)
             if check and rc not in (0, PDGPU_ERR_NONFINITE):
                 self._check(rc, "calc")
+            if raw:  # the per-slot pdgpu_energies array, reused by the next call
+                return arr
             return {i: arr[i].as_dict() for i in self._natoms}
         e = Energies()
         f = np.zeros(3 * self._natoms[iid]) if want_forces else None
@@ -307,11 +312,11 @@ class Context:
             self._check(rc, "calc")
         return e.as_dict(), (f.reshape(-1, 3) if want_forces else None)
 
-    def calc_forces(self, iid=PDGPU_ALL, forces=True, check=True):
-        return self._calc(lib.pdgpu_calc_forces, iid, forces, check)
+    def calc_forces(self, iid=PDGPU_ALL, forces=True, check=True, raw=False):
+        return self._calc(lib.pdgpu_calc_forces, iid, forces, check, raw)
 
-    def calc_energies(self, iid=PDGPU_ALL, forces=True, check=True):
-        return self._calc(lib.pdgpu_calc_energies, iid, forces, check)
+    def calc_energies(self, iid=PDGPU_ALL, forces=True, check=True, raw=False):
+        return self._calc(lib.pdgpu_calc_energies, iid, forces, check, raw)
 
     def born_radii(self, iid):
         b = np.zeros(self._natoms[iid])

@@ -22,7 +22,8 @@ def load(name):
     d["name"] = name
     d["n"] = len(d["charge"])
     d["xyz"] = d["xyz"].reshape(-1, 3)
-    for k in ("force", "en_force", "nb_force", "sd_force", "sd_xyz"):
+    for k in ("force", "en_force", "nb_force", "sd_force", "sd_xyz", "bf_force", "bfp_force", "bfp_xyz",
+              "ff_force", "sdf_force", "sdf_xyz"):
         if k in d:
             d[k] = d[k].reshape(-1, 3)
     d["mid"] = int(d["mid"][0])

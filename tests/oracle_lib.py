@@ -29,17 +29,25 @@ class Params(C.Structure):
 
 class Energies(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
-        "epot", "epot_vdw", "epot_elec", "epot_pol", "epot_pol_self", "epot_pol_cross", "virial")]
+        "epot", "epot_vdw", "epot_elec", "epot_pol", "epot_pol_self", "epot_pol_cross", "virial",
+        "epot_bond", "epot_angle", "epot_torsion", "epot_improper")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Bonded(C.Structure):
+    _fields_ = [("nbonds", C.c_int), ("nangles", C.c_int), ("ntorsions", C.c_int), ("nimpropers", C.c_int),
+                ("bond_ij", _ip), ("bond_lk", _dp), ("angle_iaj", _ip), ("angle_tk", _dp),
+                ("tors_iabj", _ip), ("tors_terms", _ip), ("tors_vn", _dp), ("tors_n", _dp), ("tors_gamma", _dp),
+                ("impr_iabj", _ip), ("impr_terms", _ip), ("impr_vn", _dp), ("impr_n", _dp), ("impr_gamma", _dp)]
 
 
 class System(C.Structure):
     _fields_ = [("n", C.c_int), ("charge", _dp), ("radius", _dp), ("epsilon", _dp),
                 ("gb_radius", _dp), ("vj", _dp), ("terms123", _dp), ("mass", _dp),
                 ("mid", C.c_int), ("band", _bp), ("nsparse", C.c_int),
-                ("sp_i", _ip), ("sp_j", _ip), ("sp_o", _ip)]
+                ("sp_i", _ip), ("sp_j", _ip), ("sp_o", _ip), ("bonded", C.POINTER(Bonded))]
 
 
 class NList(C.Structure):
@@ -73,6 +81,7 @@ lib.orc_nb_calc_forces.argtypes = [C.POINTER(System), _dp, C.POINTER(NList), C.P
                                    _dp, C.POINTER(Energies)]
 lib.orc_gb_calc_energies.argtypes = [C.POINTER(System), _dp, C.POINTER(NList), C.POINTER(Params),
                                      _dp, C.POINTER(Energies), _dp]
+lib.orc_bonded_forces.argtypes = [C.POINTER(Bonded), _dp, _dp, C.POINTER(Energies)]
 lib.orc_minimise_sd.restype = C.c_int
 lib.orc_minimise_sd.argtypes = [C.POINTER(System), _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                                 _dp, C.POINTER(Energies), _dp]
@@ -105,7 +114,7 @@ class OSystem:
     """Owns numpy arrays and the ctypes view of one instance."""
 
     def __init__(self, charge, radius, epsilon, gb_radius, vj, terms123, mass, mid, band,
-                 sp_i=None, sp_j=None, sp_o=None):
+                 sp_i=None, sp_j=None, sp_o=None, bonded=None):
         f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         self.charge, self.radius, self.epsilon = f(charge), f(radius), f(epsilon)
         self.gb_radius, self.vj, self.terms123 = f(gb_radius), f(vj), f(terms123)
@@ -120,7 +129,38 @@ class OSystem:
         self.c = System(self.n, _ptr(self.charge, _dp), _ptr(self.radius, _dp), _ptr(self.epsilon, _dp),
                         _ptr(self.gb_radius, _dp), _ptr(self.vj, _dp), _ptr(self.terms123, _dp),
                         _ptr(self.mass, _dp), self.mid, _ptr(self.band, _bp), len(self.sp_i),
-                        _ptr(self.sp_i, _ip), _ptr(self.sp_j, _ip), _ptr(self.sp_o, _ip))
+                        _ptr(self.sp_i, _ip), _ptr(self.sp_j, _ip), _ptr(self.sp_o, _ip),
+                        C.pointer(bonded.c) if bonded is not None else None)
+        self.bonded = bonded  # keeps the term arrays alive
+
+
+class OBonded:
+    """FF_Bonded term lists in the layout of a golden fixture or of
+    pdgpu.synth (keys bd_idx, bd_lk, an_idx, an_tk, to_idx, to_terms, to_vn,
+    to_n, to_gamma, im_idx, im_terms, im_vn, im_n, im_gamma)."""
+
+    def __init__(self, d):
+        fi = lambda k: np.ascontiguousarray(d[k], dtype=np.int32).ravel()
+        fd = lambda k: np.ascontiguousarray(d[k], dtype=np.float64).ravel()
+        self.arr = {k: (fi(k) if k.endswith(("idx", "terms")) else fd(k)) for k in (
+            "bd_idx", "bd_lk", "an_idx", "an_tk", "to_idx", "to_terms", "to_vn", "to_n", "to_gamma",
+            "im_idx", "im_terms", "im_vn", "im_n", "im_gamma")}
+        a = self.arr
+        self.c = Bonded(len(a["bd_lk"]) // 2, len(a["an_tk"]) // 2, len(a["to_terms"]), len(a["im_terms"]),
+                        _ptr(a["bd_idx"], _ip), _ptr(a["bd_lk"], _dp), _ptr(a["an_idx"], _ip),
+                        _ptr(a["an_tk"], _dp), _ptr(a["to_idx"], _ip), _ptr(a["to_terms"], _ip),
+                        _ptr(a["to_vn"], _dp), _ptr(a["to_n"], _dp), _ptr(a["to_gamma"], _dp),
+                        _ptr(a["im_idx"], _ip), _ptr(a["im_terms"], _ip), _ptr(a["im_vn"], _dp),
+                        _ptr(a["im_n"], _dp), _ptr(a["im_gamma"], _dp))
+
+
+def bonded_forces(bonded, xyz):
+    """FF_Bonded::calcForces: (energies dict, forces (n,3) in N)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
+    f = np.zeros_like(xyz)
+    e = Energies()
+    lib.orc_bonded_forces(C.byref(bonded.c), _ptr(xyz, _dp), _ptr(f, _dp), C.byref(e))
+    return e.as_dict(), f.reshape(-1, 3)
 
 
 def nlist_build(sysm, xyz, list_cutoff, padding, shadow=True):

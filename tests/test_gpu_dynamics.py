@@ -175,7 +175,8 @@ def test_server_poisson_requests_complete_and_match():
             st = ctx.minimise_sd(i, 60, 2e5, 1, 0.1)
             e, _ = ctx.calc_forces(i)
         assert st == r.steps
-        assert e["epot"] == r.energies["epot"]
+        # same bits alone and in the mix (a collapsed GB-only run is NaN in both)
+        assert np.float64(e["epot"]).tobytes() == np.float64(r.energies["epot"]).tobytes()
 
 
 def test_admission_respects_budget():
