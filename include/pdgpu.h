@@ -79,8 +79,20 @@ typedef struct pdgpu_options {
     double nlist_padding;         /* NeighbourList Padding (skin)                  */
     int64_t device_memory_budget; /* bytes the context may allocate; 0: 80% free  */
     int32_t max_instances;        /* instance slots; 0: 8192                       */
-    int32_t reserved;
+    int32_t components;           /* PDGPU_FF_* bits evaluated; default all        */
 } pdgpu_options;
+
+/* Forcefield components of a context: FF_GeneralizedBorn_Still and the four
+ * term kinds of FF_Bonded (its DoBonds / DoAngles / DoTorsions / DoImpropers,
+ * forcefields/ffbonded.h:147-150). Bonded terms contribute only for
+ * instances given term lists with pdgpu_bonded_set. */
+#define PDGPU_FF_GB 1
+#define PDGPU_FF_BONDS 2
+#define PDGPU_FF_ANGLES 4
+#define PDGPU_FF_TORSIONS 8
+#define PDGPU_FF_IMPROPERS 16
+#define PDGPU_FF_BONDED (PDGPU_FF_BONDS | PDGPU_FF_ANGLES | PDGPU_FF_TORSIONS | PDGPU_FF_IMPROPERS)
+#define PDGPU_FF_ALL (PDGPU_FF_GB | PDGPU_FF_BONDED)
 
 /* One molecule. Per-atom arrays have natoms entries. The bond-order band and
  * the sparse list are PD's BondOrder (workspace/bondorder.h:43-112): band has
@@ -102,10 +114,35 @@ typedef struct pdgpu_topology {
     const int32_t *sparse_i, *sparse_j, *sparse_order;
 } pdgpu_topology;
 
+/* FF_Bonded's assembled term lists for one instance (forcefields/ffbonded.h:
+ * 22-72, as FF_Bonded::setup builds them, forcefields/ffbonded.cpp:13-486),
+ * atom indices local to the instance. Bond: E = k (d-l)^2 with l in
+ * Angstrom, k in J/molecule/A^2. Angle i-a-j (apex a): E = k (theta-theta0)^2,
+ * theta0 in rad, k in J/molecule/rad^2. Torsion / improper i-a-b-j with
+ * 1..4 terms: E = Vn (1 + cos(n phi + gamma)) for n > 0, Vn (phi-gamma)^2
+ * (difference wrapped to [-pi, pi]) for n == 0; phi is the IUPAC dihedral,
+ * Vn in J/molecule, gamma in rad; 4 entries per dihedral in vn/n/gamma. */
+typedef struct pdgpu_bonded {
+    int32_t nbonds, nangles, ntorsions, nimpropers;
+    const int32_t *bond_ij;        /* 2 per bond                    */
+    const double *bond_lk;         /* 2 per bond: l, k              */
+    const int32_t *angle_iaj;      /* 3 per angle                   */
+    const double *angle_tk;        /* 2 per angle: theta0, k        */
+    const int32_t *torsion_iabj;   /* 4 per torsion                 */
+    const int32_t *torsion_terms;  /* 1 per torsion                 */
+    const double *torsion_vn, *torsion_n, *torsion_gamma;
+    const int32_t *improper_iabj;
+    const int32_t *improper_terms;
+    const double *improper_vn, *improper_n, *improper_gamma;
+} pdgpu_bonded;
+
 /* Energy components (workspace/hamiltonian.h:23-90), J/molecule. */
 typedef struct pdgpu_energies {
     double epot, epot_vdw, epot_elec, epot_pol, epot_pol_self, epot_pol_cross;
-    double virial;          /* InternalVirial (energy path)                      */
+    double virial;          /* InternalVirial: GB energy path + bond stretching  */
+    double epot_bond, epot_angle;
+    double epot_torsion;    /* torsions + impropers, as the Hamiltonian holds it */
+    double epot_improper;   /* FF_Bonded::getEImproper                           */
     int64_t pairs;          /* unordered pairs within Cutoff (pair pass)         */
     int64_t born_pairs;     /* ordered pairs in the Born/chain-rule sums         */
     int32_t status;         /* PDGPU_OK or PDGPU_ERR_NONFINITE for this instance */
@@ -128,6 +165,14 @@ const char *pdgpu_last_error(const pdgpu_ctx *ctx);
  * (forcefields/gbff.cpp:40-87) and WorkSpace::bondorder() upload. */
 int pdgpu_instance_add(pdgpu_ctx *ctx, const pdgpu_topology *t, int32_t *id);
 int pdgpu_instance_remove(pdgpu_ctx *ctx, int32_t id);
+
+/* FF_Bonded::setup for one instance: uploads its term lists (replacing any
+ * earlier ones). Energy units J/molecule as described at pdgpu_bonded. */
+int pdgpu_bonded_set(pdgpu_ctx *ctx, int32_t id, const pdgpu_bonded *b);
+
+/* Device bytes the records of pdgpu_bonded_set take (no reference
+ * counterpart; for admission by memory budget beside pdgpu_instance_bytes). */
+int64_t pdgpu_bonded_bytes(int32_t nbonds, int32_t nangles, int32_t ntorsions, int32_t nimpropers);
 int pdgpu_instance_natoms(const pdgpu_ctx *ctx, int32_t id);
 
 /* WorkSpace::cur.atom[].p in / out. */
@@ -146,15 +191,20 @@ int pdgpu_nlist_build(pdgpu_ctx *ctx, int32_t id);
 int pdgpu_nlist_export(pdgpu_ctx *ctx, int32_t id, int32_t shadow, int32_t *row_start,
                        uint32_t *entries, int64_t capacity, int64_t *count);
 
-/* FF_GeneralizedBorn_Still::calcForces (forcefields/gbff.cpp:168-180):
- * Born radii, fused pair pass, chain rule. energies: one record for id, or
- * an array indexed by instance id for PDGPU_ALL (may be NULL). forces:
- * natoms*3 Newton for a single id, or NULL to keep them on the device. */
+/* Forcefield::calcForces over the context's components
+ * (forcefields/forcefield.cpp:220-227): FF_Bonded::calcForces
+ * (forcefields/ffbonded.cpp:809-824) and FF_GeneralizedBorn_Still::calcForces
+ * (forcefields/gbff.cpp:168-180: Born radii, fused pair pass, chain rule).
+ * energies: one record for id, or an array indexed by instance id for
+ * PDGPU_ALL (may be NULL). forces: natoms*3 Newton for a single id, or NULL
+ * to keep them on the device. */
 int pdgpu_calc_forces(pdgpu_ctx *ctx, int32_t id, pdgpu_energies *energies, double *forces);
 
-/* FF_GeneralizedBorn_Still::calcEnergies (forcefields/gbff.cpp:161-166):
- * Born radii, FF_NonBonded energy path (its forces included, as the
- * reference writes them), exact pairwise GB energy. */
+/* Forcefield::calcEnergies (forcefields/forcefield.cpp:212-218):
+ * FF_Bonded::calcEnergies (= its calcForces, forcefields/ffbonded.cpp:804)
+ * and FF_GeneralizedBorn_Still::calcEnergies (forcefields/gbff.cpp:161-166:
+ * Born radii, FF_NonBonded energy path with its forces, as the reference
+ * writes them, exact pairwise GB energy). */
 int pdgpu_calc_energies(pdgpu_ctx *ctx, int32_t id, pdgpu_energies *energies, double *forces);
 
 /* Device results of the last evaluation: Born radii (Angstrom), forces (N). */
@@ -207,7 +257,8 @@ enum pdgpu_kernel_group {
     PDGPU_KERNEL_CHAIN = 4,     /* chain rule through the Born radii         */
     PDGPU_KERNEL_INTEGRATE = 5, /* integrator / minimiser updates            */
     PDGPU_KERNEL_ENUM = 6,      /* per-evaluation pair enumeration           */
-    PDGPU_KERNEL_NGROUPS = 7
+    PDGPU_KERNEL_BONDED = 7,    /* FF_Bonded terms                           */
+    PDGPU_KERNEL_NGROUPS = 8
 };
 int pdgpu_profile(pdgpu_ctx *ctx, int32_t enable);
 int pdgpu_profile_read(pdgpu_ctx *ctx, double *ms, int64_t *launches, int32_t n);

@@ -2,6 +2,7 @@
 // declared in include/pdgpu.h.
 #include "../../include/pdgpu.h"
 #include "pdgpu_device.h"
+#include "pdgpu_bonded.h"
 #include "pdgpu_integrate.h"
 
 #include <hip/hip_runtime.h>
@@ -85,6 +86,7 @@ struct Instance {
     bool active = false;
     int natoms = 0, nblocks = 0, block0 = 0, mid = 0, nsparse = 0;
     int64_t band_off = 0, band_len = 0, sp_off = 0;
+    int64_t bd_off = 0, bd_len = 0, an_off = 0, an_len = 0, dh_off = 0, dh_len = 0; // FF_Bonded records
     bool listed = false;  // tile rows built
     bool has_vel = false;
     bool sd = false;      // a minimisation was started and not finished
@@ -106,7 +108,8 @@ struct pdgpu_ctx {
     // capacities
     int64_t cap_blocks = 0, cap_band = 0, cap_sparse = 0, cap_tiles = 0;
     int max_inst = 0;
-    Ranges blocks, bandr, spr;
+    Ranges blocks, bandr, spr, bdr, anr, dhr;
+    int64_t cap_bd = 0, cap_an = 0, cap_dh = 0;
     std::vector<Instance> inst;
     int64_t tri_sum = 0; // sum over instances of nb*(nb+1)/2 (tile bound)
     int64_t ntiles = 0;  // tiles after the last build
@@ -121,6 +124,7 @@ struct pdgpu_ctx {
     DevBuf inst_need, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf band, sparse;
     DevBuf plist, tile_cnt, overflow;
+    DevBuf bd_idx, bd_par, an_idx, an_par, dh_idx, dh_meta, dh_vn, dh_n, dh_gamma; // FF_Bonded records
     int64_t plist_cap = 0; // tiles the pair-list buffer holds
 
     // host mirrors of per-block metadata
@@ -157,6 +161,25 @@ struct pdgpu_ctx {
         s.plist_cap = (int)std::min<int64_t>(plist_cap, INT32_MAX);
         s.overflow = (int *)overflow.p;
         return s;
+    }
+
+    // FF_Bonded records of instance id (PDGPU_ALL: every record slot)
+    BondedState B(int id) const {
+        BondedState b;
+        b.bd_idx = (const int4 *)bd_idx.p; b.bd_par = (const double2 *)bd_par.p;
+        b.an_idx = (const int4 *)an_idx.p; b.an_par = (const double2 *)an_par.p;
+        b.dh_idx = (const int4 *)dh_idx.p; b.dh_meta = (const int *)dh_meta.p;
+        b.dh_vn = (const double4 *)dh_vn.p; b.dh_n = (const double4 *)dh_n.p; b.dh_gamma = (const double4 *)dh_gamma.p;
+        if (id == PDGPU_ALL) {
+            b.b0 = 0; b.b1 = (int)cap_bd; b.a0 = 0; b.a1 = (int)cap_an; b.d0 = 0; b.d1 = (int)cap_dh;
+        } else {
+            const Instance &I = inst[id];
+            b.b0 = (int)I.bd_off; b.b1 = (int)(I.bd_off + I.bd_len);
+            b.a0 = (int)I.an_off; b.a1 = (int)(I.an_off + I.an_len);
+            b.d0 = (int)I.dh_off; b.d1 = (int)(I.dh_off + I.dh_len);
+        }
+        b.comps = opt.components;
+        return b;
     }
 };
 
@@ -429,21 +452,11 @@ int64_t span_tiles(const pdgpu_ctx *c, int id) {
     return std::min<int64_t>(c->ntiles, (int64_t)nb * (nb + 1) / 2);
 }
 
-// One evaluation: MODE 0 = calcForces, MODE 1 = calcEnergies.
+// The GB kernels of one evaluation: enumeration, Born sums, Born radii,
+// pair pass and chain rule (MODE 0) or the energy-path pair pass (MODE 1).
 template <typename R>
-int launch_eval(pdgpu_ctx *c, int id, int mode) {
-    const DevState S = c->S();
-    const Span sp = span_of(c, id);
-    const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk;
-    const int na = a1 - a0;
-    const int ns = sp.s1 - sp.s0;
-    int groups, tpw;
-    pair_grid(c, span_tiles(c, id), &groups, &tpw);
-    {
-        ProfScope ps(c, PDGPU_KERNEL_ATOM);
-        hipLaunchKernelGGL(k_inst_clear, dim3((ns + 255) / 256), dim3(256), 0, c->stream, S, sp.s0, sp.s1);
-        hipLaunchKernelGGL(k_prepare<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
-    }
+int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int groups, int tpw, int mode) {
+    const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk, na = a1 - a0;
     {
         ProfScope ps(c, PDGPU_KERNEL_ENUM);
         hipLaunchKernelGGL(k_enum<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
@@ -467,6 +480,33 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
         ProfScope ps(c, PDGPU_KERNEL_PAIR);
         hipLaunchKernelGGL((k_pair<R, 1>), dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
     }
+    return PDGPU_OK;
+}
+
+// One evaluation: MODE 0 = calcForces, MODE 1 = calcEnergies.
+template <typename R>
+int launch_eval(pdgpu_ctx *c, int id, int mode) {
+    const DevState S = c->S();
+    const Span sp = span_of(c, id);
+    const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk;
+    const int na = a1 - a0;
+    const int ns = sp.s1 - sp.s0;
+    int groups, tpw;
+    pair_grid(c, span_tiles(c, id), &groups, &tpw);
+    const bool gb = c->opt.components & PDGPU_FF_GB;
+    {
+        ProfScope ps(c, PDGPU_KERNEL_ATOM);
+        hipLaunchKernelGGL(k_inst_clear, dim3((ns + 255) / 256), dim3(256), 0, c->stream, S, sp.s0, sp.s1);
+        if (gb) hipLaunchKernelGGL(k_prepare<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+        else hipLaunchKernelGGL(k_clear_grad<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+    }
+    if (gb) TRY(launch_gb<R>(c, S, sp, groups, tpw, mode));
+    const BondedState B = c->B(id);
+    const int nterms = (B.b1 - B.b0) + (B.a1 - B.a0) + (B.d1 - B.d0);
+    if ((c->opt.components & PDGPU_FF_BONDED) && nterms > 0) {
+        ProfScope ps(c, PDGPU_KERNEL_BONDED);
+        hipLaunchKernelGGL(k_bonded<R>, dim3((nterms + 255) / 256), dim3(256), 0, c->stream, S, B);
+    }
     {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
         hipLaunchKernelGGL(k_finalize<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
@@ -476,7 +516,7 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
 }
 
 int eval(pdgpu_ctx *c, int id, int mode) {
-    TRY(ensure_listed(c, id));
+    if (c->opt.components & PDGPU_FF_GB) TRY(ensure_listed(c, id));
     return c->mixed ? launch_eval<float>(c, id, mode) : launch_eval<double>(c, id, mode);
 }
 
@@ -513,8 +553,14 @@ int read_energies(pdgpu_ctx *c, int id, pdgpu_energies *out, int mode) {
         E.epot_pol_cross = v[kEPolCross];
         E.epot_pol_self = v[kEPolSelf];
         E.epot_pol = v[kEPolCross] + v[kEPolSelf];
-        E.virial = (mode == 1) ? v[kEVirial] : 0.0;
-        E.epot = E.epot_vdw + E.epot_elec + E.epot_pol;
+        // the GB force path adds no virial; the energy path and bond
+        // stretching do (ffbonded.cpp:924)
+        E.virial = v[kEVirial];
+        E.epot_bond = v[kEBond];
+        E.epot_angle = v[kEAngle];
+        E.epot_torsion = v[kETorsion] + v[kEImproper]; // calcImproperForces adds to epot_torsion
+        E.epot_improper = v[kEImproper];
+        E.epot = E.epot_vdw + E.epot_elec + E.epot_pol + E.epot_bond + E.epot_angle + E.epot_torsion;
         E.pairs = (int64_t)cnt[(size_t)k * kCNum + kCPairs];
         E.born_pairs = (int64_t)cnt[(size_t)k * kCNum + kCBornPairs];
         const bool bad = st[k] != 0 || !std::isfinite(E.epot);
@@ -565,6 +611,41 @@ int scatter3(pdgpu_ctx *c, int id, DevBuf &x, DevBuf &y, DevBuf &z, const double
     return PDGPU_OK;
 }
 
+// Reserves n records in one FF_Bonded record family (bufs / esz: its
+// arrays and their record sizes), growing the arrays when needed. New
+// array space is zeroed, which marks its records free.
+int term_alloc(pdgpu_ctx *c, Ranges &r, int64_t &cap, int64_t n, DevBuf *const *bufs, const size_t *esz, int nbuf,
+               int64_t *off) {
+    *off = 0;
+    if (n == 0) return PDGPU_OK;
+    int64_t o = r.alloc(n);
+    if (o < 0) {
+        const int64_t need = cap - r.tail_free() + n;
+        const int64_t nc = std::max<int64_t>(std::max<int64_t>(need, cap * 3 / 2), 1024);
+        if (nc > INT32_MAX) return fail(c, PDGPU_ERR_NOMEM, "too many bonded terms");
+        for (int k = 0; k < nbuf; ++k) TRY(resize(c, *bufs[k], (size_t)nc, esz[k]));
+        r.grow(nc);
+        cap = nc;
+        o = r.alloc(n);
+        if (o < 0) return fail(c, PDGPU_ERR_NOMEM, "bonded term allocation failed");
+    }
+    *off = o;
+    return PDGPU_OK;
+}
+
+// Frees the FF_Bonded records of an instance (zeroed records are skipped).
+int bonded_release(pdgpu_ctx *c, Instance &I) {
+    if (I.bd_len) HIPCHECK(c, hipMemsetAsync((int4 *)c->bd_idx.p + I.bd_off, 0, I.bd_len * 16, c->stream));
+    if (I.an_len) HIPCHECK(c, hipMemsetAsync((int4 *)c->an_idx.p + I.an_off, 0, I.an_len * 16, c->stream));
+    if (I.dh_len) HIPCHECK(c, hipMemsetAsync((int *)c->dh_meta.p + I.dh_off, 0, I.dh_len * 4, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->bdr.release(I.bd_off, I.bd_len);
+    c->anr.release(I.an_off, I.an_len);
+    c->dhr.release(I.dh_off, I.dh_len);
+    I.bd_off = I.bd_len = I.an_off = I.an_len = I.dh_off = I.dh_len = 0;
+    return PDGPU_OK;
+}
+
 } // namespace
 
 // =================================================================== C ABI
@@ -576,6 +657,7 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     *out = nullptr;
     if (o->fast_mode != 1) return PDGPU_ERR_UNSUPPORTED;
     if (o->precision != PDGPU_FP64 && o->precision != PDGPU_MIXED) return PDGPU_ERR_INVALID;
+    if ((o->components & ~PDGPU_FF_ALL) != 0) return PDGPU_ERR_INVALID;
     if (!(o->cutoff > 0) || !(o->vdw_cutoff > 0) || o->inner_cutoff > o->cutoff ||
         o->vdw_inner_cutoff > o->vdw_cutoff || !(o->p5 > 0) || !(o->dielectric_solute > 0) ||
         !(o->dielectric_solvent > 0) || o->nlist_padding < 0)
@@ -627,6 +709,9 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     c->cap_sparse = c->sparse.bytes / 16;
     c->spr.init(c->cap_sparse);
     c->cap_tiles = c->tiles.bytes / 8;
+    c->bdr.init(0);
+    c->anr.init(0);
+    c->dhr.init(0);
     *out = c;
     return PDGPU_OK;
 }
@@ -640,7 +725,9 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
                      &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->row_count, &c->row_off,
                      &c->tiles, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
-                     &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow};
+                     &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
+                     &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
+                     &c->dh_gamma};
     for (DevBuf *b : all) release(*b);
     for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : c->prof_pool) (void)hipEventDestroy(e);
@@ -816,6 +903,7 @@ extern "C" int pdgpu_instance_remove(pdgpu_ctx *c, int32_t id) {
     const int zero1 = 0;
     HIPCHECK(c, hipMemcpyAsync((int *)c->inst_nat.p + id, &zero1, 4, hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    TRY(bonded_release(c, I));
     for (int k = 0; k < I.nblocks; ++k) c->h_blk_inst[I.block0 + k] = -1;
     c->blocks.release(I.block0, I.nblocks);
     c->bandr.release(I.band_off, I.band_len);
@@ -831,6 +919,104 @@ extern "C" int pdgpu_instance_remove(pdgpu_ctx *c, int32_t id) {
     HIPCHECK(c, hipGetLastError());
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return PDGPU_OK;
+}
+
+extern "C" int pdgpu_bonded_set(pdgpu_ctx *c, int32_t id, const pdgpu_bonded *b) {
+    if (!c || !valid_id(c, id) || !b) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    Instance &I = c->inst[id];
+    const int n = I.natoms;
+    const int nb = b->nbonds, na = b->nangles, nt = b->ntorsions, ni = b->nimpropers;
+    if (nb < 0 || na < 0 || nt < 0 || ni < 0 || (nb && (!b->bond_ij || !b->bond_lk)) ||
+        (na && (!b->angle_iaj || !b->angle_tk)) ||
+        (nt && (!b->torsion_iabj || !b->torsion_terms || !b->torsion_vn || !b->torsion_n || !b->torsion_gamma)) ||
+        (ni && (!b->improper_iabj || !b->improper_terms || !b->improper_vn || !b->improper_n || !b->improper_gamma)))
+        return fail(c, PDGPU_ERR_INVALID, "invalid bonded term lists");
+    auto in_range = [n](const int32_t *p, int64_t cnt) {
+        for (int64_t k = 0; k < cnt; ++k)
+            if (p[k] < 0 || p[k] >= n) return false;
+        return true;
+    };
+    if (!in_range(b->bond_ij, 2 * (int64_t)nb) || !in_range(b->angle_iaj, 3 * (int64_t)na) ||
+        !in_range(b->torsion_iabj, 4 * (int64_t)nt) || !in_range(b->improper_iabj, 4 * (int64_t)ni))
+        return fail(c, PDGPU_ERR_INVALID, "bonded term refers to an atom outside the instance");
+    for (int k = 0; k < nt; ++k)
+        if (b->torsion_terms[k] < 0 || b->torsion_terms[k] > 4) return fail(c, PDGPU_ERR_INVALID, "torsion terms");
+    for (int k = 0; k < ni; ++k)
+        if (b->improper_terms[k] < 0 || b->improper_terms[k] > 4) return fail(c, PDGPU_ERR_INVALID, "improper terms");
+    TRY(bonded_release(c, I));
+    const int64_t nd = (int64_t)nt + ni;
+    {
+        DevBuf *bb[] = {&c->bd_idx, &c->bd_par};
+        const size_t be[] = {16, 16};
+        TRY(term_alloc(c, c->bdr, c->cap_bd, nb, bb, be, 2, &I.bd_off));
+        I.bd_len = nb;
+        DevBuf *ab[] = {&c->an_idx, &c->an_par};
+        TRY(term_alloc(c, c->anr, c->cap_an, na, ab, be, 2, &I.an_off));
+        I.an_len = na;
+        DevBuf *db[] = {&c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n, &c->dh_gamma};
+        const size_t de[] = {16, 4, 32, 32, 32};
+        TRY(term_alloc(c, c->dhr, c->cap_dh, nd, db, de, 5, &I.dh_off));
+        I.dh_len = nd;
+    }
+    const int g0 = I.block0 * kBlk, s1 = id + 1;
+    const double inv = 1.0 / kJPerKcal; // J/molecule -> kcal/mol
+    std::vector<int4> bi(nb), ai(na), di(nd);
+    std::vector<double2> bp(nb), ap(na);
+    std::vector<int> dm(nd);
+    std::vector<double4> dv(nd), dn(nd), dg(nd);
+    for (int k = 0; k < nb; ++k) {
+        bi[k] = make_int4(g0 + b->bond_ij[2 * k], g0 + b->bond_ij[2 * k + 1], s1, 0);
+        bp[k] = make_double2(b->bond_lk[2 * k], b->bond_lk[2 * k + 1] * inv);
+    }
+    for (int k = 0; k < na; ++k) {
+        ai[k] = make_int4(g0 + b->angle_iaj[3 * k], g0 + b->angle_iaj[3 * k + 1], g0 + b->angle_iaj[3 * k + 2], s1);
+        ap[k] = make_double2(b->angle_tk[2 * k], b->angle_tk[2 * k + 1] * inv);
+    }
+    for (int64_t k = 0; k < nd; ++k) {
+        const bool im = k >= nt;
+        const int64_t r = im ? k - nt : k;
+        const int32_t *q = (im ? b->improper_iabj : b->torsion_iabj) + 4 * r;
+        const double *v = (im ? b->improper_vn : b->torsion_vn) + 4 * r;
+        const double *nn = (im ? b->improper_n : b->torsion_n) + 4 * r;
+        const double *g = (im ? b->improper_gamma : b->torsion_gamma) + 4 * r;
+        const int terms = (im ? b->improper_terms : b->torsion_terms)[r];
+        di[k] = make_int4(g0 + q[0], g0 + q[1], g0 + q[2], g0 + q[3]);
+        dm[k] = s1 | (terms << 24) | ((im ? 1 : 0) << 28);
+        double vv[4], ww[4], gg[4];
+        for (int t = 0; t < 4; ++t) {
+            const bool used = t < terms;
+            vv[t] = used ? v[t] * inv : 0.0;
+            ww[t] = used ? nn[t] : 0.0;
+            gg[t] = used ? g[t] : 0.0;
+        }
+        dv[k] = make_double4(vv[0], vv[1], vv[2], vv[3]);
+        dn[k] = make_double4(ww[0], ww[1], ww[2], ww[3]);
+        dg[k] = make_double4(gg[0], gg[1], gg[2], gg[3]);
+    }
+    hipStream_t s = c->stream;
+    if (nb) {
+        HIPCHECK(c, hipMemcpyAsync((int4 *)c->bd_idx.p + I.bd_off, bi.data(), nb * 16, hipMemcpyHostToDevice, s));
+        HIPCHECK(c, hipMemcpyAsync((double2 *)c->bd_par.p + I.bd_off, bp.data(), nb * 16, hipMemcpyHostToDevice, s));
+    }
+    if (na) {
+        HIPCHECK(c, hipMemcpyAsync((int4 *)c->an_idx.p + I.an_off, ai.data(), na * 16, hipMemcpyHostToDevice, s));
+        HIPCHECK(c, hipMemcpyAsync((double2 *)c->an_par.p + I.an_off, ap.data(), na * 16, hipMemcpyHostToDevice, s));
+    }
+    if (nd) {
+        HIPCHECK(c, hipMemcpyAsync((int4 *)c->dh_idx.p + I.dh_off, di.data(), nd * 16, hipMemcpyHostToDevice, s));
+        HIPCHECK(c, hipMemcpyAsync((double4 *)c->dh_vn.p + I.dh_off, dv.data(), nd * 32, hipMemcpyHostToDevice, s));
+        HIPCHECK(c, hipMemcpyAsync((double4 *)c->dh_n.p + I.dh_off, dn.data(), nd * 32, hipMemcpyHostToDevice, s));
+        HIPCHECK(c, hipMemcpyAsync((double4 *)c->dh_gamma.p + I.dh_off, dg.data(), nd * 32, hipMemcpyHostToDevice, s));
+        HIPCHECK(c, hipMemcpyAsync((int *)c->dh_meta.p + I.dh_off, dm.data(), nd * 4, hipMemcpyHostToDevice, s));
+    }
+    HIPCHECK(c, hipStreamSynchronize(s));
+    return PDGPU_OK;
+}
+
+extern "C" int64_t pdgpu_bonded_bytes(int32_t nbonds, int32_t nangles, int32_t ntorsions, int32_t nimpropers) {
+    if (nbonds < 0 || nangles < 0 || ntorsions < 0 || nimpropers < 0) return -1;
+    return 32 * (int64_t)nbonds + 32 * (int64_t)nangles + 116 * ((int64_t)ntorsions + nimpropers);
 }
 
 extern "C" int pdgpu_set_positions(pdgpu_ctx *c, int32_t id, const double *xyz) {

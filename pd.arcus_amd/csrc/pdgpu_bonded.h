@@ -1,0 +1,215 @@
+// pdgpu_bonded.h -- FF_Bonded on the device (forcefields/ffbonded.cpp:809-1590):
+// harmonic bonds, harmonic angles, Fourier / harmonic dihedrals for
+// torsions and impropers. One work item per term; every term reads its
+// 2-4 atoms' FP64 positions and adds its gradient to the same per-atom
+// accumulators as the pair kernels (64-bit fixed point in the mixed path,
+// FP64 atomics in the FP64 path), so bonded and GB forces come out of one
+// k_finalize. The arithmetic is FP64 in both precisions: the terms are
+// O(atoms) and their cost is the memory traffic, not the maths.
+#pragma once
+
+#include "pdgpu_device.h"
+
+namespace pdgpu {
+
+// component bits (pdgpu_options::components)
+constexpr int kFFGB = 1, kFFBonds = 2, kFFAngles = 4, kFFTorsions = 8, kFFImpropers = 16;
+
+// Term records. Atom indices are global (padded layout); the owning
+// instance is stored as slot + 1 so that zeroed memory is a free record.
+struct BondedState {
+    const int4 *bd_idx;      // (i, j, slot+1, 0)
+    const double2 *bd_par;   // (l [A], k [kcal/mol/A^2])
+    const int4 *an_idx;      // (i, apex a, j, slot+1)
+    const double2 *an_par;   // (theta0 [rad], k [kcal/mol/rad^2])
+    const int4 *dh_idx;      // (i, a, b, j)
+    const int *dh_meta;      // (slot+1) | terms << 24 | improper << 28
+    const double4 *dh_vn;    // Vn [kcal/mol], 4 terms
+    const double4 *dh_n;     // periodicity n (0: harmonic)
+    const double4 *dh_gamma; // phase gamma [rad]
+    int b0, b1, a0, a1, d0, d1; // record ranges evaluated
+    int comps;
+};
+
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 d3_sub(const D3 &a, const D3 &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 d3_cross(const D3 &a, const D3 &b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double d3_dot(const D3 &a, const D3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 d3_lin(double s, const D3 &a, double t, const D3 &b) {
+    return {s * a.x + t * b.x, s * a.y + t * b.y, s * a.z + t * b.z};
+}
+__device__ __forceinline__ D3 d3_at(const DevState &S, int a) { return {S.px[a], S.py[a], S.pz[a]}; }
+
+// Adds gradient g (kcal/mol/A, dE/dx) of atom a.
+template <typename R>
+__device__ __forceinline__ void grad_add(const DevState &S, int a, const D3 &g, int *st) {
+    acc_add<R>(S.acc_fx, a, g.x, kForceScale, st);
+    acc_add<R>(S.acc_fy, a, g.y, kForceScale, st);
+    acc_add<R>(S.acc_fz, a, g.z, kForceScale, st);
+}
+
+// Per-instance energy slot += e for every valid lane. Called by the whole
+// wavefront. In the mixed path each term's energy is rounded to fixed point
+// on its own and the integers are summed, so an instance's total does not
+// depend on which other terms share its wavefront.
+template <typename R>
+__device__ void add_term_energy(const DevState &S, bool valid, int inst, int slot, double e) {
+    const unsigned long long vm = __ballot(valid);
+    if (!vm) return;
+    const int lead = __ffsll((unsigned long long)vm) - 1;
+    const int key = valid ? inst * kENum + slot : -1;
+    const int key0 = __shfl(key, lead, 64);
+    const bool same = __ballot(valid && key != key0) == 0;
+    if constexpr (Prec<R>::fixed) {
+        long long iv = 0;
+        if (valid) {
+            const double s = e * kEnergyScale;
+            if (fabs(s) < kFixedLimit) iv = __double2ll_rn(s);
+            else atomicOr(S.inst_status + inst, (s == s) ? kStatOverflow : kStatNonFinite);
+        }
+        unsigned long long *acc = reinterpret_cast<unsigned long long *>(S.inst_energy);
+        if (same) {
+            const long long tot = wave_sum(iv);
+            if (lane_id() == lead && tot != 0) atomicAdd(acc + key0, (unsigned long long)tot);
+        } else if (valid && iv != 0) {
+            atomicAdd(acc + key, (unsigned long long)iv);
+        }
+    } else {
+        double *acc = reinterpret_cast<double *>(S.inst_energy);
+        if (valid && !(fabs(e) < 1e300)) atomicOr(S.inst_status + inst, kStatNonFinite);
+        if (same) {
+            const double tot = wave_sum(valid ? e : 0.0);
+            if (lane_id() == lead && tot != 0.0) unsafeAtomicAdd(acc + key0, tot);
+        } else if (valid && e != 0.0) {
+            unsafeAtomicAdd(acc + key, e);
+        }
+    }
+}
+
+// Dihedral i-a-b-j. phi is the IUPAC angle, which is what the reference's
+// phi = -atan2(sin, cos) over its normals evaluates to
+// (forcefields/ffbonded.cpp:1269-1296). The gradient uses the closed form of
+// dphi/dx (Blondel & Karplus, J. Comput. Chem. 17:1132, 1996) instead of the
+// reference's two-branch cos/sin derivative; both are the exact gradient.
+template <typename R>
+__device__ double dihedral_term(const DevState &S, const int4 &q, int terms, const double4 &vn, const double4 &nn,
+                                const double4 &gm, int *st) {
+    const D3 ri = d3_at(S, q.x), ra = d3_at(S, q.y), rb = d3_at(S, q.z), rj = d3_at(S, q.w);
+    const D3 b1 = d3_sub(ra, ri), b2 = d3_sub(rb, ra), b3 = d3_sub(rj, rb);
+    const D3 m = d3_cross(b1, b2), n = d3_cross(b2, b3);
+    const double m2 = d3_dot(m, m), n2 = d3_dot(n, n), b22 = d3_dot(b2, b2);
+    const double lb2 = sqrt(b22);
+    const double phi = atan2(lb2 * d3_dot(b1, n), d3_dot(m, n));
+    const double V[4] = {vn.x, vn.y, vn.z, vn.w}, N[4] = {nn.x, nn.y, nn.z, nn.w}, G[4] = {gm.x, gm.y, gm.z, gm.w};
+    double e = 0, dedphi = 0;
+    for (int t = 0; t < 4; ++t) {
+        if (t >= terms) break;
+        if (N[t] > 0) {
+            double s, c;
+            sincos(N[t] * phi + G[t], &s, &c);
+            e += V[t] * (1.0 + c);
+            dedphi -= N[t] * V[t] * s;
+        } else {
+            double d = phi - G[t];
+            if (d < -M_PI) d += 2.0 * M_PI;
+            else if (d > M_PI) d -= 2.0 * M_PI;
+            e += V[t] * d * d;
+            dedphi += 2.0 * V[t] * d;
+        }
+    }
+    const D3 gi = d3_lin(-dedphi * lb2 / m2, m, 0.0, m);
+    const D3 gj = d3_lin(dedphi * lb2 / n2, n, 0.0, n);
+    const double p1 = d3_dot(b1, b2) / b22, p3 = d3_dot(b3, b2) / b22;
+    grad_add<R>(S, q.x, gi, st);
+    grad_add<R>(S, q.y, d3_lin(-(1.0 + p1), gi, p3, gj), st);
+    grad_add<R>(S, q.z, d3_lin(-(1.0 + p3), gj, p1, gi), st);
+    grad_add<R>(S, q.w, gj, st);
+    return e;
+}
+
+// One launch over the record ranges [b0,b1) bonds, [a0,a1) angles,
+// [d0,d1) dihedrals, one thread per record.
+template <typename R>
+__global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nb = B.b1 - B.b0, na = B.a1 - B.a0, nd = B.d1 - B.d0;
+    bool valid = false;
+    int inst = -1, slot = kEBond;
+    double e = 0, vir = 0;
+    if (t < nb) {
+        const int4 id = B.bd_idx[B.b0 + t];
+        if (id.z > 0 && (B.comps & kFFBonds)) {
+            // calcBondForces (ffbonded.cpp:885-940): E = k(d-l)^2
+            valid = true;
+            inst = id.z - 1;
+            int *st = S.inst_status + inst;
+            const double2 p = B.bd_par[B.b0 + t];
+            const D3 v = d3_sub(d3_at(S, id.x), d3_at(S, id.y));
+            const double d = sqrt(d3_dot(v, v));
+            const double dd = d - p.x;
+            e = p.y * dd * dd;
+            const double dedd = 2.0 * p.y * dd;
+            vir = dedd * d; // InternalVirial += d * force (ffbonded.cpp:924)
+            const D3 g = d3_lin(dedd / d, v, 0.0, v);
+            grad_add<R>(S, id.x, g, st);
+            grad_add<R>(S, id.y, d3_lin(-1.0, g, 0.0, g), st);
+        }
+    } else if (t < nb + na) {
+        const int r = B.a0 + (t - nb);
+        const int4 id = B.an_idx[r];
+        const double2 p = B.an_par[r];
+        if (id.w > 0 && (B.comps & kFFAngles) && p.y != 0.0) {
+            // calcAngleForces (ffbonded.cpp:994-1120): E = k(theta-theta0)^2
+            valid = true;
+            inst = id.w - 1;
+            slot = kEAngle;
+            int *st = S.inst_status + inst;
+            const D3 ra = d3_at(S, id.y);
+            const D3 vi = d3_sub(d3_at(S, id.x), ra), vj = d3_sub(d3_at(S, id.z), ra);
+            const double li = 1.0 / sqrt(d3_dot(vi, vi)), lj = 1.0 / sqrt(d3_dot(vj, vj));
+            const D3 ui = d3_lin(li, vi, 0.0, vi), uj = d3_lin(lj, vj, 0.0, vj);
+            const double c = d3_dot(ui, uj);
+            const double th = acos(c), sn = sin(th);
+            const double dth = th - p.x;
+            e = p.y * dth * dth;
+            const double dedth = 2.0 * p.y * dth;
+            // dtheta/dx_i = (u_i cos - u_j) / (|v_i| sin)
+            const D3 gi = d3_lin(dedth * li / sn * c, ui, -dedth * li / sn, uj);
+            const D3 gj = d3_lin(dedth * lj / sn * c, uj, -dedth * lj / sn, ui);
+            grad_add<R>(S, id.x, gi, st);
+            grad_add<R>(S, id.z, gj, st);
+            grad_add<R>(S, id.y, d3_lin(-1.0, gi, -1.0, gj), st);
+        }
+    } else if (t < nb + na + nd) {
+        const int r = B.d0 + (t - nb - na);
+        const int meta = B.dh_meta[r];
+        const int s1 = meta & 0xFFFFFF, terms = (meta >> 24) & 7, impr = (meta >> 28) & 1;
+        if (s1 > 0 && (B.comps & (impr ? kFFImpropers : kFFTorsions))) {
+            // calcDihedralForces (ffbonded.cpp:1199-1440)
+            valid = true;
+            inst = s1 - 1;
+            slot = impr ? kEImproper : kETorsion;
+            e = dihedral_term<R>(S, B.dh_idx[r], terms, B.dh_vn[r], B.dh_n[r], B.dh_gamma[r], S.inst_status + inst);
+        }
+    }
+    add_term_energy<R>(S, valid, inst, slot, e);
+    add_term_energy<R>(S, valid && t < nb, inst, kEVirial, vir);
+}
+
+// Clears the force accumulators where the GB kernels (which clear them in
+// k_born_final) do not run.
+template <typename R>
+__global__ void k_clear_grad(DevState S, int atom_begin, int atom_end) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= atom_end) return;
+    acc_store<R>(S.acc_fx, a, 0.0, 1.0);
+    acc_store<R>(S.acc_fy, a, 0.0, 1.0);
+    acc_store<R>(S.acc_fz, a, 0.0, 1.0);
+    acc_store<R>(S.acc_deda, a, 0.0, 1.0);
+}
+
+} // namespace pdgpu

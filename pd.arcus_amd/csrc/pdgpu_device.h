@@ -30,8 +30,9 @@ constexpr int kTileDiag = 1;   // I == J: only j < i pairs
 constexpr int kTileNear = 2;   // some pair has index gap < max_index_delta: band lookups
 constexpr int kTileSparse = 4; // some off-diagonal bond order falls in the tile
 
-// per-instance energy record slots
-enum { kEVdw = 0, kEElec, kEPolCross, kEPolSelf, kEVirial, kENum = 8 };
+// per-instance energy record slots; the potential energy is the sum of
+// slots kEVdw .. kEImproper
+enum { kEVdw = 0, kEElec, kEPolCross, kEPolSelf, kEBond, kEAngle, kETorsion, kEImproper, kEVirial, kENum = 10 };
 enum { kCPairs = 0, kCBornPairs, kCNum = 2 };
 // per-instance status bits
 constexpr int kStatNonFinite = 1;

@@ -33,7 +33,7 @@ enum { kActNone = 0, kActMove = 1, kActRestore = 2, kActSave = 3 };
 template <typename R>
 __device__ __forceinline__ double inst_epot_kcal(const DevState &S, int s) {
     double e = 0;
-    for (int k = kEVdw; k <= kEPolSelf; ++k) e += acc_read<R>(S.inst_energy, (size_t)s * kENum + k, kEnergyScale);
+    for (int k = kEVdw; k <= kEImproper; ++k) e += acc_read<R>(S.inst_energy, (size_t)s * kENum + k, kEnergyScale);
     return e;
 }
 

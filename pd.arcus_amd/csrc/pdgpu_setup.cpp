@@ -41,6 +41,7 @@ extern "C" void pdgpu_options_default(pdgpu_options *o) {
     o->device_memory_budget = 0;
     o->max_instances = 0;
     o->tiles_per_wave = 0;
+    o->components = PDGPU_FF_ALL;
 }
 
 // BondOrder::reinit (workspace/bondorder.cpp:77-195). The band holds the

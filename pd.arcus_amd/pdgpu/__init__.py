@@ -46,8 +46,13 @@ EXPORTS = (
     "pdgpu_minimise_sd", "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_md_vv",
     "pdgpu_bondorder_build", "pdgpu_fixed_terms", "pdgpu_instance_bytes", "pdgpu_bytes_free",
     "pdgpu_profile", "pdgpu_profile_read", "pdgpu_sd_start", "pdgpu_sd_advance", "pdgpu_sd_finish",
+    "pdgpu_bonded_set", "pdgpu_bonded_bytes",
 )
-KERNEL_GROUPS = ("nlist", "atom", "born", "pair", "chain", "integrate", "enum")
+# forcefield components (pdgpu_options.components)
+FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
+FF_BONDED = FF_BONDS | FF_ANGLES | FF_TORSIONS | FF_IMPROPERS
+FF_ALL = FF_GB | FF_BONDED
+KERNEL_GROUPS = ("nlist", "atom", "born", "pair", "chain", "integrate", "enum", "bonded")
 
 
 class PdgpuError(RuntimeError):
@@ -64,7 +69,20 @@ class Options(C.Structure):
                    "elec14_scaling", "dielectric_solute", "dielectric_solvent", "gbsa_still_cutoff",
                    "exp_approx_threshold", "dielectric_offset", "born_radius_offset",
                    "p1", "p2", "p3", "p4", "p5", "nlist_padding")] + \
-               [("device_memory_budget", C.c_int64), ("max_instances", C.c_int32), ("reserved", C.c_int32)]
+               [("device_memory_budget", C.c_int64), ("max_instances", C.c_int32), ("components", C.c_int32)]
+
+
+class Bonded(C.Structure):
+    _fields_ = [("nbonds", C.c_int32), ("nangles", C.c_int32), ("ntorsions", C.c_int32), ("nimpropers", C.c_int32),
+                ("bond_ij", _ip), ("bond_lk", _dp), ("angle_iaj", _ip), ("angle_tk", _dp),
+                ("torsion_iabj", _ip), ("torsion_terms", _ip), ("torsion_vn", _dp), ("torsion_n", _dp),
+                ("torsion_gamma", _dp), ("improper_iabj", _ip), ("improper_terms", _ip), ("improper_vn", _dp),
+                ("improper_n", _dp), ("improper_gamma", _dp)]
+
+
+# keys of a bonded-term dict (the layout of pdgpu.synth and of the reference fixtures)
+BONDED_KEYS = ("bd_idx", "bd_lk", "an_idx", "an_tk", "to_idx", "to_terms", "to_vn", "to_n", "to_gamma",
+               "im_idx", "im_terms", "im_vn", "im_n", "im_gamma")
 
 
 class Topology(C.Structure):
@@ -77,7 +95,8 @@ class Topology(C.Structure):
 
 class Energies(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
-        "epot", "epot_vdw", "epot_elec", "epot_pol", "epot_pol_self", "epot_pol_cross", "virial")] + \
+        "epot", "epot_vdw", "epot_elec", "epot_pol", "epot_pol_self", "epot_pol_cross", "virial",
+        "epot_bond", "epot_angle", "epot_torsion", "epot_improper")] + \
                [("pairs", C.c_int64), ("born_pairs", C.c_int64), ("status", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self):
@@ -120,6 +139,8 @@ def _load():
         "pdgpu_sd_advance": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip]),
         "pdgpu_sd_finish": (C.c_int, [C.c_void_p, C.c_int32, P(Energies), _ip]),
         "pdgpu_profile_read": (C.c_int, [C.c_void_p, _dp, P(C.c_int64), C.c_int32]),
+        "pdgpu_bonded_set": (C.c_int, [C.c_void_p, C.c_int32, P(Bonded)]),
+        "pdgpu_bonded_bytes": (C.c_int64, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -245,11 +266,32 @@ class Context:
             self.set_positions(i.value, xyz)
         return i.value
 
-    def add_system(self, s, xyz=None):
-        """s: mapping with PD per-atom arrays (see pdgpu.synth / golden fixtures)."""
-        return self.add(s["charge"], s["radius"], s["epsilon"], s["gb_radius"], s["vj"], s["terms123"],
-                        s["mid"], s["band"], s.get("mass"), s.get("sp_i"), s.get("sp_j"), s.get("sp_o"),
-                        xyz=s["xyz"] if xyz is None else xyz)
+    def add_system(self, s, xyz=None, bonded=False):
+        """s: mapping with PD per-atom arrays (see pdgpu.synth / golden fixtures).
+        bonded=True also uploads its FF_Bonded term lists (BONDED_KEYS)."""
+        i = self.add(s["charge"], s["radius"], s["epsilon"], s["gb_radius"], s["vj"], s["terms123"],
+                     s["mid"], s["band"], s.get("mass"), s.get("sp_i"), s.get("sp_j"), s.get("sp_o"),
+                     xyz=s["xyz"] if xyz is None else xyz)
+        if bonded:
+            self.set_bonded(i, s)
+        return i
+
+    def set_bonded(self, iid, d):
+        """FF_Bonded term lists (J/molecule units) from a mapping with BONDED_KEYS."""
+        fi = lambda k: _i32(np.asarray(d[k]).ravel())
+        fd = lambda k: _f64(np.asarray(d[k]).ravel())
+        a = {k: (fi(k) if k.endswith(("idx", "terms")) else fd(k)) for k in BONDED_KEYS}
+        b = Bonded(len(a["bd_lk"]) // 2, len(a["an_tk"]) // 2, len(a["to_terms"]), len(a["im_terms"]),
+                   _p(a["bd_idx"], _ip), _p(a["bd_lk"], _dp), _p(a["an_idx"], _ip), _p(a["an_tk"], _dp),
+                   _p(a["to_idx"], _ip), _p(a["to_terms"], _ip), _p(a["to_vn"], _dp), _p(a["to_n"], _dp),
+                   _p(a["to_gamma"], _dp), _p(a["im_idx"], _ip), _p(a["im_terms"], _ip), _p(a["im_vn"], _dp),
+                   _p(a["im_n"], _dp), _p(a["im_gamma"], _dp))
+        self._check(lib.pdgpu_bonded_set(self.h, iid, C.byref(b)), "bonded_set")
+
+    @staticmethod
+    def bonded_bytes(d):
+        return lib.pdgpu_bonded_bytes(len(d["bd_lk"]) // 2, len(d["an_tk"]) // 2, len(d["to_terms"]),
+                                      len(d["im_terms"]))
 
     def remove(self, iid):
         self._check(lib.pdgpu_instance_remove(self.h, iid), "instance_remove")

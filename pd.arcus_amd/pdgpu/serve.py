@@ -1,6 +1,7 @@
 """Continuous-batching minimisation server on one GPU.
 
-Requests (molecules to minimise) arrive over time. Each tick the server
+Requests (molecules to minimise, with their FF_Bonded term lists when they
+carry them) arrive over time. Each tick the server
   1. admits waiting requests, first come first served, while the resident
      set stays inside the device-memory budget (pdgpu_instance_bytes);
   2. advances every resident minimisation by `steps_per_tick` steepest-
@@ -53,7 +54,8 @@ class MinimisationServer:
         self.ctx.close()
 
     def _bytes(self, mol):
-        return self.ctx.instance_bytes(len(mol["charge"]), int(mol["mid"]), len(mol.get("sp_i", [])))
+        b = self.ctx.instance_bytes(len(mol["charge"]), int(mol["mid"]), len(mol.get("sp_i", [])))
+        return b + (Context.bonded_bytes(mol) if "bd_idx" in mol else 0)
 
     def try_admit(self, req, now):
         need = self._bytes(req.mol)
@@ -61,7 +63,7 @@ class MinimisationServer:
             raise PdgpuError(PDGPU_ERR_NOMEM, f"request {req.rid} needs {need} B, budget {self.budget} B")
         if self.resident_bytes + need > self.budget:
             return False
-        iid = self.ctx.add_system(req.mol)
+        iid = self.ctx.add_system(req.mol, bonded="bd_idx" in req.mol)
         self.ctx.sd_start(iid, self.max_steps, self.step_size, self.update_nlist, self.grms_tol)
         req.iid, req.admitted = iid, now
         self.resident[iid] = req

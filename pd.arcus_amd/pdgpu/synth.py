@@ -426,6 +426,67 @@ def _pairs(n, bonds):
     return f(p12), f(p13), f(p14)
 
 
+# FF_Bonded parameters of the generic forcefield (kcal/mol, A, rad): bond
+# and angle force constants by whether a hydrogen takes part; equilibrium
+# lengths and angles are the built geometry's. Torsions: amide C-N bonds a
+# 2-fold term that keeps the peptide plane, every other rotatable bond a
+# weak 3-fold term; impropers hold 3-coordinated C and N planar.
+K_BOND_HEAVY, K_BOND_H = 300.0, 340.0
+K_ANGLE_HEAVY, K_ANGLE_H = 60.0, 35.0
+V_AMIDE, V_SP3 = 2.5, 0.15
+V_IMPROPER_C, V_IMPROPER_N = 10.5, 1.1
+
+
+def bonded_terms(xyz, el, bonds):
+    """FF_Bonded term lists (BONDED_KEYS layout, PD units) for a molecule."""
+    n = len(xyz)
+    kj = J_PER_KCALMOL
+    nb = [[] for _ in range(n)]
+    for a, b in bonds:
+        nb[a].append(int(b))
+        nb[b].append(int(a))
+    nb = [sorted(v) for v in nb]
+    isH = np.array([e.startswith("H") for e in el])
+    p12 = np.array(sorted({tuple(sorted((int(a), int(b)))) for a, b in bonds}), dtype=np.int32).reshape(-1, 2)
+    l = np.linalg.norm(xyz[p12[:, 0]] - xyz[p12[:, 1]], axis=1)
+    kb = np.where(isH[p12[:, 0]] | isH[p12[:, 1]], K_BOND_H, K_BOND_HEAVY) * kj
+    ang = [(i, c, j) for c in range(n) for x, i in enumerate(nb[c]) for j in nb[c][x + 1:]]
+    ang = np.array(ang, dtype=np.int32).reshape(-1, 3)
+    u = xyz[ang[:, 0]] - xyz[ang[:, 1]]
+    v = xyz[ang[:, 2]] - xyz[ang[:, 1]]
+    th0 = np.arccos(np.clip((u * v).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1), -1, 1))
+    ka = np.where(isH[ang[:, 0]] | isH[ang[:, 2]], K_ANGLE_H, K_ANGLE_HEAVY) * kj
+    amide_c = {c for c in range(n) if el[c] == "C" and any(el[o] == "O" and len(nb[o]) == 1 for o in nb[c])}
+    tors, tv, tn, tg = [], [], [], []
+    for b, c in p12:
+        amide = (b in amide_c and el[c] == "N") or (c in amide_c and el[b] == "N")
+        for a in nb[b]:
+            if a == c:
+                continue
+            for d in nb[c]:
+                if d == b or d == a:
+                    continue
+                tors.append((a, b, c, d))
+                tv.append(V_AMIDE if amide else V_SP3)
+                tn.append(2.0 if amide else 3.0)
+                tg.append(np.pi if amide else 0.0)
+    m = len(tors)
+    z4 = lambda vals: np.concatenate([np.asarray(vals, dtype=float).reshape(-1, 1), np.zeros((len(vals), 3))], 1)
+    imp, iv = [], []
+    for c in range(n):
+        if len(nb[c]) == 3 and el[c] in ("C", "N"):
+            x, y, w = nb[c]
+            imp.append((x, y, c, w))
+            iv.append(V_IMPROPER_C if el[c] == "C" else V_IMPROPER_N)
+    k = len(imp)
+    return {"bd_idx": p12.ravel(), "bd_lk": np.stack([l, kb], 1).ravel(),
+            "an_idx": ang.ravel(), "an_tk": np.stack([th0, ka], 1).ravel(),
+            "to_idx": np.array(tors, dtype=np.int32).ravel(), "to_terms": np.ones(m, np.int32),
+            "to_vn": (z4(tv) * kj).ravel(), "to_n": z4(tn).ravel(), "to_gamma": z4(tg).ravel(),
+            "im_idx": np.array(imp, dtype=np.int32).ravel(), "im_terms": np.ones(k, np.int32),
+            "im_vn": (z4(iv) * kj).ravel(), "im_n": z4([2.0] * k).ravel(), "im_gamma": z4([np.pi] * k).ravel()}
+
+
 def _finish(xyz, el, q, resid, bonds):
     n = len(xyz)
     p12, p13, p14 = _pairs(n, bonds)
@@ -441,26 +502,30 @@ def _finish(xyz, el, q, resid, bonds):
     return {"xyz": xyz, "charge": q, "radius": rad, "epsilon": eps, "gb_radius": gbr, "mass": mass,
             "resid": resid, "a12": p12[:, 0], "b12": p12[:, 1], "d12": d12, "a13": p13[:, 0], "b13": p13[:, 1],
             "d13": d13, "a14": p14[:, 0], "b14": p14[:, 1], "mid": mid, "band": band, "sp_i": si, "sp_j": sj,
-            "sp_o": so, "vj": vj, "terms123": t123, "n": n}
+            "sp_o": so, "vj": vj, "terms123": t123, "n": n, "el": list(el), "bonds": np.asarray(bonds)}
 
 
-def peptide(seed, residues, mode="extended"):
-    """One molecule of `residues` residues with a seeded random sequence."""
+def peptide(seed, residues, mode="extended", bonded=False):
+    """One molecule of `residues` residues with a seeded random sequence;
+    bonded=True adds FF_Bonded term lists (bonded_terms)."""
     seq = random_sequence(seed, residues)
-    return build(seq, seed=seed, mode=mode)
+    m = build(seq, seed=seed, mode=mode)
+    if bonded:
+        m.update(bonded_terms(m["xyz"], m["el"], m["bonds"]))
+    return m
 
 
 def _peptide_job(args):
-    seed, residues, mode = args
-    return peptide(seed, residues, mode)
+    seed, residues, mode, bonded = args
+    return peptide(seed, residues, mode, bonded)
 
 
-def many(seeds, residues, mode="extended", workers=None):
+def many(seeds, residues, mode="extended", workers=None, bonded=False):
     """Builds many molecules in parallel worker processes (call this before
     any GPU context exists in the process: workers are forked)."""
     import multiprocessing as mp
     import os
-    jobs = list(zip(seeds, residues, [mode] * len(seeds)))
+    jobs = list(zip(seeds, residues, [mode] * len(seeds), [bonded] * len(seeds)))
     workers = workers or min(16, os.cpu_count() or 1)
     if workers <= 1 or len(jobs) < 8:
         return [_peptide_job(j) for j in jobs]

@@ -158,7 +158,7 @@ def test_continuous_batching_same_alone_and_in_mix():
 
 
 def test_server_poisson_requests_complete_and_match():
-    mols = [synth.peptide(seed=60 + k, residues=int(r)) for k, r in
+    mols = [synth.peptide(seed=60 + k, residues=int(r), bonded=True) for k, r in
             enumerate(np.random.default_rng(3).integers(6, 25, size=10))]
     reqs = [serve.Request(k, m, float(t)) for k, (m, t) in
             enumerate(zip(mols, serve.poisson_arrivals(400.0, len(mols), seed=4)))]
@@ -169,14 +169,17 @@ def test_server_poisson_requests_complete_and_match():
     finally:
         srv.close()
     assert len(out) == len(reqs)
+    assert not any(r.failed for r in out)
     for r in out[:3]:
         with pdgpu.Context(opts(pdgpu.MIXED)) as ctx:
-            i = ctx.add_system(r.mol)
+            i = ctx.add_system(r.mol, bonded=True)
             st = ctx.minimise_sd(i, 60, 2e5, 1, 0.1)
             e, _ = ctx.calc_forces(i)
+            x = ctx.get_positions(i)
         assert st == r.steps
-        # same bits alone and in the mix (a collapsed GB-only run is NaN in both)
-        assert np.float64(e["epot"]).tobytes() == np.float64(r.energies["epot"]).tobytes()
+        # same bits alone and in the mix
+        assert e["epot"] == r.energies["epot"]
+        assert np.array_equal(x, r.xyz)
 
 
 def test_admission_respects_budget():
