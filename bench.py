@@ -1,21 +1,30 @@
 """Benchmark of the MI355X GB / nonbonded hot path.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload md|batch]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--sections batch,md,serving]
 
-Default workload (md): one 500-residue peptide-like chain (~8,000 atoms),
-backbone phi/psi drawn 60/40 from the alpha / beta basins, clashes relieved
-by 100 steepest-descent steps, then velocity-Verlet MD at dt = 1 fs from
-300 K Maxwell velocities: 12 A cutoff (9 A inner), 2 A list skin with a
-rebuild every 10 steps, Still GB with eps_out 78.5. A step is one MD step:
-half kick + drift, Born radii, fused LJ + Coulomb + GB pair pass, chain
-rule, half kick. The mixed-precision path is timed for `value`; the FP64
-path runs the same steps for comparison.
+Headline (`value`): the batched path. 4,096 independent extended
+conformers of 10-150 residues (~5.3 M atoms), each with its own tile list
+(10 A cutoff, 7 A inner, 2 A skin) and Still GB (eps_out 78.5), all
+evaluated together. A step is one energy + force evaluation of every
+conformer: pair enumeration, Born radii, fused LJ + Coulomb + GB pair
+pass, chain rule. `value` is atom-pair evaluations per second (unordered
+pairs within the cutoff, counted on the device) in mixed precision; the
+FP64 path runs the same steps beside it.
 
-batch: 4,096 independent extended conformers of 10-150 residues
-(~2.6 M atoms, 10 A cutoff), one energy + force evaluation of all of them
-per step.
+Sections added to the same JSON line:
+  md       one 500-residue chain (~8,200 atoms, phi/psi from the alpha /
+           beta basins, bonded terms + GB), 100 steepest-descent steps of
+           clash relief, then velocity-Verlet MD at 1 fs from 300 K, 12 A
+           cutoff, 2 A skin, list rebuilt every 10 steps; mixed and FP64.
+  serving  minimisation requests with Poisson arrivals at several rates,
+           sizes log-uniform in 20-400 residues (extended build, bonded +
+           GB, 10 A cutoff); steepest descent until the force RMS is below
+           0.1 kcal/mol/A or 1,000 steps; continuous batching with
+           admission by device-memory budget; p50/p99 latency and
+           minimisations/s.
 
-Prints one JSON line. Inputs are synthetic (pd.arcus_amd/pdgpu/synth.py).
+Inputs are synthetic (pd.arcus_amd/pdgpu/synth.py) and resident in HBM
+when the timed region starts. Prints one JSON line.
 """
 import argparse
 import json
@@ -56,157 +65,221 @@ def maxwell(mass, T, seed):
 
 
 def gpu_md(precision, mol, xyz0, vel0, steps, warmup, profile, pdgpu):
+    """Runs warmup + steps MD steps; returns the timed steps' wall time, the
+    energies before and after them and a profile of a further short run."""
     ctx = pdgpu.Context(md_options(precision, pdgpu))
-    i = ctx.add_system(mol, xyz=xyz0)
+    i = ctx.add_system(mol, xyz=xyz0, bonded=True)
     ctx.set_velocities(i, vel0)
-    ctx.md_vv(i, warmup, 1e-15, 10)
+    e0, ek0 = ctx.md_vv(i, max(1, warmup), 1e-15, 10)
     t0 = time.perf_counter()
     e, ek = ctx.md_vv(i, steps, 1e-15, 10)
     dt = time.perf_counter() - t0
     prof = None
     if profile:
         ctx.profile(True)
-        e2, _ = ctx.md_vv(i, steps, 1e-15, 10)
+        ctx.md_vv(i, min(steps, 1000), 1e-15, 10)
         prof = ctx.profile_read()
         ctx.profile(False)
     ctx.close()
-    return dt, e, ek, prof
+    return dt, (e0, ek0), (e, ek), prof
 
 
-def cpu_baseline_md(mol, xyz, params_kw, pairs_per_eval, budget_s=12.0):
-    """The oracle (tests/oracle_lib.py, a scalar C port) timed on the host on a
-    bounded sample: repeated single-point evaluations (list build + Born radii
-    + pair pass + chain rule) of the same structure for ~budget_s seconds."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    p = O.default_params(**params_kw)
-    s = O.OSystem(mol["charge"], mol["radius"], mol["epsilon"], mol["gb_radius"], mol["vj"], mol["terms123"],
-                  mol["mass"], mol["mid"], mol["band"], mol["sp_i"], mol["sp_j"], mol["sp_o"])
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        rs, ent = O.nlist_build(s, xyz, max(p.cutoff, p.vdw_cutoff), p.padding, True)
-        O.gb_calc_forces(s, xyz, rs, ent, p)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": pairs_per_eval * n / dt, "unit": "atom-pair evals/s", "cores": 1, "kind": "port",
-            "sample": f"{n} single-point evaluations (list build + Born + pair pass + chain rule) of the "
-                      f"{len(mol['charge'])}-atom md structure, {dt:.1f} s, oracle C port, 1 thread"}
+def _oracle_pairs(O, s, xyz, rs, ent, cutoff):
+    """Unordered pairs within the cutoff in an oracle neighbour list."""
+    j = (ent & 0xFFFFFF).astype(np.int64)
+    i = np.repeat(np.arange(len(rs) - 1), np.diff(rs))
+    keep = j > i
+    d2 = ((xyz[i[keep]] - xyz[j[keep]]) ** 2).sum(1)
+    return int((d2 <= cutoff * cutoff).sum())
 
 
-def cpu_baseline_batch(mols, pairs, budget_s=12.0):
-    """The oracle (scalar C port, 1 thread) on a bounded sample of the batch:
-    conformers in order, each a list build + Born + pair pass + chain rule,
-    until ~budget_s seconds have passed."""
+def _cpu_worker(job):
+    """Oracle evaluations of conformers job[0] (list of (mol, ...)) for
+    job[1] seconds: list build + Born radii + pair pass + chain rule each."""
+    mols, budget = job
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     p = O.default_params(cutoff=10.0, inner_cutoff=7.0, vdw_cutoff=10.0, vdw_inner_cutoff=7.0, vdw14_scaling=0.5,
                          elec14_scaling=0.83333, dielectric_solvent=78.5, padding=2.0)
-    done = 0
-    npairs = 0
-    t0 = time.perf_counter()
-    for m, pc in zip(mols, pairs):
+    done, npairs, busy = 0, 0, 0.0
+    t_end = time.perf_counter() + budget
+    for m in mols:
         s = O.OSystem(m["charge"], m["radius"], m["epsilon"], m["gb_radius"], m["vj"], m["terms123"], m["mass"],
                       m["mid"], m["band"], m["sp_i"], m["sp_j"], m["sp_o"])
+        t0 = time.perf_counter()
         rs, ent = O.nlist_build(s, m["xyz"], 10.0, 2.0, True)
         O.gb_calc_forces(s, m["xyz"], rs, ent, p)
+        busy += time.perf_counter() - t0
         done += 1
-        npairs += pc
-        if time.perf_counter() - t0 > budget_s:
+        npairs += _oracle_pairs(O, s, m["xyz"], rs, ent, 10.0)
+        if time.perf_counter() > t_end:
             break
-    dt = time.perf_counter() - t0
-    return {"value": npairs / dt, "unit": "atom-pair evals/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} of {len(mols)} conformers, one evaluation each (list build + Born + pair "
-                      f"pass + chain rule), {dt:.1f} s, oracle C port, 1 thread"}
+    return done, npairs, busy
 
 
-def roofline(prof, pairs, born_pairs, precision):
-    """Dominant kernel group and its achieved algorithmic FLOP rate."""
-    flops = {"pair": FLOPS_PAIR * pairs, "born": FLOPS_BORN * born_pairs, "chain": FLOPS_CHAIN * born_pairs,
-             "enum": 0}
-    name = max(("pair", "born", "chain", "enum"), key=lambda k: prof[k][0])
+def cpu_baseline_batch(mols, single_s=10.0, multi_s=6.0):
+    """The oracle (scalar C port of the reference path) on a bounded sample of
+    the batch, on the host: one thread for ~single_s seconds, then one
+    process per core (up to 16, the host share of one GPU) for ~multi_s
+    seconds of wall time. Runs before any GPU context exists (forks)."""
+    import multiprocessing as mp
+    d1, p1, b1 = _cpu_worker((mols, single_s))
+    cores = min(16, os.cpu_count() or 1)
+    chunks = [mols[k::cores] for k in range(cores)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(c, multi_s) for c in chunks])
+    wall = time.perf_counter() - t0
+    dn, pn = sum(r[0] for r in res), sum(r[1] for r in res)
+    return {"value": pn / wall, "unit": "atom-pair evals/s", "cores": cores, "kind": "port",
+            "single_thread_value": p1 / b1,
+            "sample": f"oracle C port (tests/oracle_lib.py), one evaluation (list build + Born + pair pass + "
+                      f"chain rule) per conformer of the batch: {d1} conformers on 1 thread in {b1:.1f} s; "
+                      f"{dn} conformers on {cores} processes in {wall:.1f} s wall"}
+
+
+KERNEL_NAMES = {"pair": "pdgpu::k_pair<{R}, 0>", "born": "pdgpu::k_born<{R}>", "chain": "pdgpu::k_chain<{R}>",
+                "enum": "pdgpu::k_enum<{R}>"}
+
+
+def roofline(prof, pairs, born_pairs, precision, traffic=None):
+    """Roofline of the dominant kernel. Algorithmic work per launch: 86 FLOP
+    per unordered pair within the cutoff (pair pass), 20 and 30 FLOP per
+    ordered Still pair (Born sum, chain rule). The enumeration kernel has no
+    algorithmic FLOPs of its own (it selects the pairs), so the FLOP roofline
+    is taken on the dominant of the three pair passes; the enumeration's
+    share of the step is reported beside it. `step` applies the same count
+    to the whole evaluation (all kernels)."""
+    flops = {"pair": FLOPS_PAIR * pairs, "born": FLOPS_BORN * born_pairs, "chain": FLOPS_CHAIN * born_pairs}
+    name = max(flops, key=lambda k: prof[k][0])
     ms, launches = prof[name]
     avg_s = ms / 1e3 / max(1, launches)
     achieved = flops[name] / avg_s / 1e12
     peak = PEAK_FP32_TFLOPS if precision == "mixed" else PEAK_FP64_TFLOPS
-    return {"kernel": f"k_{name}", "bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": None, "avg_launch_us": avg_s * 1e6,
+    kname = KERNEL_NAMES[name].format(R="float" if precision == "mixed" else "double")
+    total_ms = sum(v[0] for k, v in prof.items() if k in ("atom", "born", "pair", "chain", "enum", "bonded"))
+    evals = max(1, prof["pair"][1])
+    step_s = total_ms / 1e3 / evals
+    step_flops = sum(flops.values())
+    return {"kernel": kname, "bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": (traffic or {}).get(kname), "avg_launch_us": avg_s * 1e6,
             "flops_per_launch": flops[name],
+            "step": {"flops": step_flops, "kernel_time_us": step_s * 1e6, "achieved": step_flops / step_s / 1e12,
+                     "frac": step_flops / step_s / 1e12 / peak},
+            "enum_share_of_step": prof["enum"][0] / max(total_ms, 1e-12),
             "groups_ms": {k: round(v[0], 4) for k, v in prof.items()},
             "groups_launches": {k: v[1] for k, v in prof.items()}}
 
 
-def load_traffic(kernel, workload):
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+def load_traffic(name="kernels_batch4096_traffic.json"):
+    """HBM bytes per dispatch by kernel, from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE / WRITE_SIZE scaled by a copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        d = json.load(f)
-    return d.get(kernel)
+        return json.load(f)
 
 
-def run_md(args, pdgpu):
+def build_md():
     from pdgpu import synth
-    log("building 500-residue chain")
-    mol = synth.peptide(seed=2, residues=500, mode="basins")
+    log("md: building the 500-residue chain")
+    return synth.peptide(seed=2, residues=500, mode="basins", bonded=True)
+
+
+def run_md(args, pdgpu, mol):
     n = mol["n"]
-    # clash relief: 100 steepest-descent steps (PD semantics) in mixed precision
+    # clash relief: 100 steepest-descent steps (PD semantics) of the full
+    # forcefield (bonded + GB) in mixed precision
     ctx = pdgpu.Context(md_options(pdgpu.MIXED, pdgpu))
-    i = ctx.add_system(mol)
+    i = ctx.add_system(mol, bonded=True)
     ctx.minimise_sd(i, 100, 2e5, 1)
     xyz = ctx.get_positions(i)
     e0, f0 = ctx.calc_forces(i)
     ctx.close()
     vel = maxwell(mol["mass"], 300.0, seed=7)
-    log(f"relaxed: {n} atoms, {e0['pairs']} pairs in cutoff, epot {e0['epot'] / pdgpu.J_PER_KCALMOL:.1f} kcal/mol")
+    log(f"md: {n} atoms, {e0['pairs']} pairs in cutoff, epot {e0['epot'] / pdgpu.J_PER_KCALMOL:.1f} kcal/mol")
+    steps = args.md_steps
+    res = {}
+    for name, prec in (("mixed", pdgpu.MIXED), ("fp64", pdgpu.FP64)):
+        t, (ea, eka), (eb, ekb), prof = gpu_md(prec, mol, xyz, vel, steps, 10, True, pdgpu)
+        pairs = 0.5 * (ea["pairs"] + eb["pairs"])
+        etot_a, etot_b = ea["epot"] + eka, eb["epot"] + ekb
+        res[name] = {"value": pairs * steps / t, "ms_per_step": t / steps * 1e3, "pairs_per_step": pairs,
+                     "epot_kcal_end": eb["epot"] / pdgpu.J_PER_KCALMOL,
+                     "etot_drift_kcal": (etot_b - etot_a) / pdgpu.J_PER_KCALMOL,
+                     "temperature_end_K": 2.0 * ekb / (3.0 * n * KB),
+                     "roofline": roofline(prof, eb["pairs"], eb["born_pairs"], name)}
+        log(f"md {name}: {steps} steps in {t:.3f} s")
+    m, d = res["mixed"], res["fp64"]
+    return {"workload": f"single 500-residue chain ({n} atoms), velocity-Verlet MD, dt 1 fs, 300 K, "
+                        f"12 A cutoff + Still GB + bonded terms", "steps": steps, "atoms": n,
+            "unit": "atom-pair evals/s", "mixed": m, "fp64": d, "mixed_over_fp64": m["value"] / d["value"],
+            "epot_rel_diff_mixed_vs_fp64_end": abs(m["epot_kcal_end"] - d["epot_kcal_end"]) /
+            abs(d["epot_kcal_end"])}
 
-    t_mix, e_mix, ek_mix, prof = gpu_md(pdgpu.MIXED, mol, xyz, vel, args.steps, args.warmup, True, pdgpu)
-    log(f"mixed: {args.steps} steps in {t_mix:.3f} s")
-    t64, e64, ek64, prof64 = gpu_md(pdgpu.FP64, mol, xyz, vel, args.steps, args.warmup, True, pdgpu)
-    log(f"fp64:  {args.steps} steps in {t64:.3f} s")
-    pairs = e_mix["pairs"]
-    value = pairs * args.steps / t_mix
-    value64 = e64["pairs"] * args.steps / t64
-    rl = roofline(prof, pairs, e_mix["born_pairs"], "mixed")
-    rl["traffic"] = load_traffic(rl["kernel"], "md")
-    rl64 = roofline(prof64, e64["pairs"], e64["born_pairs"], "fp64")
-    out = {
-        "metric": "atom-pair evals/s", "value": value, "unit": "atom-pair evals/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_mix / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32 pair maths, int64 "
-        "fixed-point accumulation, fp64 reductions", "data": "synthetic",
-        "config": {"workload": "single 500-residue chain, velocity-Verlet MD (dt 1 fs, 300 K), 12 A cutoff + "
-                               "Still GB, mixed precision", "atoms": n, "pairs_per_step": pairs,
-                   "born_pairs_per_step": e_mix["born_pairs"], "parallelism": "single GPU"},
-        "fp64": {"value": value64, "ms_per_step": t64 / args.steps * 1e3, "roofline": rl64},
-        "mixed_over_fp64": value / value64,
-        "energy_rel_diff_mixed_vs_fp64_after_md": abs(e_mix["epot"] - e64["epot"]) / abs(e64["epot"]),
-        "roofline": rl,
-    }
-    if not args.no_cpu_baseline:
-        kw = dict(cutoff=12.0, inner_cutoff=9.0, vdw_cutoff=12.0, vdw_inner_cutoff=9.0, vdw14_scaling=0.5,
-                  elec14_scaling=0.83333, dielectric_solvent=78.5, padding=2.0)
-        log("cpu baseline (oracle port)")
-        out["cpu_baseline"] = cpu_baseline_md(mol, xyz, kw, e0["pairs"])
+
+def build_serving(args):
+    from pdgpu import synth
+    rng = np.random.default_rng(17)
+    nreq = args.serve_requests
+    sizes = np.exp(rng.uniform(np.log(20), np.log(400), size=nreq)).astype(int)
+    log(f"serving: building {nreq} requests (20-400 residues, bonded terms)")
+    t0 = time.perf_counter()
+    mols = synth.many([5000 + k for k in range(nreq)], [int(x) for x in sizes], bonded=True,
+                      workers=args.build_workers)
+    log(f"serving: built in {time.perf_counter() - t0:.1f} s")
+    return mols
+
+
+def run_serving(args, pdgpu, mols):
+    """Continuous-batching minimisation server at several Poisson rates."""
+    from pdgpu import serve
+    nreq = len(mols)
+    opts = pdgpu.default_options(precision=pdgpu.MIXED, cutoff=10.0, inner_cutoff=7.0, vdw_cutoff=10.0,
+                                 vdw_inner_cutoff=7.0, vdw14_scaling=0.5, elec14_scaling=0.83333,
+                                 dielectric_solvent=78.5, nlist_padding=2.0, max_instances=nreq)
+    out = {"workload": f"{nreq} minimisation requests per rate, sizes log-uniform 20-400 residues "
+                       f"(mean {int(np.mean([m['n'] for m in mols]))} atoms), steepest descent until force "
+                       "RMS < 0.1 kcal/mol/A or 1000 steps, 10 A cutoff + Still GB + bonded, mixed precision",
+           "memory_budget_bytes": args.serve_budget, "steps_per_tick": 10, "rates": []}
+    for rate in args.serve_rates:
+        arrivals = serve.poisson_arrivals(rate, nreq, seed=int(rate))
+        reqs = [serve.Request(k, m, float(t)) for k, (m, t) in enumerate(zip(mols, arrivals))]
+        srv = serve.MinimisationServer(opts, memory_budget=args.serve_budget, max_steps=1000, grms_tol=0.1,
+                                       steps_per_tick=10, keep_xyz=False)
+        t0 = time.perf_counter()
+        done = srv.run(reqs)
+        wall = time.perf_counter() - t0
+        srv.close()
+        summ = serve.summarize(done, wall)
+        summ["rate_req_per_s"] = rate
+        summ["wall_s"] = wall
+        out["rates"].append(summ)
+        log(f"serving {rate}/s: {summ['minimisations_per_s']:.1f} min/s, p50 {summ['p50_latency_s']:.3f} s, "
+            f"p99 {summ['p99_latency_s']:.3f} s")
     return out
 
 
-def run_batch(args, pdgpu):
+def build_batch(args):
     from pdgpu import synth
     rng = np.random.default_rng(11)
     lengths = rng.integers(10, 151, size=args.conformers)
+    log(f"building {args.conformers} conformers")
+    t0 = time.perf_counter()
+    mols = synth.many([1000 + k for k in range(args.conformers)], [int(L) for L in lengths],
+                      workers=args.build_workers)
+    log(f"built in {time.perf_counter() - t0:.1f} s")
+    return mols
+
+
+def run_batch(args, pdgpu, mols):
     opt = lambda prec: pdgpu.default_options(precision=prec, cutoff=10.0, inner_cutoff=7.0, vdw_cutoff=10.0,
                                              vdw_inner_cutoff=7.0, vdw14_scaling=0.5, elec14_scaling=0.83333,
                                              dielectric_solvent=78.5, nlist_padding=2.0,
                                              max_instances=args.conformers)
-    log(f"building {args.conformers} conformers")
-    t0 = time.perf_counter()
-    mols = synth.many([1000 + k for k in range(args.conformers)], [int(L) for L in lengths])
-    log(f"built in {time.perf_counter() - t0:.1f} s")
     natoms = sum(m["n"] for m in mols)
-    res = {}
+    res, grids = {}, {}
     for name, prec in (("mixed", pdgpu.MIXED), ("fp64", pdgpu.FP64)):
         ctx = pdgpu.Context(opt(prec))
         t0 = time.perf_counter()
@@ -214,6 +287,9 @@ def run_batch(args, pdgpu):
             ctx.add_system(m)
         ctx.nlist_build()
         log(f"{name}: {len(mols)} instances resident in {time.perf_counter() - t0:.1f} s")
+        grid = ctx.autotune(pdgpu.PDGPU_ALL, 2)
+        grids[name] = grid
+        log(f"{name}: autotuned wavefronts per CU {grid}")
         for _ in range(args.warmup):
             ctx.calc_forces(pdgpu.PDGPU_ALL, raw=True)
         t0 = time.perf_counter()
@@ -237,12 +313,9 @@ def run_batch(args, pdgpu):
     errs_c = [abs(r[k]["epot"] - r64[k]["epot"]) / comp(r64[k]) for k in r]
     worst = max(r, key=lambda k: errs[k] if False else abs(r[k]["epot"] - r64[k]["epot"]) / abs(r64[k]["epot"]))
     tot_err = abs(sum(r[k]["epot"] for k in r) - sum(r64[k]["epot"] for k in r)) / abs(sum(r64[k]["epot"] for k in r))
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle port)")
-        cpu = cpu_baseline_batch(mols, [r64[k]["pairs"] for k in sorted(r64)])
-    rl = roofline(prof, pairs, born, "mixed")
-    rl["traffic"] = load_traffic(rl["kernel"], "batch")
+    cpu = args.cpu_baseline
+    traffic = load_traffic()
+    rl = roofline(prof, pairs, born, "mixed", traffic)
     return {
         "metric": "atom-pair evals/s", "value": pairs * args.steps / dt, "unit": "atom-pair evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
@@ -250,9 +323,10 @@ def run_batch(args, pdgpu):
         "data": "synthetic",
         "config": {"workload": f"{args.conformers} extended conformers, 10-150 residues, one batched energy+force "
                                "evaluation per step, 10 A cutoff + Still GB, mixed precision",
-                   "atoms": natoms, "pairs_per_step": pairs, "parallelism": "single GPU"},
+                   "atoms": natoms, "pairs_per_step": pairs, "parallelism": "single GPU",
+                   "autotuned_waves_per_cu": grids},
         "fp64": {"value": pairs64 * args.steps / dt64, "ms_per_step": dt64 / args.steps * 1e3,
-                 "roofline": roofline(prof64, pairs64, born64, "fp64")},
+                 "roofline": roofline(prof64, pairs64, born64, "fp64", traffic)},
         "mixed_over_fp64": (pairs * args.steps / dt) / (pairs64 * args.steps / dt64),
         "max_energy_rel_err_mixed_vs_fp64": max(errs),
         "median_energy_rel_err_mixed_vs_fp64": float(np.median(errs)),
@@ -272,18 +346,38 @@ def run_batch(args, pdgpu):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="batch", choices=("md", "batch"))
+    ap.add_argument("--steps", type=int, default=20, help="timed batched evaluations")
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--conformers", type=int, default=4096)
+    ap.add_argument("--sections", default="batch,md,serving")
+    ap.add_argument("--md-steps", type=int, default=10000)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rates", type=lambda v: [float(x) for x in v.split(",")],
+                    default=[50, 100, 200, 500, 1000, 2000])
+    ap.add_argument("--serve-budget", type=int, default=4 << 30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--build-workers", type=int, default=0,
+                    help="processes building the synthetic inputs (0: up to 16; 1 under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(instances shard by molecule across GPUs without exchange, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    sections = args.sections.split(",")
     import pdgpu
-    out = run_md(args, pdgpu) if args.workload == "md" else run_batch(args, pdgpu)
+    # every input is built before the first GPU context: the builders fork
+    batch_mols = build_batch(args)
+    md_mol = build_md() if "md" in sections else None
+    serve_mols = build_serving(args) if "serving" in sections else None
+    args.cpu_baseline = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle port)")
+        args.cpu_baseline = cpu_baseline_batch(batch_mols)
+    out = run_batch(args, pdgpu, batch_mols)
+    if md_mol is not None:
+        out["md"] = run_md(args, pdgpu, md_mol)
+    if serve_mols is not None:
+        out["serving"] = run_serving(args, pdgpu, serve_mols)
     print(json.dumps(out), flush=True)
 
 

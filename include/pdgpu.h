@@ -244,6 +244,13 @@ int pdgpu_get_velocities(pdgpu_ctx *ctx, int32_t id, double *vel);
 int pdgpu_md_vv(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
                 pdgpu_energies *energies, double *ekin);
 
+/* Grid-shape autotuner (no reference counterpart): times the tile kernels
+ * of a force evaluation of id (or PDGPU_ALL) reps times at each of several
+ * wavefronts-per-CU settings and keeps the fastest per kernel for later
+ * evaluations. chosen (may be NULL) receives 4 values: enumeration, Born
+ * sum, pair pass, chain rule. Results do not depend on the setting. */
+int pdgpu_autotune(pdgpu_ctx *ctx, int32_t id, int32_t reps, int32_t *chosen);
+
 /* Optional timing of the kernel groups below with HIP events recorded on
  * the context's stream around each launch (no reference counterpart; used
  * by the benchmark). pdgpu_profile(ctx, 1) clears and starts recording,

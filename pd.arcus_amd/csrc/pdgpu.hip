@@ -126,6 +126,7 @@ struct pdgpu_ctx {
     DevBuf plist, tile_cnt, overflow;
     DevBuf bd_idx, bd_par, an_idx, an_par, dh_idx, dh_meta, dh_vn, dh_n, dh_gamma; // FF_Bonded records
     int64_t plist_cap = 0; // tiles the pair-list buffer holds
+    int grid_wpc[4] = {16, 16, 16, 16}; // wavefronts per CU of the tile kernels (enum, born, pair, chain)
 
     // host mirrors of per-block metadata
     std::vector<int> h_blk_inst, h_blk_n, h_blk_first;
@@ -330,7 +331,10 @@ Span span_of(const pdgpu_ctx *c, int id) {
     return {I.block0, I.block0 + I.nblocks, id, id + 1};
 }
 
-int waves_cap(const pdgpu_ctx *c) { return std::max(64, c->num_cus * 16); }
+// Tile kernels whose grid shape the autotuner chooses.
+enum { kGridEnum = 0, kGridBorn, kGridPair, kGridChain, kGridKinds };
+
+int waves_cap(const pdgpu_ctx *c, int kind) { return std::max(64, c->num_cus * c->grid_wpc[kind]); }
 
 hipEvent_t prof_event(pdgpu_ctx *c) {
     if (!c->prof_pool.empty()) {
@@ -365,8 +369,8 @@ struct ProfScope {
     }
 };
 
-void pair_grid(const pdgpu_ctx *c, int64_t tiles, int *groups, int *tpw) {
-    const int wcap = waves_cap(c);
+void pair_grid(const pdgpu_ctx *c, int kind, int64_t tiles, int *groups, int *tpw) {
+    const int wcap = waves_cap(c, kind);
     int t = c->opt.tiles_per_wave;
     if (t <= 0) t = (int)std::min<int64_t>(16, std::max<int64_t>(1, (tiles + wcap - 1) / wcap));
     int64_t waves = std::max<int64_t>(1, (tiles + t - 1) / t);
@@ -455,15 +459,19 @@ int64_t span_tiles(const pdgpu_ctx *c, int id) {
 // The GB kernels of one evaluation: enumeration, Born sums, Born radii,
 // pair pass and chain rule (MODE 0) or the energy-path pair pass (MODE 1).
 template <typename R>
-int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int groups, int tpw, int mode) {
+int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, int mode) {
     const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk, na = a1 - a0;
+    int g[kGridKinds], t[kGridKinds];
+    for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, tiles, &g[k], &t[k]);
     {
         ProfScope ps(c, PDGPU_KERNEL_ENUM);
-        hipLaunchKernelGGL(k_enum<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+        hipLaunchKernelGGL(k_enum<R>, dim3(g[kGridEnum]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                           t[kGridEnum]);
     }
     {
         ProfScope ps(c, PDGPU_KERNEL_BORN);
-        hipLaunchKernelGGL(k_born<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+        hipLaunchKernelGGL(k_born<R>, dim3(g[kGridBorn]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                           t[kGridBorn]);
     }
     {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
@@ -472,13 +480,16 @@ int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int groups, int t
     if (mode == 0) {
         {
             ProfScope ps(c, PDGPU_KERNEL_PAIR);
-            hipLaunchKernelGGL((k_pair<R, 0>), dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+            hipLaunchKernelGGL((k_pair<R, 0>), dim3(g[kGridPair]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                               t[kGridPair]);
         }
         ProfScope ps(c, PDGPU_KERNEL_CHAIN);
-        hipLaunchKernelGGL(k_chain<R>, dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+        hipLaunchKernelGGL(k_chain<R>, dim3(g[kGridChain]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                           t[kGridChain]);
     } else {
         ProfScope ps(c, PDGPU_KERNEL_PAIR);
-        hipLaunchKernelGGL((k_pair<R, 1>), dim3(groups), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, tpw);
+        hipLaunchKernelGGL((k_pair<R, 1>), dim3(g[kGridPair]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                           t[kGridPair]);
     }
     return PDGPU_OK;
 }
@@ -491,8 +502,6 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
     const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk;
     const int na = a1 - a0;
     const int ns = sp.s1 - sp.s0;
-    int groups, tpw;
-    pair_grid(c, span_tiles(c, id), &groups, &tpw);
     const bool gb = c->opt.components & PDGPU_FF_GB;
     {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
@@ -500,7 +509,7 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
         if (gb) hipLaunchKernelGGL(k_prepare<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
         else hipLaunchKernelGGL(k_clear_grad<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
     }
-    if (gb) TRY(launch_gb<R>(c, S, sp, groups, tpw, mode));
+    if (gb) TRY(launch_gb<R>(c, S, sp, span_tiles(c, id), mode));
     const BondedState B = c->B(id);
     const int nterms = (B.b1 - B.b0) + (B.a1 - B.a0) + (B.d1 - B.d0);
     if ((c->opt.components & PDGPU_FF_BONDED) && nterms > 0) {
@@ -1134,6 +1143,38 @@ extern "C" int pdgpu_get_born_radii(pdgpu_ctx *c, int32_t id, double *out) {
         HIPCHECK(c, hipMemcpyAsync(out, (double *)c->born.p + a0, I.natoms * 8, hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(c, hipStreamSynchronize(c->stream));
     }
+    return PDGPU_OK;
+}
+
+// Grid-shape autotuner: times each tile kernel of a force evaluation of
+// `id` (or all instances) at several wavefronts-per-CU settings and keeps
+// the fastest per kernel. Results do not depend on the grid (every tile
+// flushes its own sums), so tuning changes speed only.
+extern "C" int pdgpu_autotune(pdgpu_ctx *c, int32_t id, int32_t reps, int32_t *chosen) {
+    if (!c || (id != PDGPU_ALL && !valid_id(c, id)) || reps <= 0) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    static const int cand[] = {8, 16, 32, 64, 128, 256, 512, 1024};
+    const int kinds[kGridKinds] = {PDGPU_KERNEL_ENUM, PDGPU_KERNEL_BORN, PDGPU_KERNEL_PAIR, PDGPU_KERNEL_CHAIN};
+    double best[kGridKinds];
+    int pick[kGridKinds];
+    for (int k = 0; k < kGridKinds; ++k) { best[k] = 1e300; pick[k] = c->grid_wpc[k]; }
+    const bool was = c->prof;
+    TRY(eval(c, id, 0)); // lists built, code loaded
+    for (int w : cand) {
+        for (int k = 0; k < kGridKinds; ++k) c->grid_wpc[k] = w;
+        TRY(pdgpu_profile(c, 1));
+        for (int r = 0; r < reps; ++r) TRY(eval(c, id, 0));
+        double ms[PDGPU_KERNEL_NGROUPS];
+        int64_t n[PDGPU_KERNEL_NGROUPS];
+        TRY(pdgpu_profile_read(c, ms, n, PDGPU_KERNEL_NGROUPS));
+        for (int k = 0; k < kGridKinds; ++k)
+            if (ms[kinds[k]] < best[k]) { best[k] = ms[kinds[k]]; pick[k] = w; }
+    }
+    for (int k = 0; k < kGridKinds; ++k) {
+        c->grid_wpc[k] = pick[k];
+        if (chosen) chosen[k] = pick[k];
+    }
+    TRY(pdgpu_profile(c, was ? 1 : 0));
     return PDGPU_OK;
 }
 

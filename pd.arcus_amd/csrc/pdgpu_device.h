@@ -114,7 +114,9 @@ struct DevState {
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ int rot_i(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, 0x13C, 0xF, 0xF, false); // wave_ror:1
+    // wave_ror:1; every lane has a source, so the 'old' operand is never
+    // selected and passing v lets the result reuse v's register
+    return __builtin_amdgcn_update_dpp(v, v, 0x13C, 0xF, 0xF, false);
 }
 __device__ __forceinline__ float rot(float v) { return __int_as_float(rot_i(__float_as_int(v))); }
 __device__ __forceinline__ double rot(double v) {
@@ -446,8 +448,10 @@ __global__ void __launch_bounds__(256) k_enum(DevState S, PairParams P, int blk_
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
             R xj = pj.x, yj = pj.y, zj = pj.z;
-            int jl = (l < S.blk_n[J]) ? (J - first) * kBlk + l : -1;
-            int jlane = l;
+            const int nJ = S.blk_n[J];
+            // lane offset of one rotation (the same for every lane): after k
+            // rotations lane l holds the j-atom of lane (l + k*ldir) & 63
+            const int ldir = (rot_i(l) - l) & 63;
             const bool diag = fl & kTileDiag;
             const bool lookup = fl & (kTileNear | kTileSparse);
             const int mid = S.inst_mid[inst], jl0 = (J - first) * kBlk;
@@ -462,14 +466,18 @@ __global__ void __launch_bounds__(256) k_enum(DevState S, PairParams P, int blk_
             uint16_t *L = S.plist + (size_t)t * kTileCap;
             int nb = 0, no = 0;
             for (int k = 0; k < kBlk; ++k) {
-                if (k > 0) { xj = rot(xj); yj = rot(yj); zj = rot(zj); jl = rot(jl); jlane = rot(jlane); }
+                if (k > 0) { xj = rot(xj); yj = rot(yj); zj = rot(zj); }
+                const int jlane = (l + k * ldir) & 63;
                 const R dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
                 const R r2 = dx * dx + dy * dy + dz * dz;
-                const bool ok = vi && jl >= 0 && (!diag || jl < il) && r2 <= reach2;
+                // in a diagonal tile jl < il is jlane < l
+                const bool ok = vi && jlane < nJ && (!diag || jlane < l) && r2 <= reach2;
                 int bo = 7;
-                if (ok && lookup)
+                if (ok && lookup) {
+                    const int jl = jl0 + jlane;
                     bo = staged ? pair_bondorder_lds(S, inst, il, jl, fl, mid, sband[w], jl0)
                                 : pair_bondorder(S, inst, il, jl, fl);
+                }
                 const bool born = ok && bo > 2 && r2 <= still2;
                 const bool oth = ok && !born && r2 <= cut2;
                 const unsigned long long mb = __ballot(born), mo = __ballot(oth);

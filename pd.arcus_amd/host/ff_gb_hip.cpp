@@ -56,6 +56,7 @@ void FF_GeneralizedBorn_Still_HIP::fillOptions(pdgpu_options &o) const {
     // FF_NonBonded::calcEnergies mode selection (forcefields/nonbonded.cpp:1173-1178)
     o.elec_mode = !DoElec ? 0 : ForceSwitch ? 4 : EnergySwitch ? 3 : 1;
     o.vdw_mode = DoVdw ? 2 : 0;
+    o.components = PDGPU_FF_GB;
 }
 
 // FF_GeneralizedBorn_Still::setup (forcefields/gbff.cpp:40-87): parameters,
@@ -180,6 +181,97 @@ void FF_GeneralizedBorn_Still_HIP::infoLineHeader() const {
 // of the cutoffs forces a new setup.
 unsigned long FF_GeneralizedBorn_Still_HIP::calcCheckSum() {
     return (unsigned long)(Cutoff * VdwCutoff * 100000.0) + 1354231 * (int)AutoScaling14;
+}
+
+// ---------------------------------------------------------------- FF_Bonded
+
+FF_Bonded_HIP::FF_Bonded_HIP(WorkSpaceView &ws, BondedTerms t, int precision)
+    : ForcefieldBase(ws), Precision(precision), terms(std::move(t)) {
+    name = "Bonded Forcefield (HIP)";
+    ShortName = "Bonds";
+}
+
+FF_Bonded_HIP::~FF_Bonded_HIP() {
+    if (ctx) pdgpu_ctx_destroy(ctx);
+}
+
+// FF_Bonded::setup: the term lists are given; the instance carries only
+// what the bonded kernels read (positions). The GB parameters of the
+// topology are placeholders: this context evaluates no GB.
+void FF_Bonded_HIP::setup() {
+    const int n = (int)wspace.nAtoms();
+    pdgpu_options o;
+    pdgpu_options_default(&o);
+    o.precision = Precision;
+    o.components = (DoBonds ? PDGPU_FF_BONDS : 0) | (DoAngles ? PDGPU_FF_ANGLES : 0) |
+                   (DoTorsions ? PDGPU_FF_TORSIONS : 0) | (DoImpropers ? PDGPU_FF_IMPROPERS : 0);
+    if (ctx) { pdgpu_ctx_destroy(ctx); ctx = nullptr; }
+    check(pdgpu_ctx_create(&o, &ctx), ctx, "pdgpu_ctx_create");
+    std::vector<double> zero(n, 0.0), one(n, 1.0);
+    std::vector<int8_t> band(n > 1 ? n - 1 : 0, 0);
+    pdgpu_topology t = {};
+    t.natoms = n; t.max_index_delta = 1;
+    t.charge = zero.data(); t.radius = one.data(); t.epsilon = zero.data(); t.gb_radius = one.data();
+    t.vj = zero.data(); t.terms123 = one.data(); t.band = band.data();
+    check(pdgpu_instance_add(ctx, &t, &id), ctx, "instance_add");
+    pdgpu_bonded b = {};
+    b.nbonds = (int32_t)terms.bond_lk.size() / 2;
+    b.nangles = (int32_t)terms.angle_tk.size() / 2;
+    b.ntorsions = (int32_t)terms.torsion_terms.size();
+    b.nimpropers = (int32_t)terms.improper_terms.size();
+    b.bond_ij = terms.bond_ij.data(); b.bond_lk = terms.bond_lk.data();
+    b.angle_iaj = terms.angle_iaj.data(); b.angle_tk = terms.angle_tk.data();
+    b.torsion_iabj = terms.torsion_iabj.data(); b.torsion_terms = terms.torsion_terms.data();
+    b.torsion_vn = terms.torsion_vn.data(); b.torsion_n = terms.torsion_n.data();
+    b.torsion_gamma = terms.torsion_gamma.data();
+    b.improper_iabj = terms.improper_iabj.data(); b.improper_terms = terms.improper_terms.data();
+    b.improper_vn = terms.improper_vn.data(); b.improper_n = terms.improper_n.data();
+    b.improper_gamma = terms.improper_gamma.data();
+    check(pdgpu_bonded_set(ctx, id, &b), ctx, "bonded_set");
+    needsetup = false;
+}
+
+// FF_Bonded::calcForces (ffbonded.cpp:809-824): forces added to the
+// workspace; bonds, angles and torsions (impropers included) added to the
+// Hamiltonian, bond stretching to its InternalVirial.
+void FF_Bonded_HIP::calcForces() {
+    ensuresetup();
+    check(pdgpu_set_positions(ctx, id, wspace.pos.data()), ctx, "set_positions");
+    pdgpu_energies e;
+    std::vector<double> f(3 * wspace.nAtoms());
+    check(pdgpu_calc_forces(ctx, id, &e, f.data()), ctx, "calc_forces");
+    for (size_t k = 0; k < f.size(); ++k) wspace.force[k] += f[k];
+    epot_bond = e.epot_bond;
+    epot_angle = e.epot_angle;
+    epot_improper = e.epot_improper;
+    epot_torsion = e.epot_torsion - e.epot_improper;
+    epot = e.epot;
+    wspace.ene.epot_bond += e.epot_bond;
+    wspace.ene.epot_angle += e.epot_angle;
+    wspace.ene.epot_torsion += e.epot_torsion;
+    wspace.ene.InternalVirial += e.virial;
+    wspace.ene.epot += e.epot;
+    if (e.status != PDGPU_OK) wspace.ene.epot = NAN;
+}
+
+void FF_Bonded_HIP::info() const { // ffbonded.cpp:826-840
+    printf(" Bonds:     %8zu\n", terms.bond_lk.size() / 2);
+    printf(" Angles:    %8zu\n", terms.angle_tk.size() / 2);
+    printf(" Torsions:  %8zu\n", terms.torsion_terms.size());
+    printf(" Impropers: %8zu\n", terms.improper_terms.size());
+}
+
+void FF_Bonded_HIP::infoLine() const { // ffbonded.cpp:842-853
+    const Hamiltonian &h = wspace.ene;
+    if (DoBonds) printf("\t%6.1lf", h.epot_bond * kJ2kcalNa);
+    if (DoAngles) printf("\t%6.1lf", h.epot_angle * kJ2kcalNa);
+    if (DoTorsions) printf("\t%6.1lf", h.epot_torsion * kJ2kcalNa);
+}
+
+void FF_Bonded_HIP::infoLineHeader() const {
+    if (DoBonds) printf("\t%6s", "Ebond");
+    if (DoAngles) printf("\t%6s", "Eangle");
+    if (DoTorsions) printf("\t%6s", "Edihed");
 }
 
 } // namespace pdarcus

@@ -17,6 +17,7 @@ namespace pdarcus {
 
 struct Hamiltonian { // workspace/hamiltonian.h:23-90 (the fields the path touches)
     double epot = 0, epot_vdw = 0, epot_elec = 0, epot_pol = 0, epot_pol_self = 0, epot_pol_cross = 0;
+    double epot_bond = 0, epot_angle = 0, epot_torsion = 0;
     double InternalVirial = 0;
     void zero() { *this = Hamiltonian(); }
 };
@@ -104,6 +105,41 @@ private:
     int id = -1;
     double ffVdw14 = 1.0, ffElec14 = 1.0;
     std::vector<double> born;
+};
+
+// FF_Bonded's term lists as its setup assembles them from the parameter set
+// (forcefields/ffbonded.cpp:36-486), in PD's units: see pdgpu_bonded.
+struct BondedTerms {
+    std::vector<int32_t> bond_ij, angle_iaj, torsion_iabj, torsion_terms, improper_iabj, improper_terms;
+    std::vector<double> bond_lk, angle_tk, torsion_vn, torsion_n, torsion_gamma, improper_vn, improper_n,
+        improper_gamma;
+};
+
+// Physics::FF_Bonded (forcefields/ffbonded.h:103-214) on the device.
+class FF_Bonded_HIP : public ForcefieldBase {
+public:
+    FF_Bonded_HIP(WorkSpaceView &ws, BondedTerms terms, int precision = PDGPU_FP64);
+    ~FF_Bonded_HIP() override;
+
+    bool DoBonds = true, DoAngles = true, DoTorsions = true, DoImpropers = true;
+    int Precision;
+
+    void setup() override;
+    void calcForces() override;
+    void calcEnergies() override { calcForces(); } // ffbonded.cpp:804-807
+    void info() const override;
+    void infoLine() const override;
+    void infoLineHeader() const override;
+    double getEBond() const { return epot_bond; }
+    double getEAngle() const { return epot_angle; }
+    double getETorsion() const { return epot_torsion; }
+    double getEImproper() const { return epot_improper; }
+
+private:
+    BondedTerms terms;
+    pdgpu_ctx *ctx = nullptr;
+    int id = -1;
+    double epot_bond = 0, epot_angle = 0, epot_torsion = 0, epot_improper = 0;
 };
 
 } // namespace pdarcus

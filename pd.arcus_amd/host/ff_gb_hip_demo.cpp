@@ -6,8 +6,13 @@
 //         DielectricSolvent, Padding, ExpApproxThreshold);
 //        per atom: double x,y,z, charge, radius, epsilon, gb_radius, mass; int32 ir;
 //        12 pairs (int32 i, int32 j, double d); 13 pairs (i, j, d); 14 pairs (i, j)
+//        optionally FF_Bonded term lists: int32 nb, na, nt, ni; nb x (int32 i, j;
+//        double l, k); na x (int32 i, a, j; double theta0, k); nt then ni x
+//        (int32 i, a, b, j, terms; double vn[4], n[4], gamma[4])
 //   out: double epot, vdw, elec, pol, pol_self, pol_cross; forces[3n];
-//        then the same for calcEnergies (plus InternalVirial)
+//        then the same for calcEnergies (plus InternalVirial); with term
+//        lists, then FF_Bonded + GB in one forcefield: double epot, bond,
+//        angle, torsion, vdw, elec, pol, InternalVirial; forces[3n]
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -40,6 +45,33 @@ int main(int argc, char **argv) {
     for (int q = 0; q < n12; ++q) { int a = rd<int32_t>(f), b = rd<int32_t>(f); ws.cov12.push_back({a, b}); ws.d12.push_back(rd<double>(f)); }
     for (int q = 0; q < n13; ++q) { int a = rd<int32_t>(f), b = rd<int32_t>(f); ws.cov13.push_back({a, b}); ws.d13.push_back(rd<double>(f)); }
     for (int q = 0; q < n14; ++q) { int a = rd<int32_t>(f), b = rd<int32_t>(f); ws.cov14.push_back({a, b}); }
+    pdarcus::BondedTerms bt;
+    int32_t cnt[4];
+    const bool bonded = fread(cnt, 4, 4, f) == 4;
+    if (bonded) {
+        for (int q = 0; q < cnt[0]; ++q) {
+            for (int c = 0; c < 2; ++c) bt.bond_ij.push_back(rd<int32_t>(f));
+            for (int c = 0; c < 2; ++c) bt.bond_lk.push_back(rd<double>(f));
+        }
+        for (int q = 0; q < cnt[1]; ++q) {
+            for (int c = 0; c < 3; ++c) bt.angle_iaj.push_back(rd<int32_t>(f));
+            for (int c = 0; c < 2; ++c) bt.angle_tk.push_back(rd<double>(f));
+        }
+        for (int kind = 0; kind < 2; ++kind) {
+            auto &idx = kind ? bt.improper_iabj : bt.torsion_iabj;
+            auto &terms = kind ? bt.improper_terms : bt.torsion_terms;
+            auto &vn = kind ? bt.improper_vn : bt.torsion_vn;
+            auto &nn = kind ? bt.improper_n : bt.torsion_n;
+            auto &gm = kind ? bt.improper_gamma : bt.torsion_gamma;
+            for (int q = 0; q < cnt[2 + kind]; ++q) {
+                for (int c = 0; c < 4; ++c) idx.push_back(rd<int32_t>(f));
+                terms.push_back(rd<int32_t>(f));
+                for (int c = 0; c < 4; ++c) vn.push_back(rd<double>(f));
+                for (int c = 0; c < 4; ++c) nn.push_back(rd<double>(f));
+                for (int c = 0; c < 4; ++c) gm.push_back(rd<double>(f));
+            }
+        }
+    }
     fclose(f);
 
     pdarcus::FF_GeneralizedBorn_Still_HIP gb(ws, precision);
@@ -62,6 +94,20 @@ int main(int argc, char **argv) {
                         gb.epot_pol_cross, ws.ene.InternalVirial};
         fwrite(e2, 8, 7, o);
         fwrite(ws.force.data(), 8, ws.force.size(), o);
+        if (bonded) {
+            // Forcefield with FF_Bonded added first, then GB (forcefield.cpp:220-227)
+            pdarcus::FF_Bonded_HIP bo(ws, bt, precision);
+            bo.ensuresetup();
+            gb.calcNewList();
+            ws.zeroForces();
+            bo.calcForces();
+            gb.calcForces();
+            double e3[8] = {ws.ene.epot, ws.ene.epot_bond, ws.ene.epot_angle, ws.ene.epot_torsion,
+                            ws.ene.epot_vdw, ws.ene.epot_elec, ws.ene.epot_pol, ws.ene.InternalVirial};
+            fwrite(e3, 8, 8, o);
+            fwrite(ws.force.data(), 8, ws.force.size(), o);
+            bo.infoLineHeader();
+        }
         gb.infoLineHeader();
         printf("\n");
         gb.infoLine();

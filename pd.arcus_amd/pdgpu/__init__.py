@@ -46,7 +46,7 @@ EXPORTS = (
     "pdgpu_minimise_sd", "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_md_vv",
     "pdgpu_bondorder_build", "pdgpu_fixed_terms", "pdgpu_instance_bytes", "pdgpu_bytes_free",
     "pdgpu_profile", "pdgpu_profile_read", "pdgpu_sd_start", "pdgpu_sd_advance", "pdgpu_sd_finish",
-    "pdgpu_bonded_set", "pdgpu_bonded_bytes",
+    "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune",
 )
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
@@ -141,6 +141,7 @@ def _load():
         "pdgpu_profile_read": (C.c_int, [C.c_void_p, _dp, P(C.c_int64), C.c_int32]),
         "pdgpu_bonded_set": (C.c_int, [C.c_void_p, C.c_int32, P(Bonded)]),
         "pdgpu_bonded_bytes": (C.c_int64, [C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+        "pdgpu_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -417,6 +418,13 @@ class Context:
         self._check(lib.pdgpu_profile_read(self.h, _p(ms, _dp), cnt.ctypes.data_as(C.POINTER(C.c_int64)), n),
                     "profile_read")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_GROUPS)}
+
+    def autotune(self, iid=PDGPU_ALL, reps=2):
+        """Chooses wavefronts per CU for the enumeration, Born, pair and
+        chain-rule kernels by timing them; returns the four choices."""
+        out = np.zeros(4, np.int32)
+        self._check(lib.pdgpu_autotune(self.h, iid, reps, _p(out, _ip)), "autotune")
+        return dict(zip(("enum", "born", "pair", "chain"), (int(x) for x in out)))
 
     def bytes_free(self):
         return lib.pdgpu_bytes_free(self.h)

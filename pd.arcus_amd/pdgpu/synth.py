@@ -429,8 +429,10 @@ def _pairs(n, bonds):
 # FF_Bonded parameters of the generic forcefield (kcal/mol, A, rad): bond
 # and angle force constants by whether a hydrogen takes part; equilibrium
 # lengths and angles are the built geometry's. Torsions: amide C-N bonds a
-# 2-fold term that keeps the peptide plane, every other rotatable bond a
-# weak 3-fold term; impropers hold 3-coordinated C and N planar.
+# 2-fold term, every other rotatable bond a weak 3-fold term; impropers a
+# 2-fold term on 3-coordinated C and N. Each dihedral's phase puts its
+# minimum at the built angle (gamma = pi - n*phi0), the way PD's
+# FF_Bonded::resetEquilibrium* take the current structure as equilibrium.
 K_BOND_HEAVY, K_BOND_H = 300.0, 340.0
 K_ANGLE_HEAVY, K_ANGLE_H = 60.0, 35.0
 V_AMIDE, V_SP3 = 2.5, 0.15
@@ -457,7 +459,7 @@ def bonded_terms(xyz, el, bonds):
     th0 = np.arccos(np.clip((u * v).sum(1) / np.linalg.norm(u, axis=1) / np.linalg.norm(v, axis=1), -1, 1))
     ka = np.where(isH[ang[:, 0]] | isH[ang[:, 2]], K_ANGLE_H, K_ANGLE_HEAVY) * kj
     amide_c = {c for c in range(n) if el[c] == "C" and any(el[o] == "O" and len(nb[o]) == 1 for o in nb[c])}
-    tors, tv, tn, tg = [], [], [], []
+    tors, tv, tn = [], [], []
     for b, c in p12:
         amide = (b in amide_c and el[c] == "N") or (c in amide_c and el[b] == "N")
         for a in nb[b]:
@@ -469,7 +471,6 @@ def bonded_terms(xyz, el, bonds):
                 tors.append((a, b, c, d))
                 tv.append(V_AMIDE if amide else V_SP3)
                 tn.append(2.0 if amide else 3.0)
-                tg.append(np.pi if amide else 0.0)
     m = len(tors)
     z4 = lambda vals: np.concatenate([np.asarray(vals, dtype=float).reshape(-1, 1), np.zeros((len(vals), 3))], 1)
     imp, iv = [], []
@@ -479,12 +480,27 @@ def bonded_terms(xyz, el, bonds):
             imp.append((x, y, c, w))
             iv.append(V_IMPROPER_C if el[c] == "C" else V_IMPROPER_N)
     k = len(imp)
+    tors = np.array(tors, dtype=np.int32).reshape(-1, 4)
+    imp = np.array(imp, dtype=np.int32).reshape(-1, 4)
+    tn = np.asarray(tn, dtype=float)
+    tg = np.pi - tn * dihedral(xyz, tors)
+    ig = np.pi - 2.0 * dihedral(xyz, imp)
     return {"bd_idx": p12.ravel(), "bd_lk": np.stack([l, kb], 1).ravel(),
             "an_idx": ang.ravel(), "an_tk": np.stack([th0, ka], 1).ravel(),
-            "to_idx": np.array(tors, dtype=np.int32).ravel(), "to_terms": np.ones(m, np.int32),
+            "to_idx": tors.ravel(), "to_terms": np.ones(m, np.int32),
             "to_vn": (z4(tv) * kj).ravel(), "to_n": z4(tn).ravel(), "to_gamma": z4(tg).ravel(),
-            "im_idx": np.array(imp, dtype=np.int32).ravel(), "im_terms": np.ones(k, np.int32),
-            "im_vn": (z4(iv) * kj).ravel(), "im_n": z4([2.0] * k).ravel(), "im_gamma": z4([np.pi] * k).ravel()}
+            "im_idx": imp.ravel(), "im_terms": np.ones(k, np.int32),
+            "im_vn": (z4(iv) * kj).ravel(), "im_n": z4([2.0] * k).ravel(), "im_gamma": z4(ig).ravel()}
+
+
+def dihedral(xyz, q):
+    """IUPAC dihedral angles (rad) of the quartets q (m, 4)."""
+    b1 = xyz[q[:, 1]] - xyz[q[:, 0]]
+    b2 = xyz[q[:, 2]] - xyz[q[:, 1]]
+    b3 = xyz[q[:, 3]] - xyz[q[:, 2]]
+    m = np.cross(b1, b2)
+    n = np.cross(b2, b3)
+    return np.arctan2(np.linalg.norm(b2, axis=1) * (b1 * n).sum(1), (m * n).sum(1))
 
 
 def _finish(xyz, el, q, resid, bonds):
