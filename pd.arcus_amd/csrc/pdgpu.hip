@@ -87,6 +87,7 @@ struct Instance {
     int natoms = 0, nblocks = 0, block0 = 0, mid = 0, nsparse = 0;
     int64_t band_off = 0, band_len = 0, sp_off = 0;
     int64_t bd_off = 0, bd_len = 0, an_off = 0, an_len = 0, dh_off = 0, dh_len = 0; // FF_Bonded records
+    int64_t cs_off = 0, cs_len = 0; // candidate tile slots
     bool listed = false;  // tile rows built
     bool has_vel = false;
     bool sd = false;      // a minimisation was started and not finished
@@ -108,8 +109,8 @@ struct pdgpu_ctx {
     // capacities
     int64_t cap_blocks = 0, cap_band = 0, cap_sparse = 0, cap_tiles = 0;
     int max_inst = 0;
-    Ranges blocks, bandr, spr, bdr, anr, dhr;
-    int64_t cap_bd = 0, cap_an = 0, cap_dh = 0;
+    Ranges blocks, bandr, spr, bdr, anr, dhr, csr;
+    int64_t cap_bd = 0, cap_an = 0, cap_dh = 0, cap_cs = 0;
     std::vector<Instance> inst;
     int64_t tri_sum = 0; // sum over instances of nb*(nb+1)/2 (tile bound)
     int64_t ntiles = 0;  // tiles after the last build
@@ -124,12 +125,14 @@ struct pdgpu_ctx {
     DevBuf inst_need, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf band, sparse;
     DevBuf plist, tile_cnt, overflow;
+    DevBuf cand, cand_n, inst_cslot; // candidate pairs of the last rebuild, per tile slot
     DevBuf bd_idx, bd_par, an_idx, an_par, dh_idx, dh_meta, dh_vn, dh_n, dh_gamma; // FF_Bonded records
     int64_t plist_cap = 0; // tiles the pair-list buffer holds
     int grid_wpc[4] = {16, 16, 16, 16}; // wavefronts per CU of the tile kernels (enum, born, pair, chain)
 
     // host mirrors of per-block metadata
     std::vector<int> h_blk_inst, h_blk_n, h_blk_first;
+    std::vector<long long> h_cslot; // inst_cslot
 
     // optional per-kernel timing with HIP events on the context stream
     bool prof = false;
@@ -161,6 +164,9 @@ struct pdgpu_ctx {
         s.tile_cnt = (int2 *)tile_cnt.p;
         s.plist_cap = (int)std::min<int64_t>(plist_cap, INT32_MAX);
         s.overflow = (int *)overflow.p;
+        s.cand = (uint16_t *)cand.p;
+        s.cand_n = (int *)cand_n.p;
+        s.inst_cslot = (const long long *)inst_cslot.p;
         return s;
     }
 
@@ -396,6 +402,51 @@ int ensure_plist(pdgpu_ctx *c, int64_t tiles) {
     return PDGPU_OK;
 }
 
+int64_t span_tiles(const pdgpu_ctx *c, int id) {
+    if (id == PDGPU_ALL) return c->ntiles;
+    const int nb = c->inst[id].nblocks;
+    return std::min<int64_t>(c->ntiles, (int64_t)nb * (nb + 1) / 2);
+}
+
+// Gives every instance being rebuilt (id, all, or the flagged ones) a
+// candidate region of at least one slot per tile; regions move only when
+// they must grow, and the arena keeps the other instances' candidates.
+int ensure_cand_slots(pdgpu_ctx *c, int id, const std::vector<int> *need, const std::vector<int> &row_off) {
+    std::vector<int> redo;
+    for (int s = 0; s < c->max_inst; ++s) {
+        Instance &I = c->inst[s];
+        if (!I.active) continue;
+        const bool rebuilt = need ? (*need)[s] != 0 : (id == PDGPU_ALL || id == s);
+        if (!rebuilt) continue;
+        const int64_t tiles = row_off[I.block0 + I.nblocks] - row_off[I.block0];
+        if (tiles <= I.cs_len && I.cs_len > 0) continue;
+        c->csr.release(I.cs_off, I.cs_len);
+        I.cs_off = I.cs_len = 0;
+        const int64_t want = std::max<int64_t>(1, tiles + tiles / 4 + 1);
+        int64_t off = c->csr.alloc(want);
+        if (off < 0) {
+            const int64_t need_cap = c->cap_cs - c->csr.tail_free() + want;
+            const int64_t nc = std::max<int64_t>(need_cap, c->cap_cs * 5 / 4 + 64);
+            TRY(resize(c, c->cand, (size_t)nc * kTileCap, 2));
+            TRY(resize(c, c->cand_n, (size_t)nc, 4));
+            c->csr.grow(nc);
+            c->cap_cs = nc;
+            off = c->csr.alloc(want);
+            if (off < 0) return fail(c, PDGPU_ERR_NOMEM, "candidate slot allocation failed");
+        }
+        I.cs_off = off;
+        I.cs_len = want;
+        redo.push_back(s);
+    }
+    if (!redo.empty()) {
+        c->h_cslot.resize(c->max_inst, 0);
+        for (int s : redo) c->h_cslot[s] = c->inst[s].cs_off;
+        HIPCHECK(c, hipMemcpyAsync(c->inst_cslot.p, c->h_cslot.data(), c->max_inst * 8, hipMemcpyHostToDevice,
+                                   c->stream));
+    }
+    return PDGPU_OK;
+}
+
 // Rebuilds the tile rows of instance id (or all). With flagged, only the
 // instances whose device flag inst_need is set are rebuilt (the flags are
 // written by a kernel, so no host round trip is needed).
@@ -415,11 +466,32 @@ int launch_build(pdgpu_ctx *c, int id, bool flagged = false) {
     hipLaunchKernelGGL(k_tile_rows<1>, dim3((all + 3) / 4), dim3(256), 0, c->stream, S, c->P, 0, all);
     HIPCHECK(c, hipGetLastError());
     // the pair-list buffer needs one slot per tile: read the new total
-    int total = 0;
-    HIPCHECK(c, hipMemcpyAsync(&total, (int *)c->row_off.p + all, 4, hipMemcpyDeviceToHost, c->stream));
+    // per-instance tile counts: the pair-list buffer needs one slot per tile,
+    // and each rebuilt instance a candidate region of one slot per tile
+    std::vector<int> row_off(all + 1), need;
+    HIPCHECK(c, hipMemcpyAsync(row_off.data(), c->row_off.p, (all + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+    if (flagged) {
+        need.resize(c->max_inst);
+        HIPCHECK(c, hipMemcpyAsync(need.data(), c->inst_need.p, c->max_inst * 4, hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const int total = row_off[all];
     c->ntiles = total;
     TRY(ensure_plist(c, total));
+    TRY(ensure_cand_slots(c, id, flagged ? &need : nullptr, row_off));
+    {
+        int g, t;
+        pair_grid(c, kGridEnum, id == PDGPU_ALL || flagged ? total : span_tiles(c, id), &g, &t);
+        S = c->S();
+        if (flagged) S.inst_need = (const int *)c->inst_need.p;
+        // the mixed path enumerates from FP32 positions relative to the block
+        // centres k_block_bounds has just set
+        const int a0 = sp.b0 * kBlk, na = (sp.b1 - sp.b0) * kBlk;
+        if (c->mixed && na > 0)
+            hipLaunchKernelGGL(k_xyzq32, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a0 + na);
+        hipLaunchKernelGGL(k_cand<R>, dim3(g), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, t);
+        HIPCHECK(c, hipGetLastError());
+    }
     if (flagged) return PDGPU_OK; // flagged builds only run on instances listed before
     if (id == PDGPU_ALL) {
         for (auto &I : c->inst)
@@ -450,12 +522,6 @@ int ensure_listed(pdgpu_ctx *c, int id) {
     return PDGPU_OK;
 }
 
-int64_t span_tiles(const pdgpu_ctx *c, int id) {
-    if (id == PDGPU_ALL) return c->ntiles;
-    const int nb = c->inst[id].nblocks;
-    return std::min<int64_t>(c->ntiles, (int64_t)nb * (nb + 1) / 2);
-}
-
 // The GB kernels of one evaluation: enumeration, Born sums, Born radii,
 // pair pass and chain rule (MODE 0) or the energy-path pair pass (MODE 1).
 template <typename R>
@@ -465,7 +531,7 @@ int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, in
     for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, tiles, &g[k], &t[k]);
     {
         ProfScope ps(c, PDGPU_KERNEL_ENUM);
-        hipLaunchKernelGGL(k_enum<R>, dim3(g[kGridEnum]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+        hipLaunchKernelGGL(k_filter<R>, dim3(g[kGridEnum]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
                            t[kGridEnum]);
     }
     {
@@ -704,6 +770,7 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     if (rc == PDGPU_OK) rc = resize(c, c->inst_count, (size_t)ms * kCNum, 8);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_status, ms, 4);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_md, (size_t)ms, sizeof(InstMD));
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_cslot, (size_t)ms, 8);
     if (rc == PDGPU_OK) rc = grow_atoms(c, 64);
     if (rc == PDGPU_OK) rc = resize(c, c->band, 4096, 1);
     if (rc == PDGPU_OK) rc = resize(c, c->sparse, 64, 16);
@@ -721,6 +788,7 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     c->bdr.init(0);
     c->anr.init(0);
     c->dhr.init(0);
+    c->csr.init(0);
     *out = c;
     return PDGPU_OK;
 }
@@ -736,7 +804,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->tiles, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
                      &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
-                     &c->dh_gamma};
+                     &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot};
     for (DevBuf *b : all) release(*b);
     for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : c->prof_pool) (void)hipEventDestroy(e);
@@ -913,6 +981,7 @@ extern "C" int pdgpu_instance_remove(pdgpu_ctx *c, int32_t id) {
     HIPCHECK(c, hipMemcpyAsync((int *)c->inst_nat.p + id, &zero1, 4, hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     TRY(bonded_release(c, I));
+    c->csr.release(I.cs_off, I.cs_len);
     for (int k = 0; k < I.nblocks; ++k) c->h_blk_inst[I.block0 + k] = -1;
     c->blocks.release(I.block0, I.nblocks);
     c->bandr.release(I.band_off, I.band_len);

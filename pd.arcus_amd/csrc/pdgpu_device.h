@@ -105,6 +105,9 @@ struct DevState {
     int2 *tile_cnt;               // (born-side count, other count) per tile
     int plist_cap;                // tiles the list buffer holds
     int *overflow;                // set when a tile index exceeds plist_cap
+    uint16_t *cand;               // candidate pairs of the last rebuild, kTileCap per tile slot
+    int *cand_n;                  // candidates per tile slot
+    const long long *inst_cslot;  // first tile slot of each instance's candidate region
     void *inst_energy;            // [slot*kENum]
     unsigned long long *inst_count; // [slot*kCNum]
     int *inst_status;
@@ -344,17 +347,26 @@ __global__ void __launch_bounds__(1024) k_scan_rows(DevState S, int n) {
 
 // ----------------------------------------------------- per-atom preparation
 // Mixed path: block-relative FP32 coordinates. Also clears the Born sums.
+// Block-relative FP32 copy of the positions (mixed path), against the block
+// centres of the instance's last rebuild.
+__device__ __forceinline__ void store_xyzq32(const DevState &S, int a) {
+    const int b = a / kBlk;
+    const double4 c = S.blk_ctr[b];
+    S.xyzq32[a] = make_float4((float)(S.px[a] - c.x), (float)(S.py[a] - c.y), (float)(S.pz[a] - c.z),
+                              (float)S.q[a]);
+}
+
+__global__ void k_xyzq32(DevState S, int atom_begin, int atom_end) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < atom_end) store_xyzq32(S, a);
+}
+
 template <typename R>
 __global__ void k_prepare(DevState S, int atom_begin, int atom_end) {
     const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= atom_end) return;
     acc_store<R>(S.acc_g, a, 0.0, 1.0);
-    if constexpr (Prec<R>::fixed) {
-        const int b = a / kBlk;
-        const double4 c = S.blk_ctr[b];
-        S.xyzq32[a] = make_float4((float)(S.px[a] - c.x), (float)(S.py[a] - c.y), (float)(S.pz[a] - c.z),
-                                  (float)S.q[a]);
-    }
+    if constexpr (Prec<R>::fixed) store_xyzq32(S, a);
 }
 
 // Per-lane view of a tile's atoms.
@@ -388,16 +400,25 @@ template <typename R> __device__ __forceinline__ R ld_born(const DevState &S, si
 }
 
 // ------------------------------------------------------ pair enumeration
-// Each evaluation first walks every tile once: lane = i-atom, the 64 j-atoms
-// rotate through the lanes (DPP wave_ror:1), and only the distance and
-// bond-order tests are done. Pairs inside the cutoffs are compacted, with
-// ballot + mbcnt, into the tile's slot of a 2-byte-per-pair list:
-//   entry = i_lane | j_lane << 6 | bond_order << 12
-// Born-radius / chain-rule pairs (bond order > 2, r <= Still cutoff) are
-// packed from the front of the slot, the remaining pair-pass pairs from the
-// back. The compute kernels then work through these lists 64 pairs at a
-// time with every lane busy, instead of paying the full pair arithmetic for
-// every slot of the 64x64 tile.
+// Two stages, the way PD's list is used: candidates are enumerated when the
+// list is rebuilt, and every evaluation selects from them the pairs inside
+// its cutoffs.
+//
+// k_cand (at a rebuild) walks each tile of the rebuilt instances once:
+// lane = i-atom, the 64 j-atoms rotate through the lanes (DPP wave_ror:1),
+// and pairs with r^2 within the list radius (Cutoff + Padding, as PD's list
+// holds them) are compacted with ballot + mbcnt into the tile's candidate
+// slot: entry = i_lane | j_lane << 6 | bond_order << 12. The slot of the
+// k-th tile of an instance is inst_cslot + k; an instance keeps its tile
+// order until its own next rebuild, so its candidates stay valid while other
+// instances rebuild.
+//
+// k_filter (every evaluation) reads the candidates 64 at a time from LDS-
+// staged tile coordinates and compacts, in the same order, the born /
+// chain-rule pairs (bond order > 2, r <= Still cutoff) into the front of the
+// tile's pair list and the remaining pair-pass pairs (r <= Cutoff) into its
+// back. The compute kernels then work through these lists 64 pairs at a time
+// with every lane busy.
 //
 // Per-tile sums of the compute kernels are gathered in LDS with atomic adds
 // before one flush to the global accumulators. They are double in both
@@ -417,19 +438,26 @@ __device__ __forceinline__ unsigned long long lanes_below() {
     return l ? (~0ull >> (64 - l)) : 0ull;
 }
 
+// Candidate slot of tile t (of an instance whose first block is `first`).
+__device__ __forceinline__ long long cand_slot(const DevState &S, int inst, int first, int t) {
+    return S.inst_cslot[inst] + (t - S.row_off[first]);
+}
+
 // Band rows staged in LDS per near tile: 64 rows of up to kBandStage
 // entries. Wider bands (max_index_delta > kBandStage) are read from global
 // memory.
 constexpr int kBandStage = 64;
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_enum(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
+__global__ void __launch_bounds__(256) k_cand(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
     __shared__ int8_t sband[kWavesPerGroup][kBlk * kBandStage];
     const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
-    const R cut2 = (R)P.cut2, still2 = (R)P.still2, reach2 = cut2 > still2 ? cut2 : still2;
+    // the list radius, widened by a relative 1e-5 so that FP32 rounding never
+    // drops a pair the cutoff tests of the evaluation would keep
+    const R reach2 = (R)(P.list_lim2 * (1.0 + 1e-5));
     const unsigned long long below = lanes_below();
     for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
         const int cend = min(t_end, c + tpw);
@@ -437,11 +465,8 @@ __global__ void __launch_bounds__(256) k_enum(DevState S, PairParams P, int blk_
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF, fl = tw.y >> 24;
             const int inst = S.blk_inst[I], first = S.blk_first[I];
-            if (inst < 0 || t >= S.plist_cap) {
-                if (l == 0 && t < S.plist_cap) S.tile_cnt[t] = make_int2(0, 0);
-                if (l == 0 && t >= S.plist_cap) atomicOr(S.overflow, 1);
-                continue;
-            }
+            if (inst < 0 || (S.inst_need && !S.inst_need[inst])) continue;
+            const long long slot = cand_slot(S, inst, first, t);
             const int il = (I - first) * kBlk + l;
             const bool vi = l < S.blk_n[I];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
@@ -463,8 +488,8 @@ __global__ void __launch_bounds__(256) k_enum(DevState S, PairParams P, int blk_
                 for (int k = l; k < kBlk * mid; k += kBlk) sband[w][k] = src[k];
                 wave_lds_sync();
             }
-            uint16_t *L = S.plist + (size_t)t * kTileCap;
-            int nb = 0, no = 0;
+            uint16_t *L = S.cand + slot * kTileCap;
+            int n = 0;
             for (int k = 0; k < kBlk; ++k) {
                 if (k > 0) { xj = rot(xj); yj = rot(yj); zj = rot(zj); }
                 const int jlane = (l + k * ldir) & 63;
@@ -478,12 +503,61 @@ __global__ void __launch_bounds__(256) k_enum(DevState S, PairParams P, int blk_
                     bo = staged ? pair_bondorder_lds(S, inst, il, jl, fl, mid, sband[w], jl0)
                                 : pair_bondorder(S, inst, il, jl, fl);
                 }
-                const bool born = ok && bo > 2 && r2 <= still2;
-                const bool oth = ok && !born && r2 <= cut2;
+                const unsigned long long m = __ballot(ok);
+                if (ok) L[n + (int)__popcll(m & below)] = (uint16_t)(l | (jlane << 6) | (bo << 12));
+                n += (int)__popcll(m);
+            }
+            if (l == 0) S.cand_n[slot] = n;
+            wave_lds_sync();
+        }
+    }
+}
+
+template <typename R> struct PosR { R x, y, z; };
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
+    __shared__ PosR<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
+    const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
+    const int nw = gridDim.x * kWavesPerGroup;
+    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    const int l = lane_id(), w = threadIdx.x / 64;
+    const R cut2 = (R)P.cut2, still2 = (R)P.still2;
+    const unsigned long long below = lanes_below();
+    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
+        const int cend = min(t_end, c + tpw);
+        for (int t = c; t < cend; ++t) {
+            const int2 tw = S.tiles[t];
+            const int I = tw.x, J = tw.y & 0xFFFFFF;
+            const int inst = S.blk_inst[I], first = S.blk_first[I];
+            if (inst < 0 || t >= S.plist_cap) {
+                if (l == 0 && t < S.plist_cap) S.tile_cnt[t] = make_int2(0, 0);
+                if (l == 0 && t >= S.plist_cap) atomicOr(S.overflow, 1);
+                continue;
+            }
+            const long long slot = cand_slot(S, inst, first, t);
+            const int n = S.cand_n[slot];
+            const uint16_t *C = S.cand + slot * kTileCap;
+            uint16_t *L = S.plist + (size_t)t * kTileCap;
+            const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
+            const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+            const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+            si[w][l] = {pi.x, pi.y, pi.z};
+            sj[w][l] = {pj.x, pj.y, pj.z};
+            wave_lds_sync();
+            int nb = 0, no = 0;
+            for (int q0 = 0; q0 < n; q0 += kBlk) {
+                const int q = q0 + l;
+                const bool in = q < n;
+                const int e = in ? (int)C[q] : 0;
+                const PosR<R> A = si[w][e & 63], B = sj[w][(e >> 6) & 63];
+                const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+                const R r2 = dx * dx + dy * dy + dz * dz;
+                const bool born = in && ((e >> 12) & 7) > 2 && r2 <= still2;
+                const bool oth = in && !born && r2 <= cut2;
                 const unsigned long long mb = __ballot(born), mo = __ballot(oth);
-                const uint16_t e = (uint16_t)(l | (jlane << 6) | (bo << 12));
-                if (born) L[nb + (int)__popcll(mb & below)] = e;
-                if (oth) L[kTileCap - 1 - (no + (int)__popcll(mo & below))] = e;
+                if (born) L[nb + (int)__popcll(mb & below)] = (uint16_t)e;
+                if (oth) L[kTileCap - 1 - (no + (int)__popcll(mo & below))] = (uint16_t)e;
                 nb += (int)__popcll(mb);
                 no += (int)__popcll(mo);
             }
