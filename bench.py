@@ -242,7 +242,8 @@ def run_serving(args, pdgpu, mols):
     out = {"workload": f"{nreq} minimisation requests per rate, sizes log-uniform 20-400 residues "
                        f"(mean {int(np.mean([m['n'] for m in mols]))} atoms), steepest descent until force "
                        "RMS < 0.1 kcal/mol/A or 1000 steps, 10 A cutoff + Still GB + bonded, mixed precision",
-           "memory_budget_bytes": args.serve_budget, "steps_per_tick": 10, "rates": []}
+           "memory_budget_bytes": args.serve_budget, "steps_per_tick": 10,
+           "list_update": "rebuild per instance when an atom moved > padding/2", "rates": []}
     for rate in args.serve_rates:
         arrivals = serve.poisson_arrivals(rate, nreq, seed=int(rate))
         reqs = [serve.Request(k, m, float(t)) for k, (m, t) in enumerate(zip(mols, arrivals))]
@@ -253,6 +254,7 @@ def run_serving(args, pdgpu, mols):
         wall = time.perf_counter() - t0
         srv.close()
         summ = serve.summarize(done, wall)
+        summ["ticks"] = srv.ticks
         summ["rate_req_per_s"] = rate
         summ["wall_s"] = wall
         out["rates"].append(summ)

@@ -214,7 +214,10 @@ int pdgpu_get_forces(pdgpu_ctx *ctx, int32_t id, double *forces);
 /* Minimisation::runcore with SteepestDescent (protocols/minimise.cpp:99-235),
  * run on the device for one instance or all. PD semantics: step p += f *
  * step_size * mult, mult x1.2 on downhill, x0.5/1.2 and restore on uphill.
- * update_nlist: rebuild every that many steps (PD default 1). grms_tol > 0
+ * update_nlist: rebuild every that many steps (PD default 1); a negative
+ * value rebuilds an instance's list only when one of its atoms has moved
+ * more than half the list padding since its last rebuild (the Verlet-skin
+ * criterion, an extension PD does not have). grms_tol > 0
  * stops an instance once its force RMS falls below it (kcal/mol/Angstrom),
  * an extension PD does not have. steps_done (per id, may be NULL) receives
  * the steps each instance ran, or -1 if its energy became non-finite. */

@@ -118,11 +118,11 @@ struct pdgpu_ctx {
     // device buffers
     // per atom
     DevBuf px, py, pz, q, lj64, gb64, t123, xyzq32, lj32, gb32, born, acc_g, acc_deda, acc_fx, acc_fy, acc_fz;
-    DevBuf fx, fy, fz, deda_out, vx, vy, vz, minv, opx, opy, opz;
+    DevBuf fx, fy, fz, deda_out, vx, vy, vz, minv, opx, opy, opz, rx, ry, rz;
     // per block
     DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, row_count, row_off, tiles;
     // per instance
-    DevBuf inst_need, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
+    DevBuf any_need, inst_need, inst_moved, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf band, sparse;
     DevBuf plist, tile_cnt, overflow;
     DevBuf cand, cand_n, inst_cslot; // candidate pairs of the last rebuild, per tile slot
@@ -132,7 +132,6 @@ struct pdgpu_ctx {
 
     // host mirrors of per-block metadata
     std::vector<int> h_blk_inst, h_blk_n, h_blk_first;
-    std::vector<long long> h_cslot; // inst_cslot
 
     // optional per-kernel timing with HIP events on the context stream
     bool prof = false;
@@ -160,10 +159,13 @@ struct pdgpu_ctx {
         s.inst_energy = inst_energy.p; s.inst_count = (unsigned long long *)inst_count.p;
         s.inst_status = (int *)inst_status.p;
         s.inst_need = nullptr;
+        s.any_need = nullptr;
         s.plist = (uint16_t *)plist.p;
         s.tile_cnt = (int2 *)tile_cnt.p;
         s.plist_cap = (int)std::min<int64_t>(plist_cap, INT32_MAX);
         s.overflow = (int *)overflow.p;
+        s.rx = (double *)rx.p; s.ry = (double *)ry.p; s.rz = (double *)rz.p;
+        s.inst_moved = (int *)inst_moved.p;
         s.cand = (uint16_t *)cand.p;
         s.cand_n = (int *)cand_n.p;
         s.inst_cslot = (const long long *)inst_cslot.p;
@@ -241,7 +243,7 @@ int grow_atoms(pdgpu_ctx *c, int64_t blocks) {
     const size_t na = (size_t)blocks * kBlk;
     DevBuf *d8[] = {&c->px, &c->py, &c->pz, &c->q, &c->t123, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy,
                     &c->acc_fz, &c->fx, &c->fy, &c->fz, &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv,
-                    &c->opx, &c->opy, &c->opz};
+                    &c->opx, &c->opy, &c->opz, &c->rx, &c->ry, &c->rz};
     for (DevBuf *b : d8) TRY(resize(c, *b, na, 8));
     TRY(resize(c, c->lj64, na, 16));
     TRY(resize(c, c->gb64, na, 16));
@@ -402,58 +404,27 @@ int ensure_plist(pdgpu_ctx *c, int64_t tiles) {
     return PDGPU_OK;
 }
 
+// Tiles an evaluation of id walks, for sizing launch grids only (kernels
+// read the true range from row_off): the count read at the last counted
+// rebuild, else the upper bound.
 int64_t span_tiles(const pdgpu_ctx *c, int id) {
-    if (id == PDGPU_ALL) return c->ntiles;
+    const int64_t all = c->ntiles > 0 ? c->ntiles : c->tri_sum;
+    if (id == PDGPU_ALL) return all;
     const int nb = c->inst[id].nblocks;
-    return std::min<int64_t>(c->ntiles, (int64_t)nb * (nb + 1) / 2);
+    return std::min<int64_t>(all, (int64_t)nb * (nb + 1) / 2);
 }
 
-// Gives every instance being rebuilt (id, all, or the flagged ones) a
-// candidate region of at least one slot per tile; regions move only when
-// they must grow, and the arena keeps the other instances' candidates.
-int ensure_cand_slots(pdgpu_ctx *c, int id, const std::vector<int> *need, const std::vector<int> &row_off) {
-    std::vector<int> redo;
-    for (int s = 0; s < c->max_inst; ++s) {
-        Instance &I = c->inst[s];
-        if (!I.active) continue;
-        const bool rebuilt = need ? (*need)[s] != 0 : (id == PDGPU_ALL || id == s);
-        if (!rebuilt) continue;
-        const int64_t tiles = row_off[I.block0 + I.nblocks] - row_off[I.block0];
-        if (tiles <= I.cs_len && I.cs_len > 0) continue;
-        c->csr.release(I.cs_off, I.cs_len);
-        I.cs_off = I.cs_len = 0;
-        const int64_t want = std::max<int64_t>(1, tiles + tiles / 4 + 1);
-        int64_t off = c->csr.alloc(want);
-        if (off < 0) {
-            const int64_t need_cap = c->cap_cs - c->csr.tail_free() + want;
-            const int64_t nc = std::max<int64_t>(need_cap, c->cap_cs * 5 / 4 + 64);
-            TRY(resize(c, c->cand, (size_t)nc * kTileCap, 2));
-            TRY(resize(c, c->cand_n, (size_t)nc, 4));
-            c->csr.grow(nc);
-            c->cap_cs = nc;
-            off = c->csr.alloc(want);
-            if (off < 0) return fail(c, PDGPU_ERR_NOMEM, "candidate slot allocation failed");
-        }
-        I.cs_off = off;
-        I.cs_len = want;
-        redo.push_back(s);
-    }
-    if (!redo.empty()) {
-        c->h_cslot.resize(c->max_inst, 0);
-        for (int s : redo) c->h_cslot[s] = c->inst[s].cs_off;
-        HIPCHECK(c, hipMemcpyAsync(c->inst_cslot.p, c->h_cslot.data(), c->max_inst * 8, hipMemcpyHostToDevice,
-                                   c->stream));
-    }
-    return PDGPU_OK;
-}
-
-// Rebuilds the tile rows of instance id (or all). With flagged, only the
-// instances whose device flag inst_need is set are rebuilt (the flags are
-// written by a kernel, so no host round trip is needed).
+// Rebuilds the tile rows and candidate pairs of instance id (or all). With
+// flagged, only the instances whose device flag inst_need is set are rebuilt
+// (the flags are written by a kernel). Every instance owns a candidate region
+// for its largest possible tile count and the pair-list buffer covers all
+// tiles, so a rebuild needs nothing from the host; count = true reads the new
+// tile total back to size later launch grids (a host round trip).
 template <typename R>
-int launch_build(pdgpu_ctx *c, int id, bool flagged = false) {
+int launch_build(pdgpu_ctx *c, int id, bool flagged = false, bool count = true, const int *gate = nullptr) {
     DevState S = c->S();
     if (flagged) S.inst_need = (const int *)c->inst_need.p;
+    S.any_need = gate; // with a gate, the kernels return at once when nothing is flagged
     const Span sp = span_of(c, flagged ? PDGPU_ALL : id);
     const int nb = sp.b1 - sp.b0;
     ProfScope ps(c, PDGPU_KERNEL_NLIST);
@@ -464,33 +435,22 @@ int launch_build(pdgpu_ctx *c, int id, bool flagged = false) {
     const int all = (int)c->cap_blocks;
     hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, c->stream, S, all);
     hipLaunchKernelGGL(k_tile_rows<1>, dim3((all + 3) / 4), dim3(256), 0, c->stream, S, c->P, 0, all);
-    HIPCHECK(c, hipGetLastError());
-    // the pair-list buffer needs one slot per tile: read the new total
-    // per-instance tile counts: the pair-list buffer needs one slot per tile,
-    // and each rebuilt instance a candidate region of one slot per tile
-    std::vector<int> row_off(all + 1), need;
-    HIPCHECK(c, hipMemcpyAsync(row_off.data(), c->row_off.p, (all + 1) * 4, hipMemcpyDeviceToHost, c->stream));
-    if (flagged) {
-        need.resize(c->max_inst);
-        HIPCHECK(c, hipMemcpyAsync(need.data(), c->inst_need.p, c->max_inst * 4, hipMemcpyDeviceToHost, c->stream));
-    }
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    const int total = row_off[all];
-    c->ntiles = total;
-    TRY(ensure_plist(c, total));
-    TRY(ensure_cand_slots(c, id, flagged ? &need : nullptr, row_off));
     {
         int g, t;
-        pair_grid(c, kGridEnum, id == PDGPU_ALL || flagged ? total : span_tiles(c, id), &g, &t);
-        S = c->S();
-        if (flagged) S.inst_need = (const int *)c->inst_need.p;
+        pair_grid(c, kGridEnum, id == PDGPU_ALL || flagged ? c->ntiles : span_tiles(c, id), &g, &t);
         // the mixed path enumerates from FP32 positions relative to the block
         // centres k_block_bounds has just set
-        const int a0 = sp.b0 * kBlk, na = (sp.b1 - sp.b0) * kBlk;
+        const int a0 = sp.b0 * kBlk, na = nb * kBlk;
         if (c->mixed && na > 0)
             hipLaunchKernelGGL(k_xyzq32, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a0 + na);
         hipLaunchKernelGGL(k_cand<R>, dim3(g), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, t);
-        HIPCHECK(c, hipGetLastError());
+    }
+    HIPCHECK(c, hipGetLastError());
+    if (count) {
+        int total = 0;
+        HIPCHECK(c, hipMemcpyAsync(&total, (int *)c->row_off.p + all, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        c->ntiles = total;
     }
     if (flagged) return PDGPU_OK; // flagged builds only run on instances listed before
     if (id == PDGPU_ALL) {
@@ -762,6 +722,8 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     const int ms = c->max_inst;
     if (rc == PDGPU_OK) rc = resize(c, c->inst_nat, ms, 4);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_need, ms, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_moved, ms, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->any_need, 1, 4);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_mid, ms, 4);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_band_off, ms, 8);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_sp_off, ms, 4);
@@ -804,7 +766,8 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->tiles, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
                      &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
-                     &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot};
+                     &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot, &c->rx, &c->ry, &c->rz,
+                     &c->inst_moved, &c->any_need};
     for (DevBuf *b : all) release(*b);
     for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : c->prof_pool) (void)hipEventDestroy(e);
@@ -815,10 +778,12 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
 extern "C" int64_t pdgpu_instance_bytes(const pdgpu_ctx *c, int32_t natoms, int32_t mid, int32_t nsparse) {
     if (!c || natoms <= 0) return -1;
     const int64_t nb = (natoms + kBlk - 1) / kBlk;
-    const int64_t per_atom = 8 * 21 + 16 * 2 + (c->mixed ? 4 + 16 + 16 : 8);
+    const int64_t per_atom = 8 * 24 + 16 * 2 + (c->mixed ? 4 + 16 + 16 : 8);
     const int64_t per_block = 4 * 3 + 32 * 2 + 16 + 4 * 2;
+    // per possible tile: the tile record, a candidate slot and a pair-list slot
+    const int64_t per_tile = 8 + 2 * kTileCap + 4 + 2 * kTileCap + 8;
     return nb * kBlk * per_atom + nb * per_block + (int64_t)mid * nb * kBlk + nsparse * 16 +
-           nb * (nb + 1) / 2 * 8;
+           nb * (nb + 1) / 2 * per_tile;
 }
 
 extern "C" int64_t pdgpu_bytes_free(const pdgpu_ctx *c) { return c ? c->budget - c->used : 0; }
@@ -885,6 +850,28 @@ extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t
         }
         c->cap_tiles = nc;
     }
+    // pair lists: a candidate region of nb(nb+1)/2 tile slots (the largest
+    // tile count the instance can have) and pair-list slots for every tile
+    const int64_t tslots = (int64_t)nb * (nb + 1) / 2;
+    int64_t csoff = c->csr.alloc(tslots);
+    int rcl = PDGPU_OK;
+    if (csoff < 0) {
+        const int64_t need_cap = c->cap_cs - c->csr.tail_free() + tslots;
+        const int64_t nc = std::max<int64_t>(need_cap, c->cap_cs * 5 / 4 + 64);
+        rcl = resize(c, c->cand, (size_t)nc * kTileCap, 2);
+        if (rcl == PDGPU_OK) rcl = resize(c, c->cand_n, (size_t)nc, 4);
+        if (rcl == PDGPU_OK) {
+            c->csr.grow(nc);
+            c->cap_cs = nc;
+            csoff = c->csr.alloc(tslots);
+        }
+    }
+    if (rcl == PDGPU_OK) rcl = ensure_plist(c, tri);
+    if (rcl != PDGPU_OK || csoff < 0) {
+        c->blocks.release(b0, nb); c->bandr.release(boff, blen); c->spr.release(soff, t->nsparse);
+        if (csoff >= 0) c->csr.release(csoff, tslots);
+        return rcl != PDGPU_OK ? rcl : fail(c, PDGPU_ERR_NOMEM, "candidate slot allocation failed");
+    }
     c->tri_sum = tri;
 
     // upload per-atom data in the padded layout
@@ -948,6 +935,8 @@ extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t
     HIPCHECK(c, hipMemcpyAsync((long long *)c->inst_band_off.p + slot, &boffl, 8, hipMemcpyHostToDevice, s));
     HIPCHECK(c, hipMemcpyAsync((int *)c->inst_sp_off.p + slot, &soffi, 4, hipMemcpyHostToDevice, s));
     HIPCHECK(c, hipMemcpyAsync((int *)c->inst_sp_n.p + slot, &sn, 4, hipMemcpyHostToDevice, s));
+    const long long csl = csoff;
+    HIPCHECK(c, hipMemcpyAsync((long long *)c->inst_cslot.p + slot, &csl, 8, hipMemcpyHostToDevice, s));
     HIPCHECK(c, hipStreamSynchronize(s));
 
     Instance &I = c->inst[slot];
@@ -961,6 +950,8 @@ extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t
     I.band_off = boff;
     I.band_len = blen;
     I.sp_off = soff;
+    I.cs_off = csoff;
+    I.cs_len = tslots;
     I.listed = false;
     if (t->mass) I.mass.assign(t->mass, t->mass + n);
     *id_out = slot;

@@ -1,11 +1,12 @@
 // pdgpu_bonded.h -- FF_Bonded on the device (forcefields/ffbonded.cpp:809-1590):
 // harmonic bonds, harmonic angles, Fourier / harmonic dihedrals for
 // torsions and impropers. One work item per term; every term reads its
-// 2-4 atoms' FP64 positions and adds its gradient to the same per-atom
-// accumulators as the pair kernels (64-bit fixed point in the mixed path,
-// FP64 atomics in the FP64 path), so bonded and GB forces come out of one
-// k_finalize. The arithmetic is FP64 in both precisions: the terms are
-// O(atoms) and their cost is the memory traffic, not the maths.
+// 2-4 atoms' FP64 positions, and the gradients of a workgroup's terms are
+// summed in LDS before they reach the same per-atom accumulators as the pair
+// kernels (64-bit fixed point in the mixed path, FP64 in the FP64 path), so
+// bonded and GB forces come out of one k_finalize. The arithmetic is FP64 in
+// both precisions: the terms are O(atoms) and their cost is moving gradients,
+// not the maths.
 #pragma once
 
 #include "pdgpu_device.h"
@@ -44,13 +45,43 @@ __device__ __forceinline__ D3 d3_lin(double s, const D3 &a, double t, const D3 &
 }
 __device__ __forceinline__ D3 d3_at(const DevState &S, int a) { return {S.px[a], S.py[a], S.pz[a]}; }
 
-// Adds gradient g (kcal/mol/A, dE/dx) of atom a.
-template <typename R>
-__device__ __forceinline__ void grad_add(const DevState &S, int a, const D3 &g, int *st) {
-    acc_add<R>(S.acc_fx, a, g.x, kForceScale, st);
-    acc_add<R>(S.acc_fy, a, g.y, kForceScale, st);
-    acc_add<R>(S.acc_fz, a, g.z, kForceScale, st);
-}
+// Gradient sink of one workgroup. Terms are stored instance by instance in
+// atom order, so the 256 terms of a workgroup touch a narrow range of atoms:
+// their gradients are summed in an LDS window of kWin atoms starting at the
+// smallest atom index of the workgroup, and the window is added to the
+// global accumulators once at the end. Atoms beyond the window go straight
+// to the global accumulators. In the mixed path every contribution is
+// rounded to 64-bit fixed point before it is added, so the sums are exact
+// integers whatever the grouping.
+constexpr int kWin = 1024;
+
+template <typename R> struct GradSink {
+    unsigned long long (*win)[kWin]; // [3][kWin]: int64 fixed point (mixed) or double bits (FP64)
+    int amin;
+    const DevState *S;
+
+    __device__ void add(int a, const D3 &g, int *st) const {
+        const double v[3] = {g.x, g.y, g.z};
+        const int k = a - amin;
+        if (k >= 0 && k < kWin) {
+            for (int c = 0; c < 3; ++c) {
+                if constexpr (Prec<R>::fixed) {
+                    const double sc = v[c] * kForceScale;
+                    if (!(fabs(sc) < kFixedLimit)) { atomicOr(st, (sc == sc) ? kStatOverflow : kStatNonFinite); continue; }
+                    const long long iv = __double2ll_rn(sc);
+                    if (iv != 0) atomicAdd(&win[c][k], (unsigned long long)iv);
+                } else {
+                    if (!(fabs(v[c]) < 1e300)) atomicOr(st, kStatNonFinite);
+                    if (v[c] != 0.0) atomicAdd(reinterpret_cast<double *>(&win[c][k]), v[c]);
+                }
+            }
+        } else {
+            acc_add<R>(S->acc_fx, a, g.x, kForceScale, st);
+            acc_add<R>(S->acc_fy, a, g.y, kForceScale, st);
+            acc_add<R>(S->acc_fz, a, g.z, kForceScale, st);
+        }
+    }
+};
 
 // Per-instance energy slot += e for every valid lane. Called by the whole
 // wavefront. In the mixed path each term's energy is rounded to fixed point
@@ -96,25 +127,25 @@ __device__ void add_term_energy(const DevState &S, bool valid, int inst, int slo
 // dphi/dx (Blondel & Karplus, J. Comput. Chem. 17:1132, 1996) instead of the
 // reference's two-branch cos/sin derivative; both are the exact gradient.
 template <typename R>
-__device__ double dihedral_term(const DevState &S, const int4 &q, int terms, const double4 &vn, const double4 &nn,
-                                const double4 &gm, int *st) {
+__device__ double dihedral_term(const DevState &S, const GradSink<R> &G, const int4 &q, int terms, const double4 &vn,
+                                const double4 &nn, const double4 &gm, int *st) {
     const D3 ri = d3_at(S, q.x), ra = d3_at(S, q.y), rb = d3_at(S, q.z), rj = d3_at(S, q.w);
     const D3 b1 = d3_sub(ra, ri), b2 = d3_sub(rb, ra), b3 = d3_sub(rj, rb);
     const D3 m = d3_cross(b1, b2), n = d3_cross(b2, b3);
     const double m2 = d3_dot(m, m), n2 = d3_dot(n, n), b22 = d3_dot(b2, b2);
     const double lb2 = sqrt(b22);
     const double phi = atan2(lb2 * d3_dot(b1, n), d3_dot(m, n));
-    const double V[4] = {vn.x, vn.y, vn.z, vn.w}, N[4] = {nn.x, nn.y, nn.z, nn.w}, G[4] = {gm.x, gm.y, gm.z, gm.w};
+    const double V[4] = {vn.x, vn.y, vn.z, vn.w}, N[4] = {nn.x, nn.y, nn.z, nn.w}, Ph[4] = {gm.x, gm.y, gm.z, gm.w};
     double e = 0, dedphi = 0;
     for (int t = 0; t < 4; ++t) {
         if (t >= terms) break;
         if (N[t] > 0) {
             double s, c;
-            sincos(N[t] * phi + G[t], &s, &c);
+            sincos(N[t] * phi + Ph[t], &s, &c);
             e += V[t] * (1.0 + c);
             dedphi -= N[t] * V[t] * s;
         } else {
-            double d = phi - G[t];
+            double d = phi - Ph[t];
             if (d < -M_PI) d += 2.0 * M_PI;
             else if (d > M_PI) d -= 2.0 * M_PI;
             e += V[t] * d * d;
@@ -124,10 +155,10 @@ __device__ double dihedral_term(const DevState &S, const int4 &q, int terms, con
     const D3 gi = d3_lin(-dedphi * lb2 / m2, m, 0.0, m);
     const D3 gj = d3_lin(dedphi * lb2 / n2, n, 0.0, n);
     const double p1 = d3_dot(b1, b2) / b22, p3 = d3_dot(b3, b2) / b22;
-    grad_add<R>(S, q.x, gi, st);
-    grad_add<R>(S, q.y, d3_lin(-(1.0 + p1), gi, p3, gj), st);
-    grad_add<R>(S, q.z, d3_lin(-(1.0 + p3), gj, p1, gi), st);
-    grad_add<R>(S, q.w, gj, st);
+    G.add(q.x, gi, st);
+    G.add(q.y, d3_lin(-(1.0 + p1), gi, p3, gj), st);
+    G.add(q.z, d3_lin(-(1.0 + p3), gj, p1, gi), st);
+    G.add(q.w, gj, st);
     return e;
 }
 
@@ -135,69 +166,95 @@ __device__ double dihedral_term(const DevState &S, const int4 &q, int terms, con
 // [d0,d1) dihedrals, one thread per record.
 template <typename R>
 __global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
+    __shared__ unsigned long long win[3][kWin];
+    __shared__ int s_amin;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nb = B.b1 - B.b0, na = B.a1 - B.a0, nd = B.d1 - B.d0;
-    bool valid = false;
-    int inst = -1, slot = kEBond;
-    double e = 0, vir = 0;
+    // decode this thread's record
+    int kind = -1, r = 0, inst = -1;
+    int4 q = make_int4(0, 0, 0, 0);
     if (t < nb) {
-        const int4 id = B.bd_idx[B.b0 + t];
-        if (id.z > 0 && (B.comps & kFFBonds)) {
-            // calcBondForces (ffbonded.cpp:885-940): E = k(d-l)^2
-            valid = true;
-            inst = id.z - 1;
-            int *st = S.inst_status + inst;
-            const double2 p = B.bd_par[B.b0 + t];
-            const D3 v = d3_sub(d3_at(S, id.x), d3_at(S, id.y));
-            const double d = sqrt(d3_dot(v, v));
-            const double dd = d - p.x;
-            e = p.y * dd * dd;
-            const double dedd = 2.0 * p.y * dd;
-            vir = dedd * d; // InternalVirial += d * force (ffbonded.cpp:924)
-            const D3 g = d3_lin(dedd / d, v, 0.0, v);
-            grad_add<R>(S, id.x, g, st);
-            grad_add<R>(S, id.y, d3_lin(-1.0, g, 0.0, g), st);
-        }
+        r = B.b0 + t;
+        const int4 id = B.bd_idx[r];
+        if (id.z > 0 && (B.comps & kFFBonds)) { kind = 0; inst = id.z - 1; q = make_int4(id.x, id.y, id.x, id.x); }
     } else if (t < nb + na) {
-        const int r = B.a0 + (t - nb);
+        r = B.a0 + (t - nb);
         const int4 id = B.an_idx[r];
-        const double2 p = B.an_par[r];
-        if (id.w > 0 && (B.comps & kFFAngles) && p.y != 0.0) {
-            // calcAngleForces (ffbonded.cpp:994-1120): E = k(theta-theta0)^2
-            valid = true;
-            inst = id.w - 1;
-            slot = kEAngle;
-            int *st = S.inst_status + inst;
-            const D3 ra = d3_at(S, id.y);
-            const D3 vi = d3_sub(d3_at(S, id.x), ra), vj = d3_sub(d3_at(S, id.z), ra);
-            const double li = 1.0 / sqrt(d3_dot(vi, vi)), lj = 1.0 / sqrt(d3_dot(vj, vj));
-            const D3 ui = d3_lin(li, vi, 0.0, vi), uj = d3_lin(lj, vj, 0.0, vj);
-            const double c = d3_dot(ui, uj);
-            const double th = acos(c), sn = sin(th);
-            const double dth = th - p.x;
-            e = p.y * dth * dth;
-            const double dedth = 2.0 * p.y * dth;
-            // dtheta/dx_i = (u_i cos - u_j) / (|v_i| sin)
-            const D3 gi = d3_lin(dedth * li / sn * c, ui, -dedth * li / sn, uj);
-            const D3 gj = d3_lin(dedth * lj / sn * c, uj, -dedth * lj / sn, ui);
-            grad_add<R>(S, id.x, gi, st);
-            grad_add<R>(S, id.z, gj, st);
-            grad_add<R>(S, id.y, d3_lin(-1.0, gi, -1.0, gj), st);
+        if (id.w > 0 && (B.comps & kFFAngles) && B.an_par[r].y != 0.0) {
+            kind = 1; inst = id.w - 1; q = make_int4(id.x, id.y, id.z, id.x);
         }
     } else if (t < nb + na + nd) {
-        const int r = B.d0 + (t - nb - na);
+        r = B.d0 + (t - nb - na);
         const int meta = B.dh_meta[r];
-        const int s1 = meta & 0xFFFFFF, terms = (meta >> 24) & 7, impr = (meta >> 28) & 1;
-        if (s1 > 0 && (B.comps & (impr ? kFFImpropers : kFFTorsions))) {
-            // calcDihedralForces (ffbonded.cpp:1199-1440)
-            valid = true;
-            inst = s1 - 1;
-            slot = impr ? kEImproper : kETorsion;
-            e = dihedral_term<R>(S, B.dh_idx[r], terms, B.dh_vn[r], B.dh_n[r], B.dh_gamma[r], S.inst_status + inst);
+        const int impr = (meta >> 28) & 1;
+        if ((meta & 0xFFFFFF) > 0 && (B.comps & (impr ? kFFImpropers : kFFTorsions))) {
+            kind = 2 + impr; inst = (meta & 0xFFFFFF) - 1; q = B.dh_idx[r];
         }
     }
-    add_term_energy<R>(S, valid, inst, slot, e);
-    add_term_energy<R>(S, valid && t < nb, inst, kEVirial, vir);
+    // LDS window at the workgroup's smallest atom index
+    if (threadIdx.x == 0) s_amin = 0x7FFFFFFF;
+    for (int k = threadIdx.x; k < 3 * kWin; k += blockDim.x) (&win[0][0])[k] = 0ull;
+    __syncthreads();
+    if (kind >= 0) atomicMin(&s_amin, min(min(q.x, q.y), min(q.z, q.w)));
+    __syncthreads();
+    const GradSink<R> G{win, s_amin, &S};
+    int slot = kEBond;
+    double e = 0, vir = 0;
+    if (kind == 0) {
+        // calcBondForces (ffbonded.cpp:885-940): E = k(d-l)^2
+        int *st = S.inst_status + inst;
+        const double2 p = B.bd_par[r];
+        const D3 v = d3_sub(d3_at(S, q.x), d3_at(S, q.y));
+        const double d = sqrt(d3_dot(v, v));
+        const double dd = d - p.x;
+        e = p.y * dd * dd;
+        const double dedd = 2.0 * p.y * dd;
+        vir = dedd * d; // InternalVirial += d * force (ffbonded.cpp:924)
+        const D3 g = d3_lin(dedd / d, v, 0.0, v);
+        G.add(q.x, g, st);
+        G.add(q.y, d3_lin(-1.0, g, 0.0, g), st);
+    } else if (kind == 1) {
+        // calcAngleForces (ffbonded.cpp:994-1120): E = k(theta-theta0)^2
+        slot = kEAngle;
+        int *st = S.inst_status + inst;
+        const double2 p = B.an_par[r];
+        const D3 ra = d3_at(S, q.y);
+        const D3 vi = d3_sub(d3_at(S, q.x), ra), vj = d3_sub(d3_at(S, q.z), ra);
+        const double li = 1.0 / sqrt(d3_dot(vi, vi)), lj = 1.0 / sqrt(d3_dot(vj, vj));
+        const D3 ui = d3_lin(li, vi, 0.0, vi), uj = d3_lin(lj, vj, 0.0, vj);
+        const double c = d3_dot(ui, uj);
+        const double th = acos(c), sn = sin(th);
+        const double dth = th - p.x;
+        e = p.y * dth * dth;
+        const double dedth = 2.0 * p.y * dth;
+        // dtheta/dx_i = (u_i cos - u_j) / (|v_i| sin)
+        const D3 gi = d3_lin(dedth * li / sn * c, ui, -dedth * li / sn, uj);
+        const D3 gj = d3_lin(dedth * lj / sn * c, uj, -dedth * lj / sn, ui);
+        G.add(q.x, gi, st);
+        G.add(q.z, gj, st);
+        G.add(q.y, d3_lin(-1.0, gi, -1.0, gj), st);
+    } else if (kind >= 2) {
+        // calcDihedralForces (ffbonded.cpp:1199-1440)
+        slot = kind == 3 ? kEImproper : kETorsion;
+        const int meta = B.dh_meta[r];
+        e = dihedral_term<R>(S, G, q, (meta >> 24) & 7, B.dh_vn[r], B.dh_n[r], B.dh_gamma[r], S.inst_status + inst);
+    }
+    add_term_energy<R>(S, kind >= 0, inst, slot, e);
+    add_term_energy<R>(S, kind == 0, inst, kEVirial, vir);
+    __syncthreads();
+    // flush the window (only atoms that received something)
+    for (int k = threadIdx.x; k < kWin; k += blockDim.x) {
+        for (int c = 0; c < 3; ++c) {
+            const unsigned long long v = win[c][k];
+            if (v == 0ull) continue;
+            void *base = c == 0 ? S.acc_fx : c == 1 ? S.acc_fy : S.acc_fz;
+            if constexpr (Prec<R>::fixed) {
+                atomicAdd(reinterpret_cast<unsigned long long *>(base) + (s_amin + k), v);
+            } else {
+                unsafeAtomicAdd(reinterpret_cast<double *>(base) + (s_amin + k), __longlong_as_double((long long)v));
+            }
+        }
+    }
 }
 
 // Clears the force accumulators where the GB kernels (which clear them in

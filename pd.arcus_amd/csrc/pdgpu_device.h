@@ -101,6 +101,9 @@ struct DevState {
     const int8_t *band;
     const int4 *sparse;           // (i_local, j_local, order, 0), i<j
     const int *inst_need;         // optional per-instance rebuild flags (null: all)
+    const int *any_need;          // optional: 0 means no instance is flagged (skip the rebuild)
+    double *rx, *ry, *rz;         // positions at each atom's last list rebuild
+    int *inst_moved;              // set when an atom moved more than half the padding since
     uint16_t *plist;              // per-tile compacted pair lists, kTileCap entries each
     int2 *tile_cnt;               // (born-side count, other count) per tile
     int plist_cap;                // tiles the list buffer holds
@@ -240,7 +243,11 @@ __global__ void k_inst_clear(DevState S, int slot_begin, int slot_end) {
 
 // --------------------------------------------------- neighbour-list build
 // Bounding box and reference centre of each block (one wave per block).
+// A flagged rebuild with no flagged instance does nothing.
+__device__ __forceinline__ bool build_idle(const DevState &S) { return S.any_need && *S.any_need == 0; }
+
 __global__ void k_block_bounds(DevState S, int blk_begin, int blk_end) {
+    if (build_idle(S)) return;
     const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     if (b >= blk_end) return;
     if (S.blk_inst[b] < 0) return;
@@ -249,6 +256,7 @@ __global__ void k_block_bounds(DevState S, int blk_begin, int blk_end) {
     const bool v = l < S.blk_n[b];
     const size_t a = (size_t)b * kBlk + l;
     double x = v ? S.px[a] : 0, y = v ? S.py[a] : 0, z = v ? S.pz[a] : 0;
+    S.rx[a] = x; S.ry[a] = y; S.rz[a] = z; // reference for the displacement test
     double mnx = v ? x : 1e300, mny = v ? y : 1e300, mnz = v ? z : 1e300;
     double mxx = v ? x : -1e300, mxy = v ? y : -1e300, mxz = v ? z : -1e300;
     for (int o = 32; o > 0; o >>= 1) {
@@ -265,6 +273,24 @@ __global__ void k_block_bounds(DevState S, int blk_begin, int blk_end) {
         S.blk_half[b] = make_float4((float)(0.5 * (mxx - mnx)) + m, (float)(0.5 * (mxy - mny)) + m,
                                     (float)(0.5 * (mxz - mnz)) + m, 0.f);
     }
+}
+
+// Verlet-skin test: flags an instance when one of its atoms has moved more
+// than half the list padding since the instance's last rebuild (then a pair
+// from beyond the list radius may have come inside the cutoff).
+__global__ void k_moved(DevState S, double lim2, int blk_begin, int blk_end) {
+    const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (b >= blk_end) return;
+    const int inst = S.blk_inst[b];
+    if (inst < 0) return;
+    const int l = lane_id();
+    const size_t a = (size_t)b * kBlk + l;
+    bool far = false;
+    if (l < S.blk_n[b]) {
+        const double dx = S.px[a] - S.rx[a], dy = S.py[a] - S.ry[a], dz = S.pz[a] - S.rz[a];
+        far = !(dx * dx + dy * dy + dz * dz <= lim2); // non-finite positions count as moved
+    }
+    if (__ballot(far) && l == 0) atomicOr(S.inst_moved + inst, 1);
 }
 
 __device__ __forceinline__ bool boxes_within(const DevState &S, int I, int J, double r) {
@@ -297,6 +323,7 @@ __device__ __forceinline__ int tile_flags(const DevState &S, int inst, int I, in
 // pass 0: count tiles per I row; pass 1: write them at row_off[I].
 template <int PASS>
 __global__ void k_tile_rows(DevState S, PairParams P, int blk_begin, int blk_end) {
+    if (build_idle(S)) return;
     const int I = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     if (I >= blk_end) return;
     const int inst = S.blk_inst[I];
@@ -326,6 +353,7 @@ __global__ void k_tile_rows(DevState S, PairParams P, int blk_begin, int blk_end
 // Exclusive scan of row_count over [0, n) into row_off, total at row_off[n].
 // One workgroup of 1024 threads; deterministic.
 __global__ void __launch_bounds__(1024) k_scan_rows(DevState S, int n) {
+    if (build_idle(S)) return;
     __shared__ int part[1024];
     const int t = threadIdx.x;
     const int per = (n + 1023) / 1024;
@@ -357,6 +385,7 @@ __device__ __forceinline__ void store_xyzq32(const DevState &S, int a) {
 }
 
 __global__ void k_xyzq32(DevState S, int atom_begin, int atom_end) {
+    if (build_idle(S)) return;
     const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (a < atom_end) store_xyzq32(S, a);
 }
@@ -451,6 +480,7 @@ constexpr int kBandStage = 64;
 template <typename R>
 __global__ void __launch_bounds__(256) k_cand(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
     __shared__ int8_t sband[kWavesPerGroup][kBlk * kBandStage];
+    if (build_idle(S)) return;
     const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);

@@ -58,11 +58,23 @@ __global__ void k_sd_start(InstMD *md, int s, int max_steps, double step_size, d
 
 // Rebuild flags of running minimisations: PD refreshes the list when
 // step % UpdateNList == 0 (protocols/protocolbase.cpp:113-122).
-__global__ void k_sd_need(const InstMD *md, int *need, int s0, int s1) {
+// With update_nlist < 0 an instance rebuilds when k_moved has flagged it
+// (the Verlet-skin criterion, an extension PD does not have); the flags are
+// cleared here for the next step.
+__global__ void k_sd_need(const InstMD *md, int *need, int *moved, int *any, int s0, int s1) {
     const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= s1) return;
     const InstMD &m = md[s];
-    need[s] = (m.running && !m.done && m.update_nlist > 0 && (m.step % m.update_nlist) == 0) ? 1 : 0;
+    const bool sched = m.update_nlist > 0 && (m.step % m.update_nlist) == 0;
+    const bool skin = m.update_nlist < 0 && moved[s] != 0;
+    const bool n = m.running && !m.done && (sched || skin);
+    need[s] = n ? 1 : 0;
+    moved[s] = 0;
+    if (n) atomicOr(any, 1);
+}
+
+__global__ void k_flag_reset(int *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
 // sum of |f|^2 per instance (one wave per block)

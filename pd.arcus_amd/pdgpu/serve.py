@@ -5,7 +5,10 @@ carry them) arrive over time. Each tick the server
   1. admits waiting requests, first come first served, while the resident
      set stays inside the device-memory budget (pdgpu_instance_bytes);
   2. advances every resident minimisation by `steps_per_tick` steepest-
-     descent iterations in shared launches (pdgpu_sd_advance);
+     descent iterations in shared launches (pdgpu_sd_advance); by default
+     an instance's list is rebuilt when one of its atoms has moved more
+     than half the list padding (update_nlist=-1; PD's every-step rebuild
+     is update_nlist=1);
   3. collects the minimisations that finished (force RMS below the
      tolerance, step limit, or failure), frees their slots and memory.
 Coordinates, neighbour lists and minimiser state stay on the device for the
@@ -39,7 +42,7 @@ class Request:
 
 
 class MinimisationServer:
-    def __init__(self, options, memory_budget, max_steps=1000, grms_tol=0.1, step_size=2e5, update_nlist=1,
+    def __init__(self, options, memory_budget, max_steps=1000, grms_tol=0.1, step_size=2e5, update_nlist=-1,
                  steps_per_tick=4, keep_xyz=True):
         self.ctx = Context(options)
         self.budget = int(memory_budget)

@@ -173,7 +173,7 @@ def test_server_poisson_requests_complete_and_match():
     for r in out[:3]:
         with pdgpu.Context(opts(pdgpu.MIXED)) as ctx:
             i = ctx.add_system(r.mol, bonded=True)
-            st = ctx.minimise_sd(i, 60, 2e5, 1, 0.1)
+            st = ctx.minimise_sd(i, 60, 2e5, -1, 0.1)
             e, _ = ctx.calc_forces(i)
             x = ctx.get_positions(i)
         assert st == r.steps
@@ -190,3 +190,20 @@ def test_admission_respects_budget():
             srv.try_admit(serve.Request(0, m, 0.0), 0.0)
     finally:
         srv.close()
+
+
+def test_skin_rebuild_matches_every_step_rebuild():
+    """update_nlist = -1 (rebuild when an atom moved more than half the
+    padding) evaluates the same pairs as PD's every-step rebuild, so the
+    minimisations agree to FP64 summation order."""
+    m = synth.peptide(seed=44, residues=16, bonded=True)
+    out = {}
+    for un in (1, -1):
+        with pdgpu.Context(opts(pdgpu.FP64)) as ctx:
+            i = ctx.add_system(m, bonded=True)
+            st = ctx.minimise_sd(i, 40, 2e5, un)
+            e, _ = ctx.calc_forces(i)
+            out[un] = (st, e["epot"], ctx.get_positions(i))
+    assert out[1][0] == out[-1][0] == 40
+    assert rel(out[1][1], out[-1][1]) <= 1e-9
+    assert np.abs(out[1][2] - out[-1][2]).max() <= 1e-8
