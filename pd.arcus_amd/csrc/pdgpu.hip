@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 using namespace pdgpu;
@@ -133,6 +134,14 @@ struct pdgpu_ctx {
     // host mirrors of per-block metadata
     std::vector<int> h_blk_inst, h_blk_n, h_blk_first;
 
+    // captured step graphs, keyed by loop kind and instance, each with the
+    // launch arguments it was captured with
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<char> sig;
+    };
+    std::map<std::string, StepGraph> graphs;
+
     // optional per-kernel timing with HIP events on the context stream
     bool prof = false;
     struct ProfRec { int kind; hipEvent_t a, b; };
@@ -141,6 +150,7 @@ struct pdgpu_ctx {
 
     DevState S() const {
         DevState s;
+        std::memset(&s, 0, sizeof(s)); // padding too: graph signatures compare the bytes
         s.px = (const double *)px.p; s.py = (const double *)py.p; s.pz = (const double *)pz.p;
         s.q = (const double *)q.p;
         s.lj64 = (const double2 *)lj64.p; s.gb64 = (const double2 *)gb64.p;
@@ -175,6 +185,7 @@ struct pdgpu_ctx {
     // FF_Bonded records of instance id (PDGPU_ALL: every record slot)
     BondedState B(int id) const {
         BondedState b;
+        std::memset(&b, 0, sizeof(b));
         b.bd_idx = (const int4 *)bd_idx.p; b.bd_par = (const double2 *)bd_par.p;
         b.an_idx = (const int4 *)an_idx.p; b.an_par = (const double2 *)an_par.p;
         b.dh_idx = (const int4 *)dh_idx.p; b.dh_meta = (const int *)dh_meta.p;
@@ -681,6 +692,59 @@ int bonded_release(pdgpu_ctx *c, Instance &I) {
     return PDGPU_OK;
 }
 
+// Bytes of everything a captured step passes to its kernels: device
+// pointers, ranges, grid shapes and parameters. A graph is replayed only
+// while these are unchanged (buffers move when they grow, spans when
+// instances come and go).
+std::vector<char> graph_sig(const pdgpu_ctx *c, int id) {
+    const DevState S = c->S();
+    const BondedState B = c->B(id);
+    const Span sp = span_of(c, id);
+    const int64_t nt = span_tiles(c, id);
+    std::vector<char> v(sizeof(S) + sizeof(B) + sizeof(sp) + sizeof(nt) + sizeof(c->grid_wpc) + sizeof(c->P));
+    char *p = v.data();
+    std::memcpy(p, &S, sizeof(S)); p += sizeof(S);
+    std::memcpy(p, &B, sizeof(B)); p += sizeof(B);
+    std::memcpy(p, &sp, sizeof(sp)); p += sizeof(sp);
+    std::memcpy(p, &nt, sizeof(nt)); p += sizeof(nt);
+    std::memcpy(p, c->grid_wpc, sizeof(c->grid_wpc)); p += sizeof(c->grid_wpc);
+    std::memcpy(p, &c->P, sizeof(c->P));
+    return v;
+}
+
+// Launches the kernels `issue` puts on the context stream as one HIP graph,
+// captured on first use and again whenever the signature changes. Only
+// kernel launches are captured (no memset or copy nodes).
+template <typename F>
+int graph_run(pdgpu_ctx *c, const std::string &key, const std::vector<char> &sig, F &&issue) {
+    pdgpu_ctx::StepGraph &g = c->graphs[key];
+    if (!g.exec || g.sig != sig) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.sig.clear();
+        hipGraph_t graph = nullptr;
+        HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = issue();
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc != PDGPU_OK) {
+            if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIPCHECK(c, e);
+        const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            g.exec = nullptr;
+            return fail(c, PDGPU_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+        g.sig = sig;
+    }
+    HIPCHECK(c, hipGraphLaunch(g.exec, c->stream));
+    return PDGPU_OK;
+}
+
+bool use_graphs(const pdgpu_ctx *c) { return c->opt.use_graphs != 0 && !c->prof; }
+
 } // namespace
 
 // =================================================================== C ABI
@@ -769,6 +833,8 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot, &c->rx, &c->ry, &c->rz,
                      &c->inst_moved, &c->any_need};
     for (DevBuf *b : all) release(*b);
+    for (auto &g : c->graphs)
+        if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);

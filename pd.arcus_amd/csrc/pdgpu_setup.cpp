@@ -42,6 +42,7 @@ extern "C" void pdgpu_options_default(pdgpu_options *o) {
     o->max_instances = 0;
     o->tiles_per_wave = 0;
     o->components = PDGPU_FF_ALL;
+    o->use_graphs = 1;
 }
 
 // BondOrder::reinit (workspace/bondorder.cpp:77-195). The band holds the

@@ -69,7 +69,8 @@ class Options(C.Structure):
                    "elec14_scaling", "dielectric_solute", "dielectric_solvent", "gbsa_still_cutoff",
                    "exp_approx_threshold", "dielectric_offset", "born_radius_offset",
                    "p1", "p2", "p3", "p4", "p5", "nlist_padding")] + \
-               [("device_memory_budget", C.c_int64), ("max_instances", C.c_int32), ("components", C.c_int32)]
+               [("device_memory_budget", C.c_int64), ("max_instances", C.c_int32), ("components", C.c_int32),
+                ("use_graphs", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Bonded(C.Structure):
@@ -403,6 +404,14 @@ class Context:
         return e.as_dict(), st.value
 
     def md_vv(self, iid, steps, dt, update_nlist=10):
+        """Velocity Verlet on the device. For one instance: (energies dict,
+        kinetic energy J); for PDGPU_ALL: dicts indexed by instance id."""
+        if iid == PDGPU_ALL:
+            n = self.opt.max_instances if self.opt.max_instances > 0 else 8192
+            arr = (Energies * n)()
+            ek = np.zeros(n)
+            self._check(lib.pdgpu_md_vv(self.h, iid, steps, dt, update_nlist, arr, _p(ek, _dp)), "md_vv")
+            return {i: arr[i].as_dict() for i in self._natoms}, {i: float(ek[i]) for i in self._natoms}
         e = Energies()
         ek = np.zeros(1)
         self._check(lib.pdgpu_md_vv(self.h, iid, steps, dt, update_nlist, C.byref(e), _p(ek, _dp)), "md_vv")

@@ -207,3 +207,33 @@ def test_skin_rebuild_matches_every_step_rebuild():
     assert out[1][0] == out[-1][0] == 40
     assert rel(out[1][1], out[-1][1]) <= 1e-9
     assert np.abs(out[1][2] - out[-1][2]).max() <= 1e-8
+
+
+def test_graph_replay_equals_eager():
+    """Step loops replay captured HIP graphs; every replay must give the bits
+    the eager launches give (mixed path: order-independent sums). Eager work
+    and synchronisations (state read-back, position reads) sit between the
+    replays."""
+    mols = [synth.peptide(seed=90 + k, residues=r, bonded=True) for k, r in enumerate((9, 15))]
+    res = {}
+    for g in (0, 1):
+        with pdgpu.Context(opts(pdgpu.MIXED, use_graphs=g)) as ctx:
+            ids = [ctx.add_system(m, bonded=True) for m in mols]
+            for i in ids:
+                ctx.sd_start(i, 40, 2e5, -1, 0.0)
+            snaps = []
+            for _ in range(4):
+                ctx.sd_advance(7)
+                snaps.append([ctx.get_positions(i).copy() for i in ids])
+            for i in ids:
+                ctx.sd_finish(i)
+            rng = np.random.default_rng(5)
+            for i, m in zip(ids, mols):
+                ctx.set_velocities(i, rng.normal(size=(m["n"], 3)) * 300.0)
+            for _ in range(3):
+                e, ek = ctx.md_vv(pdgpu.PDGPU_ALL, 12, 1e-15, 5)
+                snaps.append([ctx.get_positions(i).copy() for i in ids])
+            res[g] = snaps
+    for a, b in zip(res[0], res[1]):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
