@@ -519,6 +519,33 @@ __global__ void __launch_bounds__(256) k_cand(DevState S, PairParams P, int blk_
                 wave_lds_sync();
             }
             uint16_t *L = S.cand + slot * kTileCap;
+            if (diag) {
+                // dense tile: per-lane bit rows over all 64 partners (both
+                // orders) -- candidate, excluded (bond order <= 2), 1-4 pair
+                unsigned long long mc = 0, mx = 0, m4 = 0;
+                for (int k = 1; k < kBlk; ++k) {
+                    xj = rot(xj); yj = rot(yj); zj = rot(zj);
+                    const int jlane = (l + k * ldir) & 63;
+                    const R dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    if (vi && jlane < nJ && r2 <= reach2) {
+                        const int jl = jl0 + jlane, hi = max(il, jl), lo = min(il, jl);
+                        const int bo = staged ? pair_bondorder_lds(S, inst, hi, lo, fl, mid, sband[w], jl0)
+                                              : pair_bondorder(S, inst, hi, lo, fl);
+                        const unsigned long long bit = 1ull << jlane;
+                        mc |= bit;
+                        if (bo <= 2) mx |= bit;
+                        if (bo == 3) m4 |= bit;
+                    }
+                }
+                unsigned long long *M = reinterpret_cast<unsigned long long *>(L);
+                M[l] = mc;
+                M[kBlk + l] = mx;
+                M[2 * kBlk + l] = m4;
+                if (l == 0) S.cand_n[slot] = 0;
+                wave_lds_sync();
+                continue;
+            }
             int n = 0;
             for (int k = 0; k < kBlk; ++k) {
                 if (k > 0) { xj = rot(xj); yj = rot(yj); zj = rot(zj); }
@@ -565,6 +592,10 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
                 if (l == 0 && t >= S.plist_cap) atomicOr(S.overflow, 1);
                 continue;
             }
+            if ((tw.y >> 24) & kTileDiag) { // dense tiles are evaluated from their bit rows
+                if (l == 0) S.tile_cnt[t] = make_int2(0, 0);
+                continue;
+            }
             const long long slot = cand_slot(S, inst, first, t);
             const int n = S.cand_n[slot];
             const uint16_t *C = S.cand + slot * kTileCap;
@@ -606,6 +637,46 @@ __device__ __forceinline__ int entry_pos(int q, int nb) { return q < nb ? q : kT
 // cutoff (forcefields/gbff.cpp:432-465), both directions per unordered pair.
 // Every accumulator is flushed at the end of each tile, so the sums an atom
 // receives do not depend on how tiles are spread over wavefronts.
+// Diagonal tiles (a block with itself) hold ~80% of their 2,016 pairs inside
+// the cutoff for peptides, so they are evaluated densely, without lists or
+// LDS atomics: lane = atom i, the block's atoms rotate one lane per step
+// (DPP wave_ror:1) and carry their own j-side sums with them. Steps 1..32
+// visit every unordered pair once (at step 32 both lanes of a pair meet it;
+// the lower lane takes it). After step 32 the j-side sum of atom a sits in
+// lane (a + 32) & 63. The candidate / exclusion / 1-4 bits of the pair come
+// from the tile's bit rows written at the last rebuild.
+struct DiagBits {
+    unsigned long long mc, mx, m4;
+};
+
+__device__ __forceinline__ DiagBits diag_bits(const DevState &S, int inst, int first, int t) {
+    const long long slot = cand_slot(S, inst, first, t);
+    const unsigned long long *M = reinterpret_cast<const unsigned long long *>(S.cand + slot * kTileCap);
+    const int l = lane_id();
+    return {M[l], M[kBlk + l], M[2 * kBlk + l]};
+}
+
+// does lane l take the pair with lane jlane at rotation step k
+__device__ __forceinline__ bool diag_take(const DiagBits &D, int jlane, int k, int l) {
+    return ((D.mc >> jlane) & 1ull) && (k < 32 || l < 32);
+}
+__device__ __forceinline__ int diag_bo(const DiagBits &D, int jlane) {
+    return ((D.mx >> jlane) & 1ull) ? 2 : (((D.m4 >> jlane) & 1ull) ? 3 : 7);
+}
+template <typename T> __device__ __forceinline__ T diag_home(T v) { return __shfl(v, (lane_id() + 32) & 63, 64); }
+
+// Still's pairwise Born term P4*CCF/r^4 (forcefields/gbff.cpp:448-460)
+template <typename R>
+__device__ __forceinline__ R born_term(R r2, R sR, R invP5, R P4, R P5) {
+    const R x = r2 * Prec<R>::rcp(sR * sR);
+    R ccf = (R)1;
+    if (x <= invP5) {
+        const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(x * P5);
+        ccf = h * h;
+    }
+    return P4 * ccf * Prec<R>::rcp(r2 * r2);
+}
+
 template <typename R> struct BornAtom { R x, y, z, rad, vj; };
 
 template <typename R>
@@ -616,7 +687,7 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
-    const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5;
+    const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5, still2 = (R)P.still2;
     for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
         const int cend = min(t_end, c + tpw);
         for (int t = c; t < cend; ++t) {
@@ -624,6 +695,35 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
+            if ((tw.y >> 24) & kTileDiag) {
+                const DiagBits D = diag_bits(S, inst, S.blk_first[I], t);
+                const size_t a = (size_t)I * kBlk + l;
+                const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+                R rad, vj;
+                if constexpr (Prec<R>::fixed) { const float2 g = S.gb32[a]; rad = g.x; vj = g.y; }
+                else { const double2 g = S.gb64[a]; rad = g.x; vj = g.y; }
+                R xj = pi.x, yj = pi.y, zj = pi.z, rj = rad, vjj = vj, gj = 0;
+                double gi = 0;
+                int np = 0;
+                const int ldir = (rot_i(l) - l) & 63;
+                for (int k = 1; k <= 32; ++k) {
+                    xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj); vjj = rot(vjj); gj = rot(gj);
+                    const int jlane = (l + k * ldir) & 63;
+                    const R dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    if (diag_take(D, jlane, k, l) && !((D.mx >> jlane) & 1ull) && r2 <= still2) {
+                        const R term = born_term<R>(r2, rad + rj, invP5, P4, P5);
+                        gi += (double)(term * vjj);
+                        gj += term * vj;
+                        ++np;
+                    }
+                }
+                const double tot = gi + (double)diag_home(gj);
+                if (l < S.blk_n[I]) acc_add<R>(S.acc_g, a, tot, kGpolScale, S.inst_status + inst);
+                const unsigned int npw = wave_sum((unsigned int)np);
+                if (l == 0 && npw) atomicAdd(S.inst_count + (size_t)inst * kCNum + kCBornPairs, 2ull * npw);
+                continue;
+            }
             const int nb = S.tile_cnt[t].x;
             if (nb == 0) continue;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
@@ -650,14 +750,7 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
                     const BornAtom<R> A = si[w][ii], B = sj[w][jj];
                     const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
                     const R r2 = dx * dx + dy * dy + dz * dz;
-                    const R sR = A.rad + B.rad;
-                    const R x = r2 * Prec<R>::rcp(sR * sR);
-                    R ccf = (R)1;
-                    if (x <= invP5) {
-                        const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(x * P5);
-                        ccf = h * h;
-                    }
-                    const R term = P4 * ccf * Prec<R>::rcp(r2 * r2);
+                    const R term = born_term<R>(r2, A.rad + B.rad, invP5, P4, P5);
                     atomicAdd(&gi_acc[w][ii], (double)(term * B.vj));
                     atomicAdd(&gj_acc[w][jj], (double)(term * A.vj));
                 }
@@ -892,6 +985,53 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
+            if ((tw.y >> 24) & kTileDiag) {
+                const DiagBits D = diag_bits(S, inst, S.blk_first[I], t);
+                const size_t a = (size_t)I * kBlk + l;
+                const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+                PairAtom<R> A;
+                if constexpr (Prec<R>::fixed) {
+                    const float2 u = S.lj32[a];
+                    A = {pi.x, pi.y, pi.z, pi.q, u.x, u.y, ld_born<R>(S, a)};
+                } else {
+                    const double2 u = S.lj64[a];
+                    A = {pi.x, pi.y, pi.z, pi.q, u.x, u.y, ld_born<R>(S, a)};
+                }
+                PairAtom<R> B = A;
+                R fjx = 0, fjy = 0, fjz = 0, daj_acc = 0;
+                double fix = 0, fiy = 0, fiz = 0, dai_acc = 0;
+                LaneEnergy E;
+                const int ldir = (rot_i(l) - l) & 63;
+                for (int k = 1; k <= 32; ++k) {
+                    B.x = rot(B.x); B.y = rot(B.y); B.z = rot(B.z); B.q = rot(B.q);
+                    B.rad = rot(B.rad); B.eps = rot(B.eps); B.a = rot(B.a);
+                    fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
+                    if (MODE == 0) daj_acc = rot(daj_acc);
+                    const int jlane = (l + k * ldir) & 63;
+                    const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    if (diag_take(D, jlane, k, l) && r2 <= cut2) {
+                        R dai = 0, daj = 0;
+                        const R g = pair_term<R, MODE>(P, A, B, diag_bo(D, jlane), r2, E, dai, daj);
+                        const R gx = g * dx, gy = g * dy, gz = g * dz;
+                        fix += (double)gx; fiy += (double)gy; fiz += (double)gz;
+                        fjx -= gx; fjy -= gy; fjz -= gz;
+                        if (MODE == 0) { dai_acc += (double)dai; daj_acc += daj; }
+                    }
+                }
+                int *st = S.inst_status + inst;
+                const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
+                             gz = fiz + (double)diag_home(fjz);
+                const double gd = MODE == 0 ? dai_acc + (double)diag_home(daj_acc) : 0.0;
+                if (l < S.blk_n[I]) {
+                    acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
+                    acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
+                    acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
+                    if (MODE == 0) acc_add<R>(S.acc_deda, a, gd, kForceScale, st);
+                }
+                flush_energy<R>(S, inst, E);
+                continue;
+            }
             const int2 cnt = S.tile_cnt[t];
             const int ntot = cnt.x + cnt.y;
             if (ntot == 0) continue;
@@ -969,6 +1109,22 @@ __device__ __forceinline__ R chain_weight(const DevState &S, size_t a) {
 
 template <typename R> struct ChainAtom { R x, y, z, rad, vj, wt; };
 
+// d/dr of the Still term over r, without the atom weights:
+// P4 (4 CCF - dCCF) / r^6 (forcefields/gbff.cpp:1770-1795)
+template <typename R>
+__device__ __forceinline__ R chain_coef(R r2, R sR, R invP5, R P4, R P5) {
+    const R x = r2 * Prec<R>::rcp(sR * sR);
+    R ccf = (R)1, dccf = (R)0;
+    if (x <= invP5) {
+        const R f = x * P5;
+        const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(f);
+        ccf = h * h;
+        dccf = (R)2 * h * Prec<R>::sinpi_(f) * (f * (R)3.14159265358979323846);
+    }
+    const R r6 = r2 * r2 * r2;
+    return P4 * ((R)4 * ccf - dccf) * Prec<R>::rcp(r6);
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
     __shared__ ChainAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
@@ -977,7 +1133,7 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
-    const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5;
+    const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5, still2 = (R)P.still2;
     for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
         const int cend = min(t_end, c + tpw);
         for (int t = c; t < cend; ++t) {
@@ -985,6 +1141,44 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
+            if ((tw.y >> 24) & kTileDiag) {
+                const DiagBits D = diag_bits(S, inst, S.blk_first[I], t);
+                const size_t a = (size_t)I * kBlk + l;
+                const bool vi = l < S.blk_n[I];
+                const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+                R rad, vjv;
+                if constexpr (Prec<R>::fixed) { const float2 g = S.gb32[a]; rad = g.x; vjv = g.y; }
+                else { const double2 g = S.gb64[a]; rad = g.x; vjv = g.y; }
+                const ChainAtom<R> A = {pi.x, pi.y, pi.z, rad, vjv, vi ? chain_weight<R>(S, a) : (R)0};
+                ChainAtom<R> B = A;
+                R fjx = 0, fjy = 0, fjz = 0;
+                double fix = 0, fiy = 0, fiz = 0;
+                const int ldir = (rot_i(l) - l) & 63;
+                for (int k = 1; k <= 32; ++k) {
+                    B.x = rot(B.x); B.y = rot(B.y); B.z = rot(B.z);
+                    B.rad = rot(B.rad); B.vj = rot(B.vj); B.wt = rot(B.wt);
+                    fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
+                    const int jlane = (l + k * ldir) & 63;
+                    const R dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    if (diag_take(D, jlane, k, l) && !((D.mx >> jlane) & 1ull) && r2 <= still2) {
+                        const R coef = chain_coef<R>(r2, A.rad + B.rad, invP5, P4, P5) *
+                                       (A.wt * B.vj + B.wt * A.vj);
+                        const R gx = coef * dx, gy = coef * dy, gz = coef * dz;
+                        fix += (double)gx; fiy += (double)gy; fiz += (double)gz;
+                        fjx -= gx; fjy -= gy; fjz -= gz;
+                    }
+                }
+                const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
+                             gz = fiz + (double)diag_home(fjz);
+                if (vi) {
+                    int *st = S.inst_status + inst;
+                    acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
+                    acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
+                    acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
+                }
+                continue;
+            }
             const int nb = S.tile_cnt[t].x;
             if (nb == 0) continue;
             const bool vi = l < S.blk_n[I], vj = l < S.blk_n[J];
@@ -1012,17 +1206,7 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
                     const ChainAtom<R> A = si[w][ii], B = sj[w][jj];
                     const R dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
                     const R r2 = dx * dx + dy * dy + dz * dz;
-                    const R sR = A.rad + B.rad;
-                    const R x = r2 * Prec<R>::rcp(sR * sR);
-                    R ccf = (R)1, dccf = (R)0;
-                    if (x <= invP5) {
-                        const R f = x * P5;
-                        const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(f);
-                        ccf = h * h;
-                        dccf = (R)2 * h * Prec<R>::sinpi_(f) * (f * (R)3.14159265358979323846);
-                    }
-                    const R r6 = r2 * r2 * r2;
-                    const R coef = P4 * ((R)4 * ccf - dccf) * Prec<R>::rcp(r6) * (A.wt * B.vj + B.wt * A.vj);
+                    const R coef = chain_coef<R>(r2, A.rad + B.rad, invP5, P4, P5) * (A.wt * B.vj + B.wt * A.vj);
                     const R gx = coef * dx, gy = coef * dy, gz = coef * dz;
                     atomicAdd(&fi[w][0][ii], gx);
                     atomicAdd(&fi[w][1][ii], gy);
