@@ -81,7 +81,9 @@ typedef struct pdgpu_options {
     int32_t max_instances;        /* instance slots; 0: 8192                       */
     int32_t components;           /* PDGPU_FF_* bits evaluated; default all        */
     int32_t use_graphs;           /* 1: step loops replay captured HIP graphs      */
-    int32_t reserved;
+    int32_t dense_tile_pairs;     /* off-diagonal tiles with at least this many    */
+                                  /* candidate pairs are evaluated densely (lane   */
+                                  /* rotation, no lists); 0: never                 */
 } pdgpu_options;
 
 /* Forcefield components of a context: FF_GeneralizedBorn_Still and the four

@@ -331,6 +331,9 @@ void setup_params(pdgpu_ctx *c) {
     P.fs_cutV = P.fs_sA * (P.fs_sB * (1.0 / rc) + P.fs_sC * rc + P.fs_sD * rc * rc * rc +
                            P.fs_sE * rc * rc * rc * rc2);
     P.fs_eshift = (ri < rc) ? 1 / ri + (fin - P.fs_cutV) : 0;
+    // off by default: for off-diagonal tiles of peptides the filtered list
+    // with LDS accumulation is faster than lane rotation at every threshold
+    P.dense_pairs = o.dense_tile_pairs > 0 ? o.dense_tile_pairs : kTileCap + 1;
     P.elec_mode = o.elec_mode;
     P.vdw_mode = o.vdw_mode;
 }

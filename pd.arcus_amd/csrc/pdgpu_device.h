@@ -66,6 +66,7 @@ struct PairParams {
     double fs_sA, fs_sB, fs_sC, fs_sD, fs_sE, fs_cutV, fs_eshift;
     double cut, inner2;
     int elec_mode, vdw_mode;
+    int dense_pairs; // off-diagonal tiles with at least this many candidates are stored dense
 };
 
 // Device pointers of one context.
@@ -547,13 +548,13 @@ __global__ void __launch_bounds__(256) k_cand(DevState S, PairParams P, int blk_
                 continue;
             }
             int n = 0;
+            unsigned long long mc = 0, mx = 0, m4 = 0;
             for (int k = 0; k < kBlk; ++k) {
                 if (k > 0) { xj = rot(xj); yj = rot(yj); zj = rot(zj); }
                 const int jlane = (l + k * ldir) & 63;
                 const R dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
                 const R r2 = dx * dx + dy * dy + dz * dz;
-                // in a diagonal tile jl < il is jlane < l
-                const bool ok = vi && jlane < nJ && (!diag || jlane < l) && r2 <= reach2;
+                const bool ok = vi && jlane < nJ && r2 <= reach2;
                 int bo = 7;
                 if (ok && lookup) {
                     const int jl = jl0 + jlane;
@@ -561,8 +562,24 @@ __global__ void __launch_bounds__(256) k_cand(DevState S, PairParams P, int blk_
                                 : pair_bondorder(S, inst, il, jl, fl);
                 }
                 const unsigned long long m = __ballot(ok);
-                if (ok) L[n + (int)__popcll(m & below)] = (uint16_t)(l | (jlane << 6) | (bo << 12));
+                if (ok) {
+                    L[n + (int)__popcll(m & below)] = (uint16_t)(l | (jlane << 6) | (bo << 12));
+                    const unsigned long long bit = 1ull << jlane;
+                    mc |= bit;
+                    if (bo <= 2) mx |= bit;
+                    if (bo == 3) m4 |= bit;
+                }
                 n += (int)__popcll(m);
+            }
+            if (n >= P.dense_pairs) {
+                // dense enough that lane rotation beats list + LDS atomics:
+                // keep the bit rows instead (cand_n < 0 marks the format)
+                wave_lds_sync();
+                unsigned long long *M = reinterpret_cast<unsigned long long *>(L);
+                M[l] = mc;
+                M[kBlk + l] = mx;
+                M[2 * kBlk + l] = m4;
+                n = -1;
             }
             if (l == 0) S.cand_n[slot] = n;
             wave_lds_sync();
@@ -598,6 +615,10 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
             }
             const long long slot = cand_slot(S, inst, first, t);
             const int n = S.cand_n[slot];
+            if (n < 0) { // dense off-diagonal tile
+                if (l == 0) S.tile_cnt[t] = make_int2(-1, -1);
+                continue;
+            }
             const uint16_t *C = S.cand + slot * kTileCap;
             uint16_t *L = S.plist + (size_t)t * kTileCap;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
@@ -637,14 +658,16 @@ __device__ __forceinline__ int entry_pos(int q, int nb) { return q < nb ? q : kT
 // cutoff (forcefields/gbff.cpp:432-465), both directions per unordered pair.
 // Every accumulator is flushed at the end of each tile, so the sums an atom
 // receives do not depend on how tiles are spread over wavefronts.
-// Diagonal tiles (a block with itself) hold ~80% of their 2,016 pairs inside
-// the cutoff for peptides, so they are evaluated densely, without lists or
-// LDS atomics: lane = atom i, the block's atoms rotate one lane per step
-// (DPP wave_ror:1) and carry their own j-side sums with them. Steps 1..32
+// Dense tiles are evaluated without lists or LDS atomics: lane = atom i, the
+// j atoms rotate one lane per step (DPP wave_ror:1) and carry their own
+// j-side sums with them. Diagonal tiles (a block with itself, ~80% of their
+// 2,016 pairs inside the cutoff for peptides) are always dense: steps 1..32
 // visit every unordered pair once (at step 32 both lanes of a pair meet it;
-// the lower lane takes it). After step 32 the j-side sum of atom a sits in
-// lane (a + 32) & 63. The candidate / exclusion / 1-4 bits of the pair come
-// from the tile's bit rows written at the last rebuild.
+// the lower lane takes it) and after step 32 the j-side sum of atom a sits
+// in lane (a + 32) & 63. Off-diagonal tiles are dense when their candidate
+// count reached PairParams::dense_pairs at the last rebuild: 64 steps, after
+// which the j-side sums are home again. The candidate / exclusion / 1-4 bits
+// of each pair come from the tile's bit rows written at that rebuild.
 struct DiagBits {
     unsigned long long mc, mx, m4;
 };
@@ -657,8 +680,8 @@ __device__ __forceinline__ DiagBits diag_bits(const DevState &S, int inst, int f
 }
 
 // does lane l take the pair with lane jlane at rotation step k
-__device__ __forceinline__ bool diag_take(const DiagBits &D, int jlane, int k, int l) {
-    return ((D.mc >> jlane) & 1ull) && (k < 32 || l < 32);
+__device__ __forceinline__ bool diag_take(const DiagBits &D, int jlane, int k, int l, bool diag) {
+    return ((D.mc >> jlane) & 1ull) && (!diag || k < 32 || l < 32);
 }
 __device__ __forceinline__ int diag_bo(const DiagBits &D, int jlane) {
     return ((D.mx >> jlane) & 1ull) ? 2 : (((D.m4 >> jlane) & 1ull) ? 3 : 7);
@@ -695,36 +718,51 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
-            if ((tw.y >> 24) & kTileDiag) {
+            const bool diag = (tw.y >> 24) & kTileDiag;
+            const int2 cnt = S.tile_cnt[t];
+            if (diag || cnt.x < 0) {
                 const DiagBits D = diag_bits(S, inst, S.blk_first[I], t);
-                const size_t a = (size_t)I * kBlk + l;
+                const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
                 const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
-                R rad, vj;
-                if constexpr (Prec<R>::fixed) { const float2 g = S.gb32[a]; rad = g.x; vj = g.y; }
-                else { const double2 g = S.gb64[a]; rad = g.x; vj = g.y; }
-                R xj = pi.x, yj = pi.y, zj = pi.z, rj = rad, vjj = vj, gj = 0;
+                const AtomXYZ<R> pj = diag ? pi : load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+                R rad, vj, rj, vjj;
+                if constexpr (Prec<R>::fixed) {
+                    const float2 g = S.gb32[a], h = S.gb32[b];
+                    rad = g.x; vj = g.y; rj = h.x; vjj = h.y;
+                } else {
+                    const double2 g = S.gb64[a], h = S.gb64[b];
+                    rad = g.x; vj = g.y; rj = h.x; vjj = h.y;
+                }
+                R xj = pj.x, yj = pj.y, zj = pj.z, gj = 0;
                 double gi = 0;
                 int np = 0;
                 const int ldir = (rot_i(l) - l) & 63;
-                for (int k = 1; k <= 32; ++k) {
+                const int steps = diag ? 32 : kBlk;
+                for (int k = 1; k <= steps; ++k) {
                     xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj); vjj = rot(vjj); gj = rot(gj);
                     const int jlane = (l + k * ldir) & 63;
                     const R dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
                     const R r2 = dx * dx + dy * dy + dz * dz;
-                    if (diag_take(D, jlane, k, l) && !((D.mx >> jlane) & 1ull) && r2 <= still2) {
+                    if (diag_take(D, jlane, k, l, diag) && !((D.mx >> jlane) & 1ull) && r2 <= still2) {
                         const R term = born_term<R>(r2, rad + rj, invP5, P4, P5);
                         gi += (double)(term * vjj);
                         gj += term * vj;
                         ++np;
                     }
                 }
-                const double tot = gi + (double)diag_home(gj);
-                if (l < S.blk_n[I]) acc_add<R>(S.acc_g, a, tot, kGpolScale, S.inst_status + inst);
+                int *st = S.inst_status + inst;
+                if (diag) {
+                    const double tot = gi + (double)diag_home(gj);
+                    if (l < S.blk_n[I]) acc_add<R>(S.acc_g, a, tot, kGpolScale, st);
+                } else { // 64 rotations bring the j-side sums home
+                    if (l < S.blk_n[I]) acc_add<R>(S.acc_g, a, gi, kGpolScale, st);
+                    if (l < S.blk_n[J]) acc_add<R>(S.acc_g, b, (double)gj, kGpolScale, st);
+                }
                 const unsigned int npw = wave_sum((unsigned int)np);
                 if (l == 0 && npw) atomicAdd(S.inst_count + (size_t)inst * kCNum + kCBornPairs, 2ull * npw);
                 continue;
             }
-            const int nb = S.tile_cnt[t].x;
+            const int nb = cnt.x;
             if (nb == 0) continue;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
@@ -985,24 +1023,29 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
-            if ((tw.y >> 24) & kTileDiag) {
+            const bool diag = (tw.y >> 24) & kTileDiag;
+            const int2 cnt = S.tile_cnt[t];
+            if (diag || cnt.x < 0) {
                 const DiagBits D = diag_bits(S, inst, S.blk_first[I], t);
-                const size_t a = (size_t)I * kBlk + l;
+                const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
                 const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
-                PairAtom<R> A;
+                const AtomXYZ<R> pj = diag ? pi : load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+                PairAtom<R> A, B;
                 if constexpr (Prec<R>::fixed) {
-                    const float2 u = S.lj32[a];
+                    const float2 u = S.lj32[a], v = S.lj32[b];
                     A = {pi.x, pi.y, pi.z, pi.q, u.x, u.y, ld_born<R>(S, a)};
+                    B = {pj.x, pj.y, pj.z, pj.q, v.x, v.y, ld_born<R>(S, b)};
                 } else {
-                    const double2 u = S.lj64[a];
+                    const double2 u = S.lj64[a], v = S.lj64[b];
                     A = {pi.x, pi.y, pi.z, pi.q, u.x, u.y, ld_born<R>(S, a)};
+                    B = {pj.x, pj.y, pj.z, pj.q, v.x, v.y, ld_born<R>(S, b)};
                 }
-                PairAtom<R> B = A;
                 R fjx = 0, fjy = 0, fjz = 0, daj_acc = 0;
                 double fix = 0, fiy = 0, fiz = 0, dai_acc = 0;
                 LaneEnergy E;
                 const int ldir = (rot_i(l) - l) & 63;
-                for (int k = 1; k <= 32; ++k) {
+                const int steps = diag ? 32 : kBlk;
+                for (int k = 1; k <= steps; ++k) {
                     B.x = rot(B.x); B.y = rot(B.y); B.z = rot(B.z); B.q = rot(B.q);
                     B.rad = rot(B.rad); B.eps = rot(B.eps); B.a = rot(B.a);
                     fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
@@ -1010,7 +1053,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                     const int jlane = (l + k * ldir) & 63;
                     const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
                     const R r2 = dx * dx + dy * dy + dz * dz;
-                    if (diag_take(D, jlane, k, l) && r2 <= cut2) {
+                    if (diag_take(D, jlane, k, l, diag) && r2 <= cut2) {
                         R dai = 0, daj = 0;
                         const R g = pair_term<R, MODE>(P, A, B, diag_bo(D, jlane), r2, E, dai, daj);
                         const R gx = g * dx, gy = g * dy, gz = g * dz;
@@ -1020,19 +1063,33 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                     }
                 }
                 int *st = S.inst_status + inst;
-                const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
-                             gz = fiz + (double)diag_home(fjz);
-                const double gd = MODE == 0 ? dai_acc + (double)diag_home(daj_acc) : 0.0;
-                if (l < S.blk_n[I]) {
-                    acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
-                    acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
-                    acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
-                    if (MODE == 0) acc_add<R>(S.acc_deda, a, gd, kForceScale, st);
+                if (diag) {
+                    const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
+                                 gz = fiz + (double)diag_home(fjz);
+                    const double gd = MODE == 0 ? dai_acc + (double)diag_home(daj_acc) : 0.0;
+                    if (l < S.blk_n[I]) {
+                        acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
+                        acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
+                        if (MODE == 0) acc_add<R>(S.acc_deda, a, gd, kForceScale, st);
+                    }
+                } else { // 64 rotations bring the j-side sums home
+                    if (l < S.blk_n[I]) {
+                        acc_add<R>(S.acc_fx, a, fix, kForceScale, st);
+                        acc_add<R>(S.acc_fy, a, fiy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, a, fiz, kForceScale, st);
+                        if (MODE == 0) acc_add<R>(S.acc_deda, a, dai_acc, kForceScale, st);
+                    }
+                    if (l < S.blk_n[J]) {
+                        acc_add<R>(S.acc_fx, b, (double)fjx, kForceScale, st);
+                        acc_add<R>(S.acc_fy, b, (double)fjy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, b, (double)fjz, kForceScale, st);
+                        if (MODE == 0) acc_add<R>(S.acc_deda, b, (double)daj_acc, kForceScale, st);
+                    }
                 }
                 flush_energy<R>(S, inst, E);
                 continue;
             }
-            const int2 cnt = S.tile_cnt[t];
             const int ntot = cnt.x + cnt.y;
             if (ntot == 0) continue;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
@@ -1141,27 +1198,36 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
-            if ((tw.y >> 24) & kTileDiag) {
+            const bool diag = (tw.y >> 24) & kTileDiag;
+            const int2 cnt = S.tile_cnt[t];
+            if (diag || cnt.x < 0) {
                 const DiagBits D = diag_bits(S, inst, S.blk_first[I], t);
-                const size_t a = (size_t)I * kBlk + l;
-                const bool vi = l < S.blk_n[I];
+                const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
+                const bool vi = l < S.blk_n[I], vjb = l < S.blk_n[J];
                 const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
-                R rad, vjv;
-                if constexpr (Prec<R>::fixed) { const float2 g = S.gb32[a]; rad = g.x; vjv = g.y; }
-                else { const double2 g = S.gb64[a]; rad = g.x; vjv = g.y; }
+                const AtomXYZ<R> pj = diag ? pi : load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+                R rad, vjv, rj, vjj;
+                if constexpr (Prec<R>::fixed) {
+                    const float2 g = S.gb32[a], h = S.gb32[b];
+                    rad = g.x; vjv = g.y; rj = h.x; vjj = h.y;
+                } else {
+                    const double2 g = S.gb64[a], h = S.gb64[b];
+                    rad = g.x; vjv = g.y; rj = h.x; vjj = h.y;
+                }
                 const ChainAtom<R> A = {pi.x, pi.y, pi.z, rad, vjv, vi ? chain_weight<R>(S, a) : (R)0};
-                ChainAtom<R> B = A;
+                ChainAtom<R> B = {pj.x, pj.y, pj.z, rj, vjj, vjb ? chain_weight<R>(S, b) : (R)0};
                 R fjx = 0, fjy = 0, fjz = 0;
                 double fix = 0, fiy = 0, fiz = 0;
                 const int ldir = (rot_i(l) - l) & 63;
-                for (int k = 1; k <= 32; ++k) {
+                const int steps = diag ? 32 : kBlk;
+                for (int k = 1; k <= steps; ++k) {
                     B.x = rot(B.x); B.y = rot(B.y); B.z = rot(B.z);
                     B.rad = rot(B.rad); B.vj = rot(B.vj); B.wt = rot(B.wt);
                     fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
                     const int jlane = (l + k * ldir) & 63;
                     const R dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
                     const R r2 = dx * dx + dy * dy + dz * dz;
-                    if (diag_take(D, jlane, k, l) && !((D.mx >> jlane) & 1ull) && r2 <= still2) {
+                    if (diag_take(D, jlane, k, l, diag) && !((D.mx >> jlane) & 1ull) && r2 <= still2) {
                         const R coef = chain_coef<R>(r2, A.rad + B.rad, invP5, P4, P5) *
                                        (A.wt * B.vj + B.wt * A.vj);
                         const R gx = coef * dx, gy = coef * dy, gz = coef * dz;
@@ -1169,17 +1235,30 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
                         fjx -= gx; fjy -= gy; fjz -= gz;
                     }
                 }
-                const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
-                             gz = fiz + (double)diag_home(fjz);
-                if (vi) {
-                    int *st = S.inst_status + inst;
-                    acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
-                    acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
-                    acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
+                int *st = S.inst_status + inst;
+                if (diag) {
+                    const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
+                                 gz = fiz + (double)diag_home(fjz);
+                    if (vi) {
+                        acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
+                        acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
+                    }
+                } else { // 64 rotations bring the j-side sums home
+                    if (vi) {
+                        acc_add<R>(S.acc_fx, a, fix, kForceScale, st);
+                        acc_add<R>(S.acc_fy, a, fiy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, a, fiz, kForceScale, st);
+                    }
+                    if (vjb) {
+                        acc_add<R>(S.acc_fx, b, (double)fjx, kForceScale, st);
+                        acc_add<R>(S.acc_fy, b, (double)fjy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, b, (double)fjz, kForceScale, st);
+                    }
                 }
                 continue;
             }
-            const int nb = S.tile_cnt[t].x;
+            const int nb = cnt.x;
             if (nb == 0) continue;
             const bool vi = l < S.blk_n[I], vj = l < S.blk_n[J];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;

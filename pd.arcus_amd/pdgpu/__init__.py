@@ -70,7 +70,7 @@ class Options(C.Structure):
                    "exp_approx_threshold", "dielectric_offset", "born_radius_offset",
                    "p1", "p2", "p3", "p4", "p5", "nlist_padding")] + \
                [("device_memory_budget", C.c_int64), ("max_instances", C.c_int32), ("components", C.c_int32),
-                ("use_graphs", C.c_int32), ("reserved", C.c_int32)]
+                ("use_graphs", C.c_int32), ("dense_tile_pairs", C.c_int32)]
 
 
 class Bonded(C.Structure):

@@ -171,3 +171,21 @@ def test_remove_and_reuse_slots():
         res = ctx.calc_forces(pdgpu.PDGPU_ALL)
         for g, i in ((gs[0], ids[0]), (gs[1], j), (gs[2], ids[2])):
             assert rel(res[i]["epot"], g["epot"]) <= 1e-10
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("precision", [pdgpu.FP64, pdgpu.MIXED])
+def test_dense_offdiagonal_tiles(name, precision):
+    """Every off-diagonal tile evaluated by lane rotation from its bit rows
+    (dense_tile_pairs=1) meets the same golden tolerances as the list path."""
+    g = G.load(name)
+    tol = 1e-10 if precision == pdgpu.FP64 else 1e-5
+    with context_for(g, precision, dense_tile_pairs=1) as ctx:
+        i = ctx.add_system(g)
+        e, f = ctx.calc_forces(i)
+        assert e["status"] == 0
+        assert rel(e["epot"], g["epot"]) <= tol
+        assert rms_rel(f, g["force"]) <= (1e-9 if precision == pdgpu.FP64 else 1e-4)
+        e, f = ctx.calc_energies(i)
+        assert rel(e["epot"], g["en_epot"]) <= tol
+        assert rms_rel(f, g["en_force"]) <= (1e-9 if precision == pdgpu.FP64 else 1e-4)
