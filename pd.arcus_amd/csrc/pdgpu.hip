@@ -305,6 +305,8 @@ void setup_params(pdgpu_ctx *c) {
     P.s14e = o.elec14_scaling;
     P.econv_in = econv * (1.0 / o.dielectric_solute);
     P.econv_out = econv * (1.0 / o.dielectric_solvent);
+    P.econv_in32 = (float)P.econv_in;
+    P.econv_out32 = (float)P.econv_out;
     P.premul = -(econv * 0.5) * (1 / o.dielectric_solute - 1 / o.dielectric_solvent);
     // the list radius is max(Cutoff, VdwCutoff) + Padding
     // (forcefields/nonbonded.cpp:1013, workspace/neighbourlist.cpp:258)

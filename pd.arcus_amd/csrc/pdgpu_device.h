@@ -54,6 +54,7 @@ struct PairParams {
     double s14v, s14e;     // 1-4 scalings
     double econv_in;       // econv / DielectricSolute
     double econv_out;      // econv / DielectricSolvent
+    float econv_in32, econv_out32; // the same rounded once, for the FP32 pair maths
     double premul;         // -halfeconv*(1/eps_in - 1/eps_out)
     double still2;         // min(GbsaStillCutoff, list radius)^2
     double invP5, P4, P5;
@@ -621,6 +622,7 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
             }
             const uint16_t *C = S.cand + slot * kTileCap;
             uint16_t *L = S.plist + (size_t)t * kTileCap;
+            int e_nx = l < n ? (int)C[l] : 0; // candidates are fetched one chunk ahead
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -631,7 +633,8 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
             for (int q0 = 0; q0 < n; q0 += kBlk) {
                 const int q = q0 + l;
                 const bool in = q < n;
-                const int e = in ? (int)C[q] : 0;
+                const int e = e_nx;
+                e_nx = q + kBlk < n ? (int)C[q + kBlk] : 0;
                 const PosR<R> A = si[w][e & 63], B = sj[w][(e >> 6) & 63];
                 const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
                 const R r2 = dx * dx + dy * dy + dz * dz;
@@ -764,6 +767,9 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
             }
             const int nb = cnt.x;
             if (nb == 0) continue;
+            // list entries are fetched one chunk ahead of their use
+            const uint16_t *L = S.plist + (size_t)t * kTileCap;
+            int e_nx = l < nb ? (int)L[l] : 0;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -779,11 +785,11 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
             gi_acc[w][l] = 0;
             gj_acc[w][l] = 0;
             wave_lds_sync();
-            const uint16_t *L = S.plist + (size_t)t * kTileCap;
             for (int q0 = 0; q0 < nb; q0 += kBlk) {
                 const int q = q0 + l;
+                const int e = e_nx;
+                e_nx = q + kBlk < nb ? (int)L[q + kBlk] : 0;
                 if (q < nb) {
-                    const int e = L[q];
                     const int ii = e & 63, jj = (e >> 6) & 63;
                     const BornAtom<R> A = si[w][ii], B = sj[w][jj];
                     const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
@@ -863,6 +869,13 @@ __device__ __forceinline__ void switch_fn(R r, R inner, R invSw, R &S, R &dS) {
     S = s * s;
 }
 
+// The same without a branch: below the inner radius d clamps to 0, giving
+// S = 1 and dS = -0, so S*x + y*dS == x exactly there.
+template <typename R>
+__device__ __forceinline__ void switch_fn_clamped(R r, R inner, R invSw, R &S, R &dS) {
+    switch_fn<R>(fmax(r, inner), inner, invSw, S, dS);
+}
+
 template <typename R> struct PairAtom { R x, y, z, q, rad, eps, a; };
 
 // One pair of the fused pass. Returns (dE/dr)/r (multiplies xi - xj) and
@@ -879,67 +892,57 @@ __device__ __forceinline__ R pair_term(const PairParams &P, const PairAtom<R> &A
     R g = 0;
     E.pairs++;
     if (MODE == 0) {
+        // Near pairs (u above the threshold) take the GB term and the
+        // solute-dielectric Coulomb, far pairs the solvent-dielectric Coulomb
+        // only; pairs of bond order <= 2 take no Coulomb and no LJ. Only the
+        // GB term is a branch: the Coulomb and LJ parts run for every lane
+        // with zero factors where they do not apply, so lanes of a wavefront
+        // that disagree on near/far or on the switching regions do not run
+        // the same code twice.
         const R aiaj = A.a * B.a;
         const R u = (R)(-0.25) * r2 * Prec<R>::rcp(aiaj);
+        const bool b3 = bo >= 3, near = u > (R)P.exp_thresh;
         const R s14e = (bo == 3) ? (R)P.s14e : (R)1;
-        R elec_pot = 0, elec_force = 0;
-        if (u > (R)P.exp_thresh) {
+        R Sw, dS;
+        switch_fn_clamped<R>(r, inner, invSw, Sw, dS);
+        R econv;
+        if constexpr (Prec<R>::fixed) econv = near ? P.econv_in32 : P.econv_out32;
+        else econv = near ? P.econv_in : P.econv_out;
+        R elec_pot = b3 ? econv * s14e * qiqj * rinv : (R)0;
+        const R elec_force = Sw * (-elec_pot * rinv) + elec_pot * dS;
+        elec_pot *= Sw;
+        if (near) {
             const R tg = Prec<R>::expo(u);
             const R fgb = Prec<R>::rsqrt(r2 + tg * aiaj);
             const R L = fgb * fgb;
             R epol = premul * qiqj * fgb;
             R dedr = -epol * (r - (R)0.25 * r * tg) * L;
             R dedal = -epol * tg * ((R)1 - u) * L;
-            if (bo >= 3) {
-                elec_pot = (R)P.econv_in * s14e * qiqj * rinv;
-                elec_force = -elec_pot * rinv;
-            }
-            if (r >= inner) {
-                R Sw, dS;
-                switch_fn<R>(r, inner, invSw, Sw, dS);
-                elec_force = Sw * elec_force + elec_pot * dS;
-                elec_pot *= Sw;
-                dedr = Sw * dedr + dS * epol;
-                dedal *= Sw;
-                epol *= Sw;
-            }
+            dedr = Sw * dedr + dS * epol;
+            dedal *= Sw;
+            epol *= Sw;
             g = dedr * (R)2 * rinv;
             E.pol += 2.0 * (double)epol;
             dai = dedal * B.a;
             daj = dedal * A.a;
-        } else if (bo >= 3) {
-            // far pair: GB dropped, solvent-dielectric Coulomb instead
-            elec_pot = (R)P.econv_out * s14e * qiqj * rinv;
-            elec_force = -elec_pot * rinv;
-            if (r >= inner) {
-                R Sw, dS;
-                switch_fn<R>(r, inner, invSw, Sw, dS);
-                elec_force = Sw * elec_force + elec_pot * dS;
-                elec_pot *= Sw;
-            }
         }
-        if (bo >= 3) {
-            R vdw_force = 0;
-            if (r < vcut) {
-                const R s14v = (bo == 3) ? (R)P.s14v : (R)1;
-                const R rr = (A.rad + B.rad) * rinv;
-                const R rr2 = rr * rr;
-                const R Bp = rr2 * rr2 * rr2;
-                const R Ap = Bp * Bp;
-                const R eps = s14v * A.eps * B.eps;
-                R vdw_pot = ((R)2 * eps) * ((R)0.5 * Ap - Bp);
-                vdw_force = -((R)12 * eps * rinv) * (Ap - Bp);
-                if (r > vinner) {
-                    R vS, vd;
-                    switch_fn<R>(r, vinner, vinvSw, vS, vd);
-                    vdw_force = vS * vdw_force + vdw_pot * vd;
-                    vdw_pot *= vS;
-                }
-                E.vdw += (double)vdw_pot;
-            }
-            E.elec += (double)elec_pot;
-            g += (vdw_force + elec_force) * rinv;
-        }
+        // rr = 0 for pairs without LJ keeps bonded pairs at tiny separations
+        // (a collapsing structure) from forming 0 * inf
+        const bool lj = b3 && r < vcut;
+        const R s14v = (bo == 3) ? (R)P.s14v : (R)1;
+        const R rr = lj ? (A.rad + B.rad) * rinv : (R)0;
+        const R rr2 = rr * rr;
+        const R Bp = rr2 * rr2 * rr2;
+        const R Ap = Bp * Bp;
+        const R eps = lj ? s14v * A.eps * B.eps : (R)0;
+        R vS, vd;
+        switch_fn_clamped<R>(r, vinner, vinvSw, vS, vd);
+        R vdw_pot = ((R)2 * eps) * ((R)0.5 * Ap - Bp);
+        const R vdw_force = vS * (-((R)12 * eps * rinv) * (Ap - Bp)) + vdw_pot * vd;
+        vdw_pot *= vS;
+        E.vdw += (double)vdw_pot;
+        E.elec += (double)elec_pot;
+        g += (vdw_force + elec_force) * rinv;
     } else {
         // FF_NonBonded energy path, bond-order scale tables {0,0,0,s14,1,1,1,1}
         const R sv = (bo <= 2) ? (R)0 : (bo == 3 ? (R)P.s14v : (R)1);
@@ -1092,6 +1095,9 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
             }
             const int ntot = cnt.x + cnt.y;
             if (ntot == 0) continue;
+            // list entries are fetched one chunk ahead of their use
+            const uint16_t *L = S.plist + (size_t)t * kTileCap;
+            int e_nx = l < ntot ? (int)L[entry_pos(l, cnt.x)] : 0;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -1107,12 +1113,12 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
             sj[w][l] = {pj.x, pj.y, pj.z, pj.q, Rj, ej, ld_born<R>(S, b)};
             for (int k = 0; k < 4; ++k) { fi[w][k][l] = 0; fj[w][k][l] = 0; }
             wave_lds_sync();
-            const uint16_t *L = S.plist + (size_t)t * kTileCap;
             LaneEnergy E;
             for (int q0 = 0; q0 < ntot; q0 += kBlk) {
                 const int q = q0 + l;
+                const int e = e_nx;
+                e_nx = q + kBlk < ntot ? (int)L[entry_pos(q + kBlk, cnt.x)] : 0;
                 if (q < ntot) {
-                    const int e = L[entry_pos(q, cnt.x)];
                     const int ii = e & 63, jj = (e >> 6) & 63, bo = (e >> 12) & 7;
                     const PairAtom<R> A = si[w][ii], B = sj[w][jj];
                     const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
@@ -1260,6 +1266,9 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
             }
             const int nb = cnt.x;
             if (nb == 0) continue;
+            // list entries are fetched one chunk ahead of their use
+            const uint16_t *L = S.plist + (size_t)t * kTileCap;
+            int e_nx = l < nb ? (int)L[l] : 0;
             const bool vi = l < S.blk_n[I], vj = l < S.blk_n[J];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
@@ -1276,11 +1285,11 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
             sj[w][l] = {pj.x, pj.y, pj.z, rj, vjj, vj ? chain_weight<R>(S, b) : (R)0};
             for (int k = 0; k < 3; ++k) { fi[w][k][l] = 0; fj[w][k][l] = 0; }
             wave_lds_sync();
-            const uint16_t *L = S.plist + (size_t)t * kTileCap;
             for (int q0 = 0; q0 < nb; q0 += kBlk) {
                 const int q = q0 + l;
+                const int e = e_nx;
+                e_nx = q + kBlk < nb ? (int)L[q + kBlk] : 0;
                 if (q < nb) {
-                    const int e = L[q];
                     const int ii = e & 63, jj = (e >> 6) & 63;
                     const ChainAtom<R> A = si[w][ii], B = sj[w][jj];
                     const R dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
