@@ -83,7 +83,7 @@ typedef struct pdgpu_options {
     int32_t use_graphs;           /* 1: step loops replay captured HIP graphs      */
     int32_t dense_tile_pairs;     /* off-diagonal tiles with at least this many    */
                                   /* candidate pairs are evaluated densely (lane   */
-                                  /* rotation, no lists); 0: never                 */
+                                  /* rotation, no lists); 0: 3584 (7/8 full)       */
 } pdgpu_options;
 
 /* Forcefield components of a context: FF_GeneralizedBorn_Still and the four

@@ -297,10 +297,12 @@ void setup_params(pdgpu_ctx *c) {
     P.cut2 = o.cutoff * o.cutoff;
     P.inner = o.inner_cutoff;
     P.inner2 = o.inner_cutoff * o.inner_cutoff;
-    P.invSw = 1.0 / (o.cutoff - o.inner_cutoff);
+    // a switching width of zero (InnerCutoff >= Cutoff) switches nothing
+    // inside the cutoff; 0 keeps the branch-free switch finite there
+    P.invSw = o.cutoff > o.inner_cutoff ? 1.0 / (o.cutoff - o.inner_cutoff) : 0.0;
     P.vcut = o.vdw_cutoff;
     P.vinner = o.vdw_inner_cutoff;
-    P.vinvSw = 1.0 / (o.vdw_cutoff - o.vdw_inner_cutoff);
+    P.vinvSw = o.vdw_cutoff > o.vdw_inner_cutoff ? 1.0 / (o.vdw_cutoff - o.vdw_inner_cutoff) : 0.0;
     P.s14v = o.vdw14_scaling;
     P.s14e = o.elec14_scaling;
     P.econv_in = econv * (1.0 / o.dielectric_solute);
@@ -333,9 +335,9 @@ void setup_params(pdgpu_ctx *c) {
     P.fs_cutV = P.fs_sA * (P.fs_sB * (1.0 / rc) + P.fs_sC * rc + P.fs_sD * rc * rc * rc +
                            P.fs_sE * rc * rc * rc * rc2);
     P.fs_eshift = (ri < rc) ? 1 / ri + (fin - P.fs_cutV) : 0;
-    // off by default: for off-diagonal tiles of peptides the filtered list
-    // with LDS accumulation is faster than lane rotation at every threshold
-    P.dense_pairs = o.dense_tile_pairs > 0 ? o.dense_tile_pairs : kTileCap + 1;
+    // default: tiles at least 7/8 full (all-pairs GB, compact clusters);
+    // at lower fill the filtered list with LDS accumulation is faster
+    P.dense_pairs = o.dense_tile_pairs > 0 ? o.dense_tile_pairs : kTileCap / 8 * 7;
     P.elec_mode = o.elec_mode;
     P.vdw_mode = o.vdw_mode;
 }

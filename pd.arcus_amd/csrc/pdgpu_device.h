@@ -48,9 +48,9 @@ constexpr double kFixedLimit = 4.0e18;            // |value*scale| beyond this i
 struct PairParams {
     double cut2;           // Cutoff^2
     double inner;          // InnerCutoff
-    double invSw;          // 1/(Cutoff-InnerCutoff)
+    double invSw;          // 1/(Cutoff-InnerCutoff), 0 for a zero width
     double vcut, vinner;   // VdwCutoff, VdwInnerCutoff
-    double vinvSw;         // 1/(VdwCutoff-VdwInnerCutoff)
+    double vinvSw;         // 1/(VdwCutoff-VdwInnerCutoff), 0 for a zero width
     double s14v, s14e;     // 1-4 scalings
     double econv_in;       // econv / DielectricSolute
     double econv_out;      // econv / DielectricSolvent
@@ -656,6 +656,76 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
 // front, the others from the back.
 __device__ __forceinline__ int entry_pos(int q, int nb) { return q < nb ? q : kTileCap - 1 - (q - nb); }
 
+struct LaneEnergy {
+    double vdw = 0, elec = 0, pol = 0, vir = 0;
+    unsigned int pairs = 0;
+};
+
+// Per-wavefront running sums of the instance-level results of the pair
+// passes (energies, pair counts) over the tiles the wavefront processes,
+// written with one atomic per field when the wavefront moves on to another
+// instance and at its end. A single large instance would otherwise take one
+// atomic per tile on the same few addresses. In fixed point every tile's
+// energy is rounded on its own before it is added, exactly as a per-tile
+// atomic would round it, so totals do not depend on how tiles are spread
+// over wavefronts.
+template <typename R> struct InstSink {
+    using Acc = typename std::conditional<Prec<R>::fixed, long long, double>::type;
+    int inst = -1;
+    Acc e[4] = {0, 0, 0, 0}; // vdw, elec, pol cross, virial
+    unsigned long long n[kCNum] = {0, 0};
+    int bad = 0;
+
+    __device__ __forceinline__ void to(const DevState &S, int next) {
+        if (next != inst) {
+            flush(S);
+            inst = next;
+        }
+    }
+    __device__ __forceinline__ void energy(int k, double v) {
+        if (v == 0.0) return;
+        if constexpr (Prec<R>::fixed) {
+            const double sv = v * kEnergyScale;
+            if (!(fabs(sv) < kFixedLimit)) {
+                bad |= (sv == sv) ? kStatOverflow : kStatNonFinite;
+                return;
+            }
+            e[k] += __double2ll_rn(sv);
+        } else {
+            if (!(fabs(v) < 1e300)) bad |= kStatNonFinite;
+            e[k] += v;
+        }
+    }
+    // wave-uniform: call from every lane of the wavefront
+    __device__ __forceinline__ void take(const DevState &S, int next, const LaneEnergy &le) {
+        to(S, next);
+        energy(0, wave_sum(le.vdw));
+        energy(1, wave_sum(le.elec));
+        energy(2, wave_sum(le.pol));
+        energy(3, wave_sum(le.vir));
+        n[kCPairs] += wave_sum(le.pairs);
+    }
+    __device__ __forceinline__ void flush(const DevState &S) {
+        if (inst >= 0 && lane_id() == 0) {
+            const int field[4] = {kEVdw, kEElec, kEPolCross, kEVirial};
+            for (int k = 0; k < 4; ++k) {
+                if (e[k] == 0) continue;
+                const size_t i = (size_t)inst * kENum + field[k];
+                if constexpr (Prec<R>::fixed)
+                    atomicAdd(reinterpret_cast<unsigned long long *>(S.inst_energy) + i, (unsigned long long)e[k]);
+                else
+                    unsafeAtomicAdd(reinterpret_cast<double *>(S.inst_energy) + i, e[k]);
+            }
+            for (int k = 0; k < kCNum; ++k)
+                if (n[k]) atomicAdd(S.inst_count + (size_t)inst * kCNum + k, n[k]);
+            if (bad) atomicOr(S.inst_status + inst, bad);
+        }
+        for (int k = 0; k < 4; ++k) e[k] = 0;
+        for (int k = 0; k < kCNum; ++k) n[k] = 0;
+        bad = 0;
+    }
+};
+
 // ------------------------------------------------------------ Born radii A4
 // Gpol_i += P4*V_j*CCF/r^4 over pairs with bond order > 2 and r <= Still
 // cutoff (forcefields/gbff.cpp:432-465), both directions per unordered pair.
@@ -714,6 +784,7 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
     const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5, still2 = (R)P.still2;
+    InstSink<R> sink;
     for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
         const int cend = min(t_end, c + tpw);
         for (int t = c; t < cend; ++t) {
@@ -762,7 +833,8 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
                     if (l < S.blk_n[J]) acc_add<R>(S.acc_g, b, (double)gj, kGpolScale, st);
                 }
                 const unsigned int npw = wave_sum((unsigned int)np);
-                if (l == 0 && npw) atomicAdd(S.inst_count + (size_t)inst * kCNum + kCBornPairs, 2ull * npw);
+                sink.to(S, inst);
+                sink.n[kCBornPairs] += 2ull * npw;
                 continue;
             }
             const int nb = cnt.x;
@@ -803,10 +875,12 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
             int *st = S.inst_status + inst;
             if (l < S.blk_n[I]) acc_add<R>(S.acc_g, a, gi_acc[w][l], kGpolScale, st);
             if (l < S.blk_n[J]) acc_add<R>(S.acc_g, b, gj_acc[w][l], kGpolScale, st);
-            if (l == 0) atomicAdd(S.inst_count + (size_t)inst * kCNum + kCBornPairs, (unsigned long long)(2 * nb));
+            sink.to(S, inst);
+            sink.n[kCBornPairs] += (unsigned long long)(2 * nb);
             wave_lds_sync();
         }
     }
+    sink.flush(S);
 }
 
 // Born radius a_i = -halfeconv/Gpol + offset (gbff.cpp:463); self term
@@ -841,25 +915,6 @@ __global__ void k_born_final(DevState S, PairParams P, int atom_begin, int atom_
 }
 
 // Energies of one lane over one tile.
-struct LaneEnergy {
-    double vdw = 0, elec = 0, pol = 0, vir = 0;
-    unsigned int pairs = 0;
-};
-
-template <typename R>
-__device__ __forceinline__ void flush_energy(const DevState &S, int inst, const LaneEnergy &e) {
-    const double v = wave_sum(e.vdw), el = wave_sum(e.elec), p = wave_sum(e.pol), vr = wave_sum(e.vir);
-    const unsigned int np = wave_sum(e.pairs);
-    if (lane_id() == 0) {
-        int *st = S.inst_status + inst;
-        if (v != 0.0) acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEVdw, v, kEnergyScale, st);
-        if (el != 0.0) acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEElec, el, kEnergyScale, st);
-        if (p != 0.0) acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEPolCross, p, kEnergyScale, st);
-        if (vr != 0.0) acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEVirial, vr, kEnergyScale, st);
-        if (np) atomicAdd(S.inst_count + (size_t)inst * kCNum + kCPairs, (unsigned long long)np);
-    }
-}
-
 // Switching function S and dS/dr of PD (forcefields/gbff.cpp:1605-1614).
 template <typename R>
 __device__ __forceinline__ void switch_fn(R r, R inner, R invSw, R &S, R &dS) {
@@ -1019,6 +1074,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
     const R cut2 = (R)P.cut2;
+    InstSink<R> sink;
     for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
         const int cend = min(t_end, c + tpw);
         for (int t = c; t < cend; ++t) {
@@ -1090,7 +1146,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                         if (MODE == 0) acc_add<R>(S.acc_deda, b, (double)daj_acc, kForceScale, st);
                     }
                 }
-                flush_energy<R>(S, inst, E);
+                sink.take(S, inst, E);
                 continue;
             }
             const int ntot = cnt.x + cnt.y;
@@ -1154,10 +1210,11 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                 acc_add<R>(S.acc_fz, b, fj[w][2][l], kForceScale, st);
                 if (MODE == 0) acc_add<R>(S.acc_deda, b, fj[w][3][l], kForceScale, st);
             }
-            flush_energy<R>(S, inst, E);
+            sink.take(S, inst, E);
             wave_lds_sync();
         }
     }
+    sink.flush(S);
 }
 
 // ---------------------------------------------------------- chain rule A6

@@ -42,6 +42,51 @@ def oracle_forces(m, xyz=None, **kw):
     return e, f
 
 
+@pytest.mark.parametrize("precision", [pdgpu.FP64, pdgpu.MIXED])
+def test_zero_width_switch_matches_oracle(precision):
+    # InnerCutoff == Cutoff (and the same for vdW): no switching region, the
+    # case PD's own defaults give when only Cutoff is set to 12
+    m = synth.peptide(seed=77, residues=18)
+    kw = dict(inner_cutoff=12.0, vdw_inner_cutoff=12.0)
+    eo, fo = oracle_forces(m, **kw)
+    with pdgpu.Context(opts(precision, **kw)) as ctx:
+        i = ctx.add_system(m)
+        e, f = ctx.calc_forces(i)
+        assert e["status"] == 0
+        assert rel(e["epot"], eo["epot"]) <= (1e-10 if precision == pdgpu.FP64 else 1e-5)
+        assert rms_rel(f, fo) <= (1e-9 if precision == pdgpu.FP64 else 1e-4)
+
+
+ALLPAIRS = dict(cutoff=1.0e4, inner_cutoff=1.0e4, vdw_cutoff=12.0, vdw_inner_cutoff=9.0)
+
+
+@pytest.mark.parametrize("precision", [pdgpu.FP64, pdgpu.MIXED])
+def test_all_pairs_gb_matches_oracle(precision):
+    # all-pairs GB: Cutoff (Coulomb + GB) and the Still cutoff beyond the
+    # molecule, LJ at 12 A; full tiles are evaluated densely, partial ones
+    # (the last block) from lists
+    m = synth.peptide(seed=88, residues=40)
+    kw = dict(ALLPAIRS)
+    p = oparams(still_cutoff=1.0e4, **kw)
+    s = osys(m)
+    rs, ent = O.nlist_build(s, m["xyz"], max(p.cutoff, p.vdw_cutoff), p.padding, True)
+    eo, fo, born_o, _ = O.gb_calc_forces(s, m["xyz"], rs, ent, p)
+    en, fn, _ = O.gb_calc_energies(s, m["xyz"], rs, ent, p)
+    n = m["n"]
+    tol_e = 1e-10 if precision == pdgpu.FP64 else 1e-5
+    tol_f = 1e-9 if precision == pdgpu.FP64 else 1e-4
+    with pdgpu.Context(opts(precision, gbsa_still_cutoff=1.0e4, **kw)) as ctx:
+        i = ctx.add_system(m)
+        e, f = ctx.calc_forces(i)
+        assert e["status"] == 0
+        assert e["pairs"] == n * (n - 1) // 2
+        assert rel(e["epot"], eo["epot"]) <= tol_e
+        assert rms_rel(f, fo) <= tol_f
+        e, f = ctx.calc_energies(i)
+        assert rel(e["epot"], en["epot"]) <= tol_e
+        assert rms_rel(f, fn) <= tol_f
+
+
 @pytest.fixture(scope="module")
 def conformers():
     rng = np.random.default_rng(5)
