@@ -133,6 +133,9 @@ struct pdgpu_ctx {
 
     // host mirrors of per-block metadata
     std::vector<int> h_blk_inst, h_blk_n, h_blk_first;
+    // pinned host staging for per-instance results (energies, counts, status)
+    void *h_stage = nullptr;
+    size_t h_stage_bytes = 0;
 
     // captured step graphs, keyed by loop kind and instance, each with the
     // launch arguments it was captured with
@@ -568,24 +571,28 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
     return PDGPU_OK;
 }
 
-int eval(pdgpu_ctx *c, int id, int mode) {
-    if (c->opt.components & PDGPU_FF_GB) TRY(ensure_listed(c, id));
-    return c->mixed ? launch_eval<float>(c, id, mode) : launch_eval<double>(c, id, mode);
-}
 
 // Converts device per-instance records into PD's units.
 int read_energies(pdgpu_ctx *c, int id, pdgpu_energies *out, int mode) {
     const Span sp = span_of(c, id);
     const int ns = sp.s1 - sp.s0;
-    std::vector<long long> e((size_t)ns * kENum);
-    std::vector<unsigned long long> cnt((size_t)ns * kCNum);
-    std::vector<int> st(ns);
-    HIPCHECK(c, hipMemcpyAsync(e.data(), (char *)c->inst_energy.p + (size_t)sp.s0 * kENum * 8, e.size() * 8,
+    // one pinned staging area: the three copies are plain DMA, no bounce
+    const size_t eb = (size_t)ns * kENum * 8, cb = (size_t)ns * kCNum * 8, sb = (size_t)ns * 4;
+    if (c->h_stage_bytes < eb + cb + sb) {
+        if (c->h_stage) (void)hipHostFree(c->h_stage);
+        c->h_stage = nullptr;
+        c->h_stage_bytes = 0;
+        HIPCHECK(c, hipHostMalloc(&c->h_stage, eb + cb + sb, hipHostMallocDefault));
+        c->h_stage_bytes = eb + cb + sb;
+    }
+    const long long *e = (const long long *)c->h_stage;
+    const unsigned long long *cnt = (const unsigned long long *)((char *)c->h_stage + eb);
+    const int *st = (const int *)((char *)c->h_stage + eb + cb);
+    HIPCHECK(c, hipMemcpyAsync((void *)e, (char *)c->inst_energy.p + (size_t)sp.s0 * kENum * 8, eb,
                                hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(cnt.data(), (char *)c->inst_count.p + (size_t)sp.s0 * kCNum * 8, cnt.size() * 8,
+    HIPCHECK(c, hipMemcpyAsync((void *)cnt, (char *)c->inst_count.p + (size_t)sp.s0 * kCNum * 8, cb,
                                hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(st.data(), (int *)c->inst_status.p + sp.s0, st.size() * 4, hipMemcpyDeviceToHost,
-                               c->stream));
+    HIPCHECK(c, hipMemcpyAsync((void *)st, (int *)c->inst_status.p + sp.s0, sb, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     int rc = PDGPU_OK;
     for (int k = 0; k < ns; ++k) {
@@ -752,6 +759,16 @@ int graph_run(pdgpu_ctx *c, const std::string &key, const std::vector<char> &sig
 
 bool use_graphs(const pdgpu_ctx *c) { return c->opt.use_graphs != 0 && !c->prof; }
 
+// One evaluation of `id` (or every instance). Lists are brought up to date
+// eagerly (a rebuild reads counts back); the kernels of the evaluation are
+// then replayed as a HIP graph when graphs are on.
+int eval(pdgpu_ctx *c, int id, int mode) {
+    if (c->opt.components & PDGPU_FF_GB) TRY(ensure_listed(c, id));
+    auto issue = [&] { return c->mixed ? launch_eval<float>(c, id, mode) : launch_eval<double>(c, id, mode); };
+    if (!use_graphs(c)) return issue();
+    return graph_run(c, "calc:" + std::to_string(id) + ":" + std::to_string(mode), graph_sig(c, id), issue);
+}
+
 } // namespace
 
 // =================================================================== C ABI
@@ -844,6 +861,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : c->prof_pool) (void)hipEventDestroy(e);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1281,8 +1299,9 @@ extern "C" int pdgpu_get_born_radii(pdgpu_ctx *c, int32_t id, double *out) {
 
 // Grid-shape autotuner: times each tile kernel of a force evaluation of
 // `id` (or all instances) at several wavefronts-per-CU settings and keeps
-// the fastest per kernel. Results do not depend on the grid (every tile
-// flushes its own sums), so tuning changes speed only.
+// the fastest per kernel. Results do not depend on the grid (every tile's
+// sums are rounded to fixed point on their own), so tuning changes speed
+// only.
 extern "C" int pdgpu_autotune(pdgpu_ctx *c, int32_t id, int32_t reps, int32_t *chosen) {
     if (!c || (id != PDGPU_ALL && !valid_id(c, id)) || reps <= 0) return PDGPU_ERR_INVALID;
     (void)hipSetDevice(c->device);

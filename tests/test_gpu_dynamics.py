@@ -282,3 +282,29 @@ def test_graph_replay_equals_eager():
     for a, b in zip(res[0], res[1]):
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+
+
+def test_evaluation_graph_replay_equals_eager():
+    """calc_forces / calc_energies replay a captured graph of the evaluation
+    kernels; each replay (positions moved eagerly in between, energies and
+    forces read back after each) gives the bits of eager launches."""
+    mols = [synth.peptide(seed=120 + k, residues=r, bonded=True) for k, r in enumerate((7, 12, 18))]
+    rng = np.random.default_rng(9)
+    moves = [[rng.normal(scale=0.05, size=(m["n"], 3)) for m in mols] for _ in range(4)]
+    res = {}
+    for g in (0, 1):
+        out = []
+        with pdgpu.Context(opts(pdgpu.MIXED, use_graphs=g, max_instances=8)) as ctx:
+            ids = [ctx.add_system(m, bonded=True) for m in mols]
+            for mv in moves:
+                for i, m, d in zip(ids, mols, mv):
+                    ctx.set_positions(i, m["xyz"] + d)
+                r = ctx.calc_forces(pdgpu.PDGPU_ALL)
+                out.append([(r[i]["epot"], ctx.forces(i).copy()) for i in ids])
+                e, f = ctx.calc_energies(ids[1])
+                out.append([(e["epot"], f.copy())])
+        res[g] = out
+    for a, b in zip(res[0], res[1]):
+        for (ea, fa), (eb, fb) in zip(a, b):
+            assert ea == eb
+            assert np.array_equal(fa, fb)
