@@ -121,7 +121,7 @@ struct pdgpu_ctx {
     DevBuf px, py, pz, q, lj64, gb64, t123, xyzq32, lj32, gb32, born, acc_g, acc_deda, acc_fx, acc_fy, acc_fz;
     DevBuf fx, fy, fz, deda_out, vx, vy, vz, minv, opx, opy, opz, rx, ry, rz;
     // per block
-    DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, row_count, row_off, tiles;
+    DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, row_count, row_off, tiles, tile_slot;
     // per instance
     DevBuf any_need, inst_need, inst_moved, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf band, sparse;
@@ -166,6 +166,7 @@ struct pdgpu_ctx {
         s.blk_inst = (const int *)blk_inst.p; s.blk_n = (const int *)blk_n.p; s.blk_first = (const int *)blk_first.p;
         s.blk_ctr = (double4 *)blk_ctr.p; s.blk_half = (float4 *)blk_half.p; s.blk_box = (double4 *)blk_box.p;
         s.row_count = (int *)row_count.p; s.row_off = (int *)row_off.p; s.tiles = (int2 *)tiles.p;
+        s.tile_slot = (int *)tile_slot.p;
         s.inst_mid = (const int *)inst_mid.p; s.inst_band_off = (const long long *)inst_band_off.p;
         s.inst_sp_off = (const int *)inst_sp_off.p; s.inst_sp_n = (const int *)inst_sp_n.p;
         s.band = (const int8_t *)band.p; s.sparse = (const int4 *)sparse.p;
@@ -825,6 +826,7 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     if (rc == PDGPU_OK) rc = resize(c, c->band, 4096, 1);
     if (rc == PDGPU_OK) rc = resize(c, c->sparse, 64, 16);
     if (rc == PDGPU_OK) rc = resize(c, c->tiles, 4096, 8);
+    if (rc == PDGPU_OK) rc = resize(c, c->tile_slot, 4096, 4);
     if (rc == PDGPU_OK) rc = resize(c, c->overflow, 1, 4);
     if (rc != PDGPU_OK) {
         pdgpu_ctx_destroy(c);
@@ -851,7 +853,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->born, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy, &c->acc_fz, &c->fx, &c->fy, &c->fz,
                      &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
                      &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->row_count, &c->row_off,
-                     &c->tiles, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
+                     &c->tiles, &c->tile_slot, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
                      &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
                      &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot, &c->rx, &c->ry, &c->rz,
@@ -935,6 +937,7 @@ extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t
     if (tri > c->cap_tiles) {
         const int64_t nc = std::max<int64_t>(tri, c->cap_tiles * 3 / 2);
         int rc = resize(c, c->tiles, nc, 8);
+        if (rc == PDGPU_OK) rc = resize(c, c->tile_slot, nc, 4);
         if (rc != PDGPU_OK) {
             c->blocks.release(b0, nb); c->bandr.release(boff, blen); c->spr.release(soff, t->nsparse);
             return rc;

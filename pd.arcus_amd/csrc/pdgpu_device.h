@@ -96,6 +96,7 @@ struct DevState {
     int *row_count;               // tiles per I row
     int *row_off;                 // exclusive scan of row_count (+ total at [nblocks])
     int2 *tiles;                  // (I, J | flags<<24)
+    int *tile_slot;               // candidate slot of each tile
     // per instance
     const int *inst_mid;
     const long long *inst_band_off;
@@ -338,6 +339,9 @@ __global__ void k_tile_rows(DevState S, PairParams P, int blk_begin, int blk_end
     const int first = S.blk_first[I];
     const int base = (PASS == 1) ? S.row_off[I] : 0;
     const int reserved = (PASS == 1) ? S.row_count[I] : 0; // never write past the counted row
+    // candidate slot of this row's first tile (tile index relative to the
+    // instance's first row, offset by the instance's region)
+    const long long slot0 = (PASS == 1) ? S.inst_cslot[inst] + (base - S.row_off[first]) : 0;
     int total = 0;
     for (int J0 = first; J0 <= I; J0 += 64) {
         const int J = J0 + l;
@@ -345,7 +349,10 @@ __global__ void k_tile_rows(DevState S, PairParams P, int blk_begin, int blk_end
         const unsigned long long m = __ballot(hit);
         if (PASS == 1 && hit) {
             const int local = total + (int)__popcll(m & ((1ull << l) - 1ull));
-            if (local < reserved) S.tiles[base + local] = make_int2(I, J | (tile_flags(S, inst, I, J) << 24));
+            if (local < reserved) {
+                S.tiles[base + local] = make_int2(I, J | (tile_flags(S, inst, I, J) << 24));
+                S.tile_slot[base + local] = (int)(slot0 + local);
+            }
         }
         total += (int)__popcll(m);
     }
@@ -602,9 +609,17 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
     for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
         const int cend = min(t_end, c + tpw);
         for (int t = c; t < cend; ++t) {
+            // the tile record and its candidate slot come first; the slot's
+            // count and first candidate chunk (every slot holds kTileCap
+            // entries, and tile_slot only ever names allocated slots) are
+            // requested together with the block's records
             const int2 tw = S.tiles[t];
+            const long long slot = S.tile_slot[t];
+            const uint16_t *C = S.cand + slot * kTileCap;
+            int e_nx = (int)C[l]; // candidates are fetched one chunk ahead
+            const int n = S.cand_n[slot];
             const int I = tw.x, J = tw.y & 0xFFFFFF;
-            const int inst = S.blk_inst[I], first = S.blk_first[I];
+            const int inst = S.blk_inst[I];
             if (inst < 0 || t >= S.plist_cap) {
                 if (l == 0 && t < S.plist_cap) S.tile_cnt[t] = make_int2(0, 0);
                 if (l == 0 && t >= S.plist_cap) atomicOr(S.overflow, 1);
@@ -614,15 +629,12 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
                 if (l == 0) S.tile_cnt[t] = make_int2(0, 0);
                 continue;
             }
-            const long long slot = cand_slot(S, inst, first, t);
-            const int n = S.cand_n[slot];
             if (n < 0) { // dense off-diagonal tile
                 if (l == 0) S.tile_cnt[t] = make_int2(-1, -1);
                 continue;
             }
-            const uint16_t *C = S.cand + slot * kTileCap;
+            if (l >= n) e_nx = 0;
             uint16_t *L = S.plist + (size_t)t * kTileCap;
-            int e_nx = l < n ? (int)C[l] : 0; // candidates are fetched one chunk ahead
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -745,8 +757,7 @@ struct DiagBits {
     unsigned long long mc, mx, m4;
 };
 
-__device__ __forceinline__ DiagBits diag_bits(const DevState &S, int inst, int first, int t) {
-    const long long slot = cand_slot(S, inst, first, t);
+__device__ __forceinline__ DiagBits diag_bits(const DevState &S, long long slot) {
     const unsigned long long *M = reinterpret_cast<const unsigned long long *>(S.cand + slot * kTileCap);
     const int l = lane_id();
     return {M[l], M[kBlk + l], M[2 * kBlk + l]};
@@ -794,8 +805,9 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
             if (inst < 0) continue;
             const bool diag = (tw.y >> 24) & kTileDiag;
             const int2 cnt = S.tile_cnt[t];
+            const int tslot = S.tile_slot[t];
             if (diag || cnt.x < 0) {
-                const DiagBits D = diag_bits(S, inst, S.blk_first[I], t);
+                const DiagBits D = diag_bits(S, tslot);
                 const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
                 const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
                 const AtomXYZ<R> pj = diag ? pi : load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -1084,8 +1096,9 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
             if (inst < 0) continue;
             const bool diag = (tw.y >> 24) & kTileDiag;
             const int2 cnt = S.tile_cnt[t];
+            const int tslot = S.tile_slot[t];
             if (diag || cnt.x < 0) {
-                const DiagBits D = diag_bits(S, inst, S.blk_first[I], t);
+                const DiagBits D = diag_bits(S, tslot);
                 const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
                 const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
                 const AtomXYZ<R> pj = diag ? pi : load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -1263,8 +1276,9 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
             if (inst < 0) continue;
             const bool diag = (tw.y >> 24) & kTileDiag;
             const int2 cnt = S.tile_cnt[t];
+            const int tslot = S.tile_slot[t];
             if (diag || cnt.x < 0) {
-                const DiagBits D = diag_bits(S, inst, S.blk_first[I], t);
+                const DiagBits D = diag_bits(S, tslot);
                 const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
                 const bool vi = l < S.blk_n[I], vjb = l < S.blk_n[J];
                 const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
