@@ -83,7 +83,8 @@ typedef struct pdgpu_options {
     int32_t use_graphs;           /* 1: step loops replay captured HIP graphs      */
     int32_t dense_tile_pairs;     /* off-diagonal tiles with at least this many    */
                                   /* candidate pairs are evaluated densely (lane   */
-                                  /* rotation, no lists); 0: 3584 (7/8 full)       */
+                                  /* rotation, no lists); 0: 3584 x (list radius / */
+                                  /* cutoff)^3, i.e. ~7/8 of the slots in cutoff   */
 } pdgpu_options;
 
 /* Forcefield components of a context: FF_GeneralizedBorn_Still and the four

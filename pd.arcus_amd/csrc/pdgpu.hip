@@ -339,9 +339,16 @@ void setup_params(pdgpu_ctx *c) {
     P.fs_cutV = P.fs_sA * (P.fs_sB * (1.0 / rc) + P.fs_sC * rc + P.fs_sD * rc * rc * rc +
                            P.fs_sE * rc * rc * rc * rc2);
     P.fs_eshift = (ri < rc) ? 1 / ri + (fin - P.fs_cutV) : 0;
-    // default: tiles at least 7/8 full (all-pairs GB, compact clusters);
-    // at lower fill the filtered list with LDS accumulation is faster
-    P.dense_pairs = o.dense_tile_pairs > 0 ? o.dense_tile_pairs : kTileCap / 8 * 7;
+    // default: tiles whose candidates are expected to be at least 7/8 of
+    // the 4,096 slots inside the cutoff, i.e. candidates >= 3584 times the
+    // list-to-cutoff volume ratio (all-pairs GB, where the ratio is ~1); at
+    // lower fill the filtered list with LDS accumulation is faster
+    if (o.dense_tile_pairs > 0) {
+        P.dense_pairs = o.dense_tile_pairs;
+    } else {
+        const double vr = std::pow(P.list_r / std::max(o.cutoff, 1e-6), 3.0);
+        P.dense_pairs = (int)std::min<double>(kTileCap + 1, std::ceil(kTileCap / 8 * 7 * vr));
+    }
     P.elec_mode = o.elec_mode;
     P.vdw_mode = o.vdw_mode;
 }
@@ -399,6 +406,11 @@ struct ProfScope {
     }
 };
 
+// Grid of a tile kernel for an estimated tile count. Unless the options fix
+// tiles per wavefront, the kernel is passed tpw = 0 and divides the tile
+// count it reads on the device over the waves it has, so a stale or bounding
+// estimate only changes the grid size, never how many tiles one wavefront
+// walks in sequence (the critical path of small, latency-bound launches).
 void pair_grid(const pdgpu_ctx *c, int kind, int64_t tiles, int *groups, int *tpw) {
     const int wcap = waves_cap(c, kind);
     int t = c->opt.tiles_per_wave;
@@ -408,7 +420,7 @@ void pair_grid(const pdgpu_ctx *c, int kind, int64_t tiles, int *groups, int *tp
     int g = (int)((waves + kWavesPerGroup - 1) / kWavesPerGroup);
     if (g > 8) g = (g + 7) / 8 * 8;
     *groups = g;
-    *tpw = t;
+    *tpw = c->opt.tiles_per_wave > 0 ? t : 0;
 }
 
 // Grows the per-tile pair-list buffer (kTileCap 2-byte entries per tile);
@@ -429,11 +441,14 @@ int ensure_plist(pdgpu_ctx *c, int64_t tiles) {
 // Tiles an evaluation of id walks, for sizing launch grids only (kernels
 // read the true range from row_off): the count read at the last counted
 // rebuild, else the upper bound.
+// Tile count used to shape launch grids: the last counted total for all
+// instances (or the triangle bound before any count), the instance's
+// triangle bound for one instance. Kernels loop until the device-side count
+// is covered, so an estimate only affects balance.
 int64_t span_tiles(const pdgpu_ctx *c, int id) {
-    const int64_t all = c->ntiles > 0 ? c->ntiles : c->tri_sum;
-    if (id == PDGPU_ALL) return all;
+    if (id == PDGPU_ALL) return c->ntiles > 0 ? c->ntiles : c->tri_sum;
     const int nb = c->inst[id].nblocks;
-    return std::min<int64_t>(all, (int64_t)nb * (nb + 1) / 2);
+    return (int64_t)nb * (nb + 1) / 2;
 }
 
 // Rebuilds the tile rows and candidate pairs of instance id (or all). With
@@ -458,14 +473,19 @@ int launch_build(pdgpu_ctx *c, int id, bool flagged = false, bool count = true, 
     hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(1024), 0, c->stream, S, all);
     hipLaunchKernelGGL(k_tile_rows<1>, dim3((all + 3) / 4), dim3(256), 0, c->stream, S, c->P, 0, all);
     {
+        // The new tile count is only known on the device here: size the grid
+        // for the upper bound (every instance's full triangle, capped by the
+        // grid limit) and let the kernel split the real count (tpw = 0).
         int g, t;
-        pair_grid(c, kGridEnum, id == PDGPU_ALL || flagged ? c->ntiles : span_tiles(c, id), &g, &t);
+        const int64_t bound = (id == PDGPU_ALL || flagged) ? std::max<int64_t>(c->ntiles, c->tri_sum)
+                                                           : span_tiles(c, id);
+        pair_grid(c, kGridEnum, bound, &g, &t);
         // the mixed path enumerates from FP32 positions relative to the block
         // centres k_block_bounds has just set
         const int a0 = sp.b0 * kBlk, na = nb * kBlk;
         if (c->mixed && na > 0)
             hipLaunchKernelGGL(k_xyzq32, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a0 + na);
-        hipLaunchKernelGGL(k_cand<R>, dim3(g), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, t);
+        hipLaunchKernelGGL(k_cand<R>, dim3(g), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, 0);
     }
     HIPCHECK(c, hipGetLastError());
     if (count) {
