@@ -462,15 +462,17 @@ def main():
     sections = args.sections.split(",")
     import pdgpu
     # every input is built before the first GPU context: the builders fork
-    batch_mols = build_batch(args)
+    batch_mols = build_batch(args) if "batch" in sections else None
     md_mol = build_md() if "md" in sections else None
     serve_mols = build_serving(args) if "serving" in sections else None
     ap_mol = build_allpairs(args) if "allpairs" in sections else None
     args.cpu_baseline = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and batch_mols is not None:
         log("cpu baseline (oracle port)")
         args.cpu_baseline = cpu_baseline_batch(batch_mols)
-    out = run_batch(args, pdgpu, batch_mols)
+    # the headline line needs the batch section; without it (profiling a
+    # single section) only that section's record is printed
+    out = run_batch(args, pdgpu, batch_mols) if batch_mols is not None else {}
     if md_mol is not None:
         out["md"] = run_md(args, pdgpu, md_mol)
     if serve_mols is not None:
