@@ -252,6 +252,17 @@ int pdgpu_get_velocities(pdgpu_ctx *ctx, int32_t id, double *vel);
 int pdgpu_md_vv(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
                 pdgpu_energies *energies, double *ekin);
 
+/* MolecularDynamics::runcore with Integrator = VelocityVerlet, as PD writes
+ * it (protocols/md.cpp:209-233 step loop, :788-838 update; list refresh
+ * protocols/protocolbase.cpp:113-122): each step refreshes the list when
+ * step % update_nlist == 0, evaluates forces and applies v += f*dt/2m
+ * (ekin taken), v += f*dt/2m, p += dt*(v + f*dt/2m), v += f*dt/2m. No
+ * thermostat or barostat (PD's defaults). energies are those of the last
+ * force evaluation and ekin the last step's, as in PD's workspace after
+ * run(). */
+int pdgpu_md_vv_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
+                   pdgpu_energies *energies, double *ekin);
+
 /* Grid-shape autotuner (no reference counterpart): times the tile kernels
  * of a force evaluation of id (or PDGPU_ALL) reps times at each of several
  * wavefronts-per-CU settings and keeps the fastest per kernel for later

@@ -27,6 +27,8 @@
  *   orc_md_vv            energy conservation of a two-atom system (textbook VV is
  *                        a deliberate choice; PD's own VV update differs, see
  *                        protocols/md.cpp:788-838)
+ *   orc_md_vv_pd         PD's own MolecularDynamics runs with the VelocityVerlet
+ *                        integrator (compiled reference, tests/golden md_*)
  */
 #include "pd_oracle.h"
 #include <math.h>
@@ -849,6 +851,57 @@ int orc_md_vv(const orc_system *s, double *xyz, double *vel, const orc_params *p
             for (int c = 0; c < 3; ++c) {
                 vel[3 * i + c] += f[3 * i + c] * h;
                 ekin += 0.5 * s->mass[i] * sqr(vel[3 * i + c]);
+            }
+        }
+        if (trace_epot) trace_epot[step] = e.epot;
+        if (trace_ekin) trace_ekin[step] = ekin;
+        if (!isfinite(e.epot)) { rc = -1; break; }
+    }
+    free(nl.row_start);
+    free(nl.entries);
+    free(f);
+    return rc;
+}
+
+/* MolecularDynamics::run_core with the VelocityVerlet integrator, as written
+ * (protocols/md.cpp:209-233 loop, 788-838 update, protocolbase.cpp:113-122
+ * list refresh): each step rebuilds the list when step % update_nlist == 0,
+ * evaluates forces, adds f*dt/2m to v (the kinetic energy is taken here),
+ * then adds it again, moves p by dt*(v + f*dt/2m) and adds it a third time.
+ * No thermostat or barostat (PD's defaults); momentum removal only acts with
+ * a thermostat (md.cpp:246-258). */
+int orc_md_vv_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p,
+                 int steps, double dt, int update_nlist, double *trace_epot, double *trace_ekin) {
+    const int n = s->n;
+    orc_nlist nl;
+    nl.row_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    nl.cap = (int64_t)n * 64 + 1024;
+    nl.entries = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nl.cap);
+    double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    orc_energies e;
+    int rc = steps;
+    for (int step = 0; step < steps; ++step) {
+        if (update_nlist > 0 && step % update_nlist == 0) build_list(s, xyz, p, &nl);
+        eval_forces(s, xyz, &nl, p, f, &e);
+        double ekin = 0;
+        for (int i = 0; i < n; ++i) {
+            const double tinvmass = 0.5 * dt / s->mass[i];
+            for (int c = 0; c < 3; ++c) vel[3 * i + c] += f[3 * i + c] * tinvmass;
+        }
+        for (int i = 0; i < n; ++i) /* calcKineticEnergy, md.cpp:504-518: 0.5 m |v|^2 via mag() */
+            ekin += 0.5 * s->mass[i] * sqr(sqrt(sqr(vel[3 * i]) + sqr(vel[3 * i + 1]) + sqr(vel[3 * i + 2])));
+        for (int i = 0; i < n; ++i) {
+            const double tinvmass = 0.5 * dt / s->mass[i];
+            double dv[3], dr[3];
+            for (int c = 0; c < 3; ++c) {
+                dv[c] = f[3 * i + c] * tinvmass;
+                vel[3 * i + c] += dv[c];
+            }
+            for (int c = 0; c < 3; ++c) dr[c] = dt * (vel[3 * i + c] + dv[c]);
+            for (int c = 0; c < 3; ++c) {
+                dr[c] *= 1 / kAngstrom;
+                xyz[3 * i + c] += dr[c];
+                vel[3 * i + c] += dv[c];
             }
         }
         if (trace_epot) trace_epot[step] = e.epot;

@@ -20,6 +20,7 @@
 #include "forcefields/nonbonded.h"
 #include "forcefields/gbff.h"
 #include "protocols/minimise.h"
+#include "protocols/md.h"
 
 #include <stdint.h>
 #include <cstdio>
@@ -158,6 +159,7 @@ struct Case {
     double cutoff, inner, vdw_cutoff, vdw_inner, padding, eps_out;
     bool set_vdw;   // false: leave VdwCutoff/VdwInnerCutoff at PD's defaults (quirk Q6)
     int sd_steps;   // 0: no minimisation record
+    int md_steps;   // 0: no molecular-dynamics record
 };
 
 void dump_inputs(Rec &r, WorkSpace &ws, GBProbe &gb) {
@@ -399,6 +401,73 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
         r.s("sd_done", (double)done);
         r.s("sd_stepsize", mini.StepSize);
     }
+    if (c.md_steps > 0) {
+        // PD's MolecularDynamics with the VelocityVerlet integrator on
+        // FF_Bonded + GB, after 100 steepest-descent steps (the built chain
+        // can hold clashes), from velocities drawn from a SplitMix64 stream
+        // (uniform in +-sqrt(3 kB T / m) per component, T = 300 K)
+        System sys(ffps);
+        sys.add(NewProtein(ffps, pd_grammar(c.seq)));
+        WorkSpace ws(sys);
+        ws.nlist().setPadding(c.padding);
+        Forcefield ff(ws);
+        FF_Bonded bonds(ws);
+        GBProbe gb(ws);
+        configure(gb, c);
+        ff.add(bonds);
+        ff.add(gb);
+        ff.printEnergySummary();
+        {
+            Protocol::Minimisation mini(ff);
+            mini.Algorithm = Protocol::Minimisation::SteepestDescent;
+            mini.Steps = 100;
+            mini.UpdateScr = 0;
+            mini.OutputLevel = Verbosity::Silent;
+            mini.run();
+        }
+        const int n = (int)ws.atom.size();
+        std::vector<double> v0, x0;
+        uint64_t z = 0x5EEDULL;
+        for (int i = 0; i < n; ++i) {
+            const double vmax = sqrt(3.0 * PhysicsConst::kB * 300.0 / ws.atom[i].mass);
+            double u[3];
+            for (int k = 0; k < 3; ++k) {
+                z += 0x9E3779B97F4A7C15ULL;
+                uint64_t w = z;
+                w = (w ^ (w >> 30)) * 0xBF58476D1CE4E5B9ULL;
+                w = (w ^ (w >> 27)) * 0x94D049BB133111EBULL;
+                w ^= w >> 31;
+                u[k] = (2.0 * ((double)(w >> 11) * (1.0 / 9007199254740992.0)) - 1.0) * vmax;
+            }
+            ws.cur.atom[i].v.setTo(u[0], u[1], u[2]);
+            v0.push_back(u[0]); v0.push_back(u[1]); v0.push_back(u[2]);
+            x0.push_back(ws.cur.atom[i].p.x); x0.push_back(ws.cur.atom[i].p.y); x0.push_back(ws.cur.atom[i].p.z);
+        }
+        Protocol::MolecularDynamics md(ff);
+        md.Integrator = Protocol::MolecularDynamics::VelocityVerlet;
+        md.Steps = c.md_steps;
+        md.Timestep = 1e-15;
+        md.RandVel = false;
+        md.UpdateNList = 10;
+        md.UpdateScr = 0;
+        md.UpdateTra = 0;
+        md.OutputLevel = Verbosity::Silent;
+        md.run();
+        std::vector<double> xyz, vel;
+        for (int i = 0; i < n; ++i) {
+            xyz.push_back(ws.cur.atom[i].p.x); xyz.push_back(ws.cur.atom[i].p.y); xyz.push_back(ws.cur.atom[i].p.z);
+            vel.push_back(ws.cur.atom[i].v.x); vel.push_back(ws.cur.atom[i].v.y); vel.push_back(ws.cur.atom[i].v.z);
+        }
+        r.d("md_xyz0", x0);
+        r.d("md_v0", v0);
+        r.d("md_xyz", xyz);
+        r.d("md_v", vel);
+        r.s("md_epot", ws.ene.epot);
+        r.s("md_ekin", ws.ene.ekin);
+        r.s("md_steps", (double)c.md_steps);
+        r.s("md_dt", md.Timestep);
+        r.s("md_update_nlist", (double)md.UpdateNList);
+    }
 }
 
 } // namespace
@@ -408,12 +477,12 @@ int main(int argc, char **argv) {
     FFParamSet ffps;
     ffps.readLib(argv[1]);
     std::vector<Case> cases;
-    // name, seq, cutoff, inner, vdwcut, vdwinner, padding, eps_out, set_vdw, sd_steps
-    Case c1 = {"tri_ala", "AAA", 12, 9, 12, 9, 2.0, 78.5, true, 20}; cases.push_back(c1);
+    // name, seq, cutoff, inner, vdwcut, vdwinner, padding, eps_out, set_vdw, sd_steps, md_steps
+    Case c1 = {"tri_ala", "AAA", 12, 9, 12, 9, 2.0, 78.5, true, 20, 40}; cases.push_back(c1);
     Case c2 = {"gly_ala", "GA", 12, 9, 12, 9, 2.0, 78.5, true, 0}; cases.push_back(c2);
     for (int seed = 1; seed <= 3; ++seed) {
         Case c = {"c1_seed" + std::string(1, char('0' + seed)), random_sequence(seed, 20),
-                  12, 9, 12, 9, 2.0, 78.5, true, seed == 1 ? 200 : 30};
+                  12, 9, 12, 9, 2.0, 78.5, true, seed == 1 ? 200 : 30, seed == 2 ? 60 : 0};
         cases.push_back(c);
     }
     Case c60 = {"res60_seed4", random_sequence(4, 60), 12, 9, 12, 9, 2.0, 78.5, true, 0}; cases.push_back(c60);

@@ -191,6 +191,40 @@ __global__ void k_vv(DevState S, InstMD *md, const double *minv, double *vx, dou
     }
 }
 
+// PD's VelocityVerlet update as written (protocols/md.cpp:788-838), with
+// dv = f*dt/2m from the forces of this step: v += dv (the kinetic energy is
+// taken here, md.cpp:815), v += dv, p += dt*(v + dv) / Angstrom, v += dv.
+__global__ void k_vv_pd(DevState S, InstMD *md, const double *minv, double *vx, double *vy, double *vz,
+                        double *px, double *py, double *pz, int atom_begin, int atom_end, double dt,
+                        double f_to_n, int want_ekin) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = a / kBlk;
+    const bool in = a < atom_end;
+    const int inst = in ? S.blk_inst[b] : -1;
+    const bool v = in && inst >= 0 && (a % kBlk) < S.blk_n[b];
+    double ek = 0;
+    if (v) {
+        const double tinvmass = 0.5 * dt * minv[a];
+        const double dx = S.fx[a] * f_to_n * tinvmass, dy = S.fy[a] * f_to_n * tinvmass,
+                     dz = S.fz[a] * f_to_n * tinvmass;
+        double ux = vx[a] + dx, uy = vy[a] + dy, uz = vz[a] + dz;
+        if (minv[a] > 0) {
+            const double m = sqrt(ux * ux + uy * uy + uz * uz);
+            ek = 0.5 / minv[a] * (m * m);
+        }
+        ux += dx; uy += dy; uz += dz;
+        const double s = 1.0 / 1e-10;
+        px[a] += (dt * (ux + dx)) * s;
+        py[a] += (dt * (uy + dy)) * s;
+        pz[a] += (dt * (uz + dz)) * s;
+        vx[a] = ux + dx; vy[a] = uy + dy; vz[a] = uz + dz;
+    }
+    if (want_ekin) {
+        ek = wave_sum(ek);
+        if (lane_id() == 0 && in && inst >= 0 && ek != 0.0) unsafeAtomicAdd(&md[inst].ekin, ek);
+    }
+}
+
 __global__ void k_ekin_clear(InstMD *md, int s0, int s1) {
     const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (s < s1) md[s].ekin = 0;

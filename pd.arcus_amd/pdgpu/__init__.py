@@ -46,7 +46,7 @@ EXPORTS = (
     "pdgpu_minimise_sd", "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_md_vv",
     "pdgpu_bondorder_build", "pdgpu_fixed_terms", "pdgpu_instance_bytes", "pdgpu_bytes_free",
     "pdgpu_profile", "pdgpu_profile_read", "pdgpu_sd_start", "pdgpu_sd_advance", "pdgpu_sd_finish",
-    "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune",
+    "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune", "pdgpu_md_vv_pd",
 )
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
@@ -129,6 +129,7 @@ def _load():
         "pdgpu_get_forces": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
         "pdgpu_minimise_sd": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double, _ip]),
         "pdgpu_md_vv": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, P(Energies), _dp]),
+        "pdgpu_md_vv_pd": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, P(Energies), _dp]),
         "pdgpu_bondorder_build": (C.c_int, [C.c_int32, _ip, C.c_int32, _ip, _ip, C.c_int32, _ip, _ip, C.c_int32,
                                             _ip, _ip, _ip, _bp, _ip, _ip, _ip, _ip]),
         "pdgpu_fixed_terms": (C.c_int, [C.c_int32, _dp, C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _ip, _dp,
@@ -415,6 +416,20 @@ class Context:
         e = Energies()
         ek = np.zeros(1)
         self._check(lib.pdgpu_md_vv(self.h, iid, steps, dt, update_nlist, C.byref(e), _p(ek, _dp)), "md_vv")
+        return e.as_dict(), float(ek[0])
+
+    def md_vv_pd(self, iid, steps, dt, update_nlist=10):
+        """PD's MolecularDynamics with its VelocityVerlet update as written
+        (protocols/md.cpp:788-838). Returns like md_vv."""
+        if iid == PDGPU_ALL:
+            n = self.opt.max_instances if self.opt.max_instances > 0 else 8192
+            arr = (Energies * n)()
+            ek = np.zeros(n)
+            self._check(lib.pdgpu_md_vv_pd(self.h, iid, steps, dt, update_nlist, arr, _p(ek, _dp)), "md_vv_pd")
+            return {i: arr[i].as_dict() for i in self._natoms}, {i: float(ek[i]) for i in self._natoms}
+        e = Energies()
+        ek = np.zeros(1)
+        self._check(lib.pdgpu_md_vv_pd(self.h, iid, steps, dt, update_nlist, C.byref(e), _p(ek, _dp)), "md_vv_pd")
         return e.as_dict(), float(ek[0])
 
     def profile(self, enable=True):

@@ -85,6 +85,9 @@ lib.orc_bonded_forces.argtypes = [C.POINTER(Bonded), _dp, _dp, C.POINTER(Energie
 lib.orc_minimise_sd.restype = C.c_int
 lib.orc_minimise_sd.argtypes = [C.POINTER(System), _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                                 _dp, C.POINTER(Energies), _dp]
+lib.orc_md_vv_pd.restype = C.c_int
+lib.orc_md_vv_pd.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
+                             _dp, _dp]
 lib.orc_md_vv.restype = C.c_int
 lib.orc_md_vv.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                           _dp, _dp]
@@ -271,4 +274,15 @@ def md_vv(sysm, xyz, vel, params, steps, dt, update_nlist=10):
     tk = np.zeros(steps)
     rc = lib.orc_md_vv(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), C.byref(params), steps, dt, update_nlist,
                        _ptr(te, _dp), _ptr(tk, _dp))
+    return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk
+
+
+def md_vv_pd(sysm, xyz, vel, params, steps, dt, update_nlist=10):
+    """PD's MolecularDynamics with its VelocityVerlet update as written."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float64).copy()
+    te = np.zeros(steps)
+    tk = np.zeros(steps)
+    rc = lib.orc_md_vv_pd(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), C.byref(params), steps, dt,
+                          update_nlist, _ptr(te, _dp), _ptr(tk, _dp))
     return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk

@@ -181,3 +181,24 @@ def test_full_forcefield_steepest_descent(name):
     assert rc == steps == int(d["sdf_done"])
     assert rel(ef["epot"], d["sdf_epot"]) <= 1e-10
     assert np.abs(x - d["sdf_xyz"]).max() <= 1e-9
+
+
+MD_CASES = [n for n in CASES if "md_xyz" in G.load(n)]
+
+
+@pytest.mark.parametrize("name", MD_CASES)
+def test_pd_velocity_verlet_matches_reference(name):
+    """PD's MolecularDynamics with its VelocityVerlet update as written
+    (protocols/md.cpp:788-838), FF_Bonded + GB, against the compiled
+    reference's run from the same positions and velocities."""
+    d = G.load(name)
+    s = _system(d, O.OBonded(d))
+    p = O.default_params(**d["param_dict"])
+    steps = int(d["md_steps"])
+    rc, x, v, te, tk = O.md_vv_pd(s, d["md_xyz0"], d["md_v0"], p, steps, float(d["md_dt"]),
+                                   int(d["md_update_nlist"]))
+    assert rc == steps
+    assert np.abs(x - d["md_xyz"].reshape(-1, 3)).max() <= 1e-9
+    assert np.abs(v - d["md_v"].reshape(-1, 3)).max() <= 1e-9 * np.abs(d["md_v"]).max()
+    assert rel(te[-1], d["md_epot"]) <= 1e-10
+    assert rel(tk[-1], d["md_ekin"]) <= 1e-10
