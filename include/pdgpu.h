@@ -85,6 +85,11 @@ typedef struct pdgpu_options {
                                   /* candidate pairs are evaluated densely (lane   */
                                   /* rotation, no lists); 0: 3584 x (list radius / */
                                   /* cutoff)^3, i.e. ~7/8 of the slots in cutoff   */
+    int32_t reserved;
+    /* FF_SoftVDW (forcefields/ffsoftvdw.h:39-47), used with PDGPU_FF_SOFTVDW  */
+    double softvdw_hardness;      /* Hardness, kcal/mol (default 12)               */
+    double softvdw_olap_offset;   /* OlapOffset, A (default 0.6)                   */
+    double softvdw_cutoff;        /* VdwCutoff of FF_SoftVDW, A (default 15)       */
 } pdgpu_options;
 
 /* Forcefield components of a context: FF_GeneralizedBorn_Still and the four
@@ -98,6 +103,14 @@ typedef struct pdgpu_options {
 #define PDGPU_FF_IMPROPERS 16
 #define PDGPU_FF_BONDED (PDGPU_FF_BONDS | PDGPU_FF_ANGLES | PDGPU_FF_TORSIONS | PDGPU_FF_IMPROPERS)
 #define PDGPU_FF_ALL (PDGPU_FF_GB | PDGPU_FF_BONDED)
+/* FF_SoftVDW's soft-core repulsion (calcForces_Soft, forcefields/
+ * ffsoftvdw.cpp:71-145), PD's clash-relief term; not part of PDGPU_FF_ALL.
+ * Like PD's, its energy is reported as epot_vdw. It cannot be combined with
+ * PDGPU_FF_GB in one context (PD's FF_SoftVDW resets ene.epot_vdw, so the
+ * pair depends on insertion order): ctx_create returns
+ * PDGPU_ERR_UNSUPPORTED. The soft pairs (r < R_i + R_j - OlapOffset) must
+ * lie within Cutoff, which holds for any Cutoff above ~5 A. */
+#define PDGPU_FF_SOFTVDW 32
 
 /* One molecule. Per-atom arrays have natoms entries. The bond-order band and
  * the sparse list are PD's BondOrder (workspace/bondorder.h:43-112): band has

@@ -445,6 +445,50 @@ void orc_gb_calc_forces(const orc_system *s, const double *xyz, const orc_nlist 
 }
 
 /* ---------------- A8: forcefields/nonbonded.cpp:542-893 ---------------- */
+/* FF_SoftVDW::calcForces_Soft, forcefields/ffsoftvdw.cpp:71-145: over the
+ * list pairs j < i of bond order above 1-4 with r^2 < (R_i+R_j-olap)^2 and
+ * r^2 < vdw_cutoff^2, E = H (1 - A^2 r^2)^2 with A = 1/(R_i+R_j-olap) and H
+ * the hardness in J/molecule; the force is 4 H/A (A^2 r^2 A^2 r - A^2 r)
+ * along i-j, subtracted from atom i and added to atom j. */
+void orc_softvdw_forces(const orc_system *s, const double *xyz, const orc_nlist *nl, double hardness,
+                        double olap, double vdw_cutoff, double *force, orc_energies *e) {
+    const double localHardness = hardness * (kKcal2J / kNa);
+    const double fourInvAnglocalHardness = 4.0 * (1.0 / kAngstrom) * localHardness;
+    const double sqrVdwCutoff = vdw_cutoff * vdw_cutoff;
+    double esum = 0;
+    for (int i = 0; i < s->n; ++i) {
+        const double ri = s->radius[i];
+        for (int k = nl->row_start[i]; k < nl->row_start[i + 1]; ++k) {
+            const uint32_t ent = nl->entries[k];
+            const int j = nl_j(ent);
+            if (j > i) break;
+            if (nl_bo(ent) <= 3) continue;
+            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
+                         dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+            const double sqrdist = dx * dx + dy * dy + dz * dz;
+            double vsum = ri + s->radius[j];
+            vsum -= olap;
+            const double sqrvsum = vsum * vsum;
+            if (sqrdist < sqrvsum && sqrdist < sqrVdwCutoff) {
+                const double dist = sqrt(sqrdist);
+                const double A = 1.0 / vsum;
+                const double A2X = A * A * dist;
+                const double A2X2 = A2X * dist;
+                double pot = 1.0 - A2X2;
+                pot *= pot;
+                pot *= localHardness;
+                const double fmag = fourInvAnglocalHardness * ((A2X2 * A2X) - A2X);
+                esum += pot;
+                const double sc = fmag / dist;
+                force[3 * i] -= dx * sc; force[3 * i + 1] -= dy * sc; force[3 * i + 2] -= dz * sc;
+                force[3 * j] += dx * sc; force[3 * j + 1] += dy * sc; force[3 * j + 2] += dz * sc;
+            }
+        }
+    }
+    e->epot += esum;
+    e->epot_vdw = esum;
+}
+
 void orc_nb_calc_forces(const orc_system *s, const double *xyz, const orc_nlist *nl,
                         const orc_params *p, double *force, orc_energies *e) {
     const int n = s->n;

@@ -136,6 +136,13 @@ void orc_gb_calc_forces(const orc_system *s, const double *xyz, const orc_nlist 
                         double *born, double *deda);
 
 /* A8 FF_NonBonded_CalcForces_T_fast2 (energy/force, FPU sqrt, non-verbose). */
+/* FF_SoftVDW::calcForces_Soft (forcefields/ffsoftvdw.cpp:71-145): soft-core
+ * repulsion over list pairs of bond order above 1-4, hardness in kcal/mol,
+ * olap and vdw_cutoff in A. Adds to force (N) and e->epot; sets
+ * e->epot_vdw to the soft energy, as PD does. */
+void orc_softvdw_forces(const orc_system *s, const double *xyz, const orc_nlist *nl, double hardness,
+                        double olap, double vdw_cutoff, double *force, orc_energies *e);
+
 void orc_nb_calc_forces(const orc_system *s, const double *xyz, const orc_nlist *nl,
                         const orc_params *p, double *force, orc_energies *e);
 

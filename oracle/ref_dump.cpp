@@ -19,6 +19,7 @@
 #include "forcefields/ffbonded.h"
 #include "forcefields/nonbonded.h"
 #include "forcefields/gbff.h"
+#include "forcefields/ffsoftvdw.h"
 #include "protocols/minimise.h"
 #include "protocols/md.h"
 
@@ -400,6 +401,52 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
         r.s("sd_steps", (double)c.sd_steps);
         r.s("sd_done", (double)done);
         r.s("sd_stepsize", mini.StepSize);
+    }
+    {
+        // FF_SoftVDW alone (PD's defaults: Hardness 12, OlapOffset 0.6,
+        // VdwCutoff 15) at the built geometry
+        System sys(ffps);
+        sys.add(NewProtein(ffps, pd_grammar(c.seq)));
+        WorkSpace ws(sys);
+        ws.nlist().setPadding(c.padding);
+        Forcefield ff(ws);
+        FF_SoftVDW soft(ws);
+        ff.add(soft);
+        ff.printEnergySummary();
+        ff.calcForces();
+        dump_forces(r, ws, "sv_force");
+        r.s("sv_epot", ws.ene.epot);
+        r.s("sv_vdw", ws.ene.epot_vdw);
+    }
+    if (c.sd_steps > 0) {
+        // FF_Bonded + FF_SoftVDW steepest descent: PD's soft-core clash relief
+        System sys(ffps);
+        sys.add(NewProtein(ffps, pd_grammar(c.seq)));
+        WorkSpace ws(sys);
+        ws.nlist().setPadding(c.padding);
+        Forcefield ff(ws);
+        FF_Bonded bonds(ws);
+        FF_SoftVDW soft(ws);
+        ff.add(bonds);
+        ff.add(soft);
+        ff.printEnergySummary();
+        const int steps = c.sd_steps < 50 ? c.sd_steps : 50;
+        Protocol::Minimisation mini(ff);
+        mini.Algorithm = Protocol::Minimisation::SteepestDescent;
+        mini.Steps = steps;
+        mini.UpdateScr = 0;
+        mini.OutputLevel = Verbosity::Silent;
+        int done = mini.run();
+        std::vector<double> xyz;
+        for (int i = 0; i < (int)ws.atom.size(); ++i) {
+            xyz.push_back(ws.cur.atom[i].p.x); xyz.push_back(ws.cur.atom[i].p.y); xyz.push_back(ws.cur.atom[i].p.z);
+        }
+        r.d("svsd_xyz", xyz);
+        dump_forces(r, ws, "svsd_force");
+        r.s("svsd_epot", ws.ene.epot);
+        r.s("svsd_vdw", ws.ene.epot_vdw);
+        r.s("svsd_steps", (double)steps);
+        r.s("svsd_done", (double)done);
     }
     if (c.md_steps > 0) {
         // PD's MolecularDynamics with the VelocityVerlet integrator on

@@ -311,6 +311,9 @@ void setup_params(pdgpu_ctx *c) {
     P.s14e = o.elec14_scaling;
     P.econv_in = econv * (1.0 / o.dielectric_solute);
     P.econv_out = econv * (1.0 / o.dielectric_solvent);
+    P.soft_h = o.softvdw_hardness;
+    P.soft_olap = o.softvdw_olap_offset;
+    P.soft_cut2 = o.softvdw_cutoff * o.softvdw_cutoff;
     P.econv_in32 = (float)P.econv_in;
     P.econv_out32 = (float)P.econv_out;
     P.premul = -(econv * 0.5) * (1 / o.dielectric_solute - 1 / o.dielectric_solvent);
@@ -571,13 +574,30 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
     const int na = a1 - a0;
     const int ns = sp.s1 - sp.s0;
     const bool gb = c->opt.components & PDGPU_FF_GB;
+    const bool soft = c->opt.components & PDGPU_FF_SOFTVDW;
     {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
         hipLaunchKernelGGL(k_inst_clear, dim3((ns + 255) / 256), dim3(256), 0, c->stream, S, sp.s0, sp.s1);
         if (gb) hipLaunchKernelGGL(k_prepare<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
         else hipLaunchKernelGGL(k_clear_grad<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+        if (soft && c->mixed) // block-relative FP32 positions (k_prepare's job when GB runs)
+            hipLaunchKernelGGL(k_xyzq32, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
     }
     if (gb) TRY(launch_gb<R>(c, S, sp, span_tiles(c, id), mode));
+    if (soft) {
+        // FF_SoftVDW over the cutoff-filtered lists (calcEnergies is its
+        // calcForces, ffsoftvdw.cpp:56-59)
+        int g[kGridKinds], t[kGridKinds];
+        for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, span_tiles(c, id), &g[k], &t[k]);
+        {
+            ProfScope ps(c, PDGPU_KERNEL_ENUM);
+            hipLaunchKernelGGL(k_filter<R>, dim3(g[kGridEnum]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                               t[kGridEnum]);
+        }
+        ProfScope ps(c, PDGPU_KERNEL_PAIR);
+        hipLaunchKernelGGL(k_soft<R>, dim3(g[kGridPair]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                           t[kGridPair]);
+    }
     const BondedState B = c->B(id);
     const int nterms = (B.b1 - B.b0) + (B.a1 - B.a0) + (B.d1 - B.d0);
     if ((c->opt.components & PDGPU_FF_BONDED) && nterms > 0) {
@@ -784,7 +804,7 @@ bool use_graphs(const pdgpu_ctx *c) { return c->opt.use_graphs != 0 && !c->prof;
 // eagerly (a rebuild reads counts back); the kernels of the evaluation are
 // then replayed as a HIP graph when graphs are on.
 int eval(pdgpu_ctx *c, int id, int mode) {
-    if (c->opt.components & PDGPU_FF_GB) TRY(ensure_listed(c, id));
+    if (c->opt.components & (PDGPU_FF_GB | PDGPU_FF_SOFTVDW)) TRY(ensure_listed(c, id));
     auto issue = [&] { return c->mixed ? launch_eval<float>(c, id, mode) : launch_eval<double>(c, id, mode); };
     if (!use_graphs(c)) return issue();
     return graph_run(c, "calc:" + std::to_string(id) + ":" + std::to_string(mode), graph_sig(c, id), issue);
@@ -801,7 +821,8 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     *out = nullptr;
     if (o->fast_mode != 1) return PDGPU_ERR_UNSUPPORTED;
     if (o->precision != PDGPU_FP64 && o->precision != PDGPU_MIXED) return PDGPU_ERR_INVALID;
-    if ((o->components & ~PDGPU_FF_ALL) != 0) return PDGPU_ERR_INVALID;
+    if ((o->components & ~(PDGPU_FF_ALL | PDGPU_FF_SOFTVDW)) != 0) return PDGPU_ERR_INVALID;
+    if ((o->components & PDGPU_FF_GB) && (o->components & PDGPU_FF_SOFTVDW)) return PDGPU_ERR_UNSUPPORTED;
     if (!(o->cutoff > 0) || !(o->vdw_cutoff > 0) || o->inner_cutoff > o->cutoff ||
         o->vdw_inner_cutoff > o->vdw_cutoff || !(o->p5 > 0) || !(o->dielectric_solute > 0) ||
         !(o->dielectric_solvent > 0) || o->nlist_padding < 0)

@@ -55,6 +55,9 @@ struct PairParams {
     double econv_in;       // econv / DielectricSolute
     double econv_out;      // econv / DielectricSolvent
     float econv_in32, econv_out32; // the same rounded once, for the FP32 pair maths
+    double soft_h;         // FF_SoftVDW Hardness, kcal/mol
+    double soft_olap;      // FF_SoftVDW OlapOffset, A
+    double soft_cut2;      // FF_SoftVDW VdwCutoff^2
     double premul;         // -halfeconv*(1/eps_in - 1/eps_out)
     double still2;         // min(GbsaStillCutoff, list radius)^2
     double invP5, P4, P5;
@@ -1390,6 +1393,159 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
             wave_lds_sync();
         }
     }
+}
+
+// ------------------------------------------------ FF_SoftVDW soft core
+// FF_SoftVDW::calcForces_Soft (forcefields/ffsoftvdw.cpp:71-145): each pair
+// of bond order above 1-4 with r < s = R_i + R_j - OlapOffset and
+// r < VdwCutoff takes E = H (1 - r^2/s^2)^2, so (dE/dr)/r =
+// 4 H / s^2 (r^2/s^2 - 1). Returns (dE/dr)/r and the energy in e (both 0
+// outside the range).
+template <typename R>
+__device__ __forceinline__ R soft_term(const PairParams &P, R rsum, R r2, R &e) {
+    const R s = rsum - (R)P.soft_olap;
+    const R s2 = s * s;
+    if (!(r2 < s2 && r2 < (R)P.soft_cut2)) {
+        e = 0;
+        return 0;
+    }
+    const R is2 = Prec<R>::rcp(s2);
+    const R x = r2 * is2;
+    const R h = (R)P.soft_h;
+    e = h * ((R)1 - x) * ((R)1 - x);
+    return (R)4 * h * is2 * (x - (R)1);
+}
+
+template <typename R> struct SoftAtom { R x, y, z, rad; };
+
+template <typename R> __device__ __forceinline__ R ld_radius(const DevState &S, size_t a) {
+    if constexpr (Prec<R>::fixed) return S.lj32[a].x;
+    else return S.lj64[a].x;
+}
+
+// Same tile walk as the pair pass: filtered lists with LDS sums, dense
+// tiles by lane rotation from their bit rows (1-2, 1-3 and 1-4 pairs are
+// excluded through the excluded and 1-4 rows).
+template <typename R>
+__global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
+    __shared__ SoftAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
+    __shared__ double fi[kWavesPerGroup][3][kBlk], fj[kWavesPerGroup][3][kBlk];
+    const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
+    const int nw = gridDim.x * kWavesPerGroup;
+    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    const int l = lane_id(), w = threadIdx.x / 64;
+    InstSink<R> sink;
+    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
+        const int cend = min(t_end, c + tpw);
+        for (int t = c; t < cend; ++t) {
+            const int2 tw = S.tiles[t];
+            const int I = tw.x, J = tw.y & 0xFFFFFF;
+            const int inst = S.blk_inst[I];
+            if (inst < 0) continue;
+            const bool diag = (tw.y >> 24) & kTileDiag;
+            const int2 cnt = S.tile_cnt[t];
+            const int tslot = S.tile_slot[t];
+            const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
+            int *st = S.inst_status + inst;
+            LaneEnergy E;
+            if (diag || cnt.x < 0) {
+                const DiagBits D = diag_bits(S, tslot);
+                const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+                const AtomXYZ<R> pj = diag ? pi : load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+                const R ri = ld_radius<R>(S, a);
+                R xj = pj.x, yj = pj.y, zj = pj.z, rj = diag ? ri : ld_radius<R>(S, b);
+                R fjx = 0, fjy = 0, fjz = 0;
+                double fix = 0, fiy = 0, fiz = 0;
+                const unsigned long long skip = D.mx | D.m4;
+                const int ldir = (rot_i(l) - l) & 63;
+                const int steps = diag ? 32 : kBlk;
+                for (int k = 1; k <= steps; ++k) {
+                    xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj);
+                    fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
+                    const int jlane = (l + k * ldir) & 63;
+                    const R dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    if (diag_take(D, jlane, k, l, diag) && !((skip >> jlane) & 1ull)) {
+                        R e;
+                        const R g = soft_term<R>(P, ri + rj, r2, e);
+                        const R gx = g * dx, gy = g * dy, gz = g * dz;
+                        fix += (double)gx; fiy += (double)gy; fiz += (double)gz;
+                        fjx -= gx; fjy -= gy; fjz -= gz;
+                        E.vdw += (double)e;
+                    }
+                }
+                if (diag) {
+                    const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
+                                 gz = fiz + (double)diag_home(fjz);
+                    if (l < S.blk_n[I]) {
+                        acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
+                        acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
+                    }
+                } else { // 64 rotations bring the j-side sums home
+                    if (l < S.blk_n[I]) {
+                        acc_add<R>(S.acc_fx, a, fix, kForceScale, st);
+                        acc_add<R>(S.acc_fy, a, fiy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, a, fiz, kForceScale, st);
+                    }
+                    if (l < S.blk_n[J]) {
+                        acc_add<R>(S.acc_fx, b, (double)fjx, kForceScale, st);
+                        acc_add<R>(S.acc_fy, b, (double)fjy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, b, (double)fjz, kForceScale, st);
+                    }
+                }
+                sink.take(S, inst, E);
+                continue;
+            }
+            const int ntot = cnt.x + cnt.y;
+            if (ntot == 0) continue;
+            const uint16_t *L = S.plist + (size_t)t * kTileCap;
+            int e_nx = l < ntot ? (int)L[entry_pos(l, cnt.x)] : 0;
+            const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
+            const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
+            si[w][l] = {pi.x, pi.y, pi.z, ld_radius<R>(S, a)};
+            sj[w][l] = {pj.x, pj.y, pj.z, ld_radius<R>(S, b)};
+            for (int k = 0; k < 3; ++k) { fi[w][k][l] = 0; fj[w][k][l] = 0; }
+            wave_lds_sync();
+            for (int q0 = 0; q0 < ntot; q0 += kBlk) {
+                const int q = q0 + l;
+                const int e = e_nx;
+                e_nx = q + kBlk < ntot ? (int)L[entry_pos(q + kBlk, cnt.x)] : 0;
+                if (q < ntot && ((e >> 12) & 7) > 3) {
+                    const int ii = e & 63, jj = (e >> 6) & 63;
+                    const SoftAtom<R> A = si[w][ii], B = sj[w][jj];
+                    const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    R en;
+                    const R g = soft_term<R>(P, A.rad + B.rad, r2, en);
+                    if (g != (R)0 || en != (R)0) {
+                        const R gx = g * dx, gy = g * dy, gz = g * dz;
+                        atomicAdd(&fi[w][0][ii], (double)gx);
+                        atomicAdd(&fi[w][1][ii], (double)gy);
+                        atomicAdd(&fi[w][2][ii], (double)gz);
+                        atomicAdd(&fj[w][0][jj], (double)-gx);
+                        atomicAdd(&fj[w][1][jj], (double)-gy);
+                        atomicAdd(&fj[w][2][jj], (double)-gz);
+                        E.vdw += (double)en;
+                    }
+                }
+            }
+            wave_lds_sync();
+            if (l < S.blk_n[I]) {
+                acc_add<R>(S.acc_fx, a, fi[w][0][l], kForceScale, st);
+                acc_add<R>(S.acc_fy, a, fi[w][1][l], kForceScale, st);
+                acc_add<R>(S.acc_fz, a, fi[w][2][l], kForceScale, st);
+            }
+            if (l < S.blk_n[J]) {
+                acc_add<R>(S.acc_fx, b, fj[w][0][l], kForceScale, st);
+                acc_add<R>(S.acc_fy, b, fj[w][1][l], kForceScale, st);
+                acc_add<R>(S.acc_fz, b, fj[w][2][l], kForceScale, st);
+            }
+            sink.take(S, inst, E);
+            wave_lds_sync();
+        }
+    }
+    sink.flush(S);
 }
 
 // Forces f = -dE/dx in kcal/mol/A (double), dE/da for export.

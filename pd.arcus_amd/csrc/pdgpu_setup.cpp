@@ -43,6 +43,10 @@ extern "C" void pdgpu_options_default(pdgpu_options *o) {
     o->tiles_per_wave = 0;
     o->components = PDGPU_FF_ALL;
     o->use_graphs = 1;
+    // FF_SoftVDW::settodefault (forcefields/ffsoftvdw.h:39-47)
+    o->softvdw_hardness = 12.0;
+    o->softvdw_olap_offset = 0.6;
+    o->softvdw_cutoff = 15.0;
 }
 
 // BondOrder::reinit (workspace/bondorder.cpp:77-195). The band holds the

@@ -51,6 +51,7 @@ EXPORTS = (
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
 FF_BONDED = FF_BONDS | FF_ANGLES | FF_TORSIONS | FF_IMPROPERS
+FF_SOFTVDW = 32  # FF_SoftVDW soft-core repulsion (not in FF_ALL; not with FF_GB)
 FF_ALL = FF_GB | FF_BONDED
 KERNEL_GROUPS = ("nlist", "atom", "born", "pair", "chain", "integrate", "enum", "bonded")
 
@@ -70,7 +71,9 @@ class Options(C.Structure):
                    "exp_approx_threshold", "dielectric_offset", "born_radius_offset",
                    "p1", "p2", "p3", "p4", "p5", "nlist_padding")] + \
                [("device_memory_budget", C.c_int64), ("max_instances", C.c_int32), ("components", C.c_int32),
-                ("use_graphs", C.c_int32), ("dense_tile_pairs", C.c_int32)]
+                ("use_graphs", C.c_int32), ("dense_tile_pairs", C.c_int32),
+                ("reserved", C.c_int32), ("softvdw_hardness", C.c_double), ("softvdw_olap_offset", C.c_double),
+                ("softvdw_cutoff", C.c_double)]
 
 
 class Bonded(C.Structure):

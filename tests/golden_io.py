@@ -23,7 +23,8 @@ def load(name):
     d["n"] = len(d["charge"])
     d["xyz"] = d["xyz"].reshape(-1, 3)
     for k in ("force", "en_force", "nb_force", "sd_force", "sd_xyz", "bf_force", "bfp_force", "bfp_xyz",
-              "ff_force", "sdf_force", "sdf_xyz"):
+              "ff_force", "sdf_force", "sdf_xyz", "sv_force", "svsd_force", "svsd_xyz", "md_xyz0", "md_v0",
+              "md_xyz", "md_v"):
         if k in d:
             d[k] = d[k].reshape(-1, 3)
     d["mid"] = int(d["mid"][0])

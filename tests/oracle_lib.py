@@ -85,6 +85,9 @@ lib.orc_bonded_forces.argtypes = [C.POINTER(Bonded), _dp, _dp, C.POINTER(Energie
 lib.orc_minimise_sd.restype = C.c_int
 lib.orc_minimise_sd.argtypes = [C.POINTER(System), _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                                 _dp, C.POINTER(Energies), _dp]
+lib.orc_softvdw_forces.restype = None
+lib.orc_softvdw_forces.argtypes = [C.POINTER(System), _dp, C.POINTER(NList), C.c_double, C.c_double, C.c_double,
+                                   _dp, C.POINTER(Energies)]
 lib.orc_md_vv_pd.restype = C.c_int
 lib.orc_md_vv_pd.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                              _dp, _dp]
@@ -286,3 +289,14 @@ def md_vv_pd(sysm, xyz, vel, params, steps, dt, update_nlist=10):
     rc = lib.orc_md_vv_pd(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), C.byref(params), steps, dt,
                           update_nlist, _ptr(te, _dp), _ptr(tk, _dp))
     return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk
+
+
+def softvdw_forces(sysm, xyz, rs, ent, hardness=12.0, olap=0.6, vdw_cutoff=15.0):
+    """FF_SoftVDW::calcForces_Soft over a shadow list (PD units)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    nl, keep = _nl(rs, ent)
+    f = np.zeros(3 * sysm.n)
+    e = Energies()
+    lib.orc_softvdw_forces(C.byref(sysm.c), _ptr(xyz, _dp), C.byref(nl), hardness, olap, vdw_cutoff, _ptr(f, _dp),
+                           C.byref(e))
+    return e.as_dict(), f.reshape(-1, 3)

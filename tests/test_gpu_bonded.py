@@ -157,3 +157,46 @@ def test_pd_velocity_verlet_matches_reference(name, precision):
     assert np.abs(v - vref).max() <= (1e-7 if precision == pdgpu.FP64 else 1e-2) * np.abs(vref).max()
     assert rel(e["epot"], g["md_epot"]) <= te
     assert rel(ek, g["md_ekin"]) <= te
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("precision", [pdgpu.FP64, pdgpu.MIXED])
+def test_softvdw_alone(name, precision):
+    """FF_SoftVDW on the device against the reference at the built geometry
+    (PD's defaults: Hardness 12 kcal/mol, OlapOffset 0.6, VdwCutoff 15)."""
+    g = G.load(name)
+    te, tf = (1e-10, 1e-9) if precision == pdgpu.FP64 else (1e-5, 1e-5)
+    with context_for(g, precision, components=pdgpu.FF_SOFTVDW) as ctx:
+        i = ctx.add_system(g)
+        e, f = ctx.calc_forces(i)
+        assert e["status"] == 0
+        assert rel(e["epot"], g["sv_epot"]) <= te
+        assert rel(e["epot_vdw"], g["sv_vdw"]) <= te
+        assert e["epot_pol"] == 0 and e["epot_elec"] == 0
+        assert rms_rel(f, g["sv_force"]) <= tf
+
+
+SVSD_CASES = [c for c in CASES if "svsd_xyz" in G.load(c)]
+
+
+@pytest.mark.parametrize("name", SVSD_CASES)
+def test_soft_clash_relief_steepest_descent(name):
+    """PD's soft-core clash relief, Minimisation (SteepestDescent) over
+    FF_Bonded + FF_SoftVDW, on the device against the reference's run."""
+    g = G.load(name)
+    steps = int(g["svsd_steps"])
+    with context_for(g, pdgpu.FP64, components=pdgpu.FF_BONDED | pdgpu.FF_SOFTVDW) as ctx:
+        i = ctx.add_system(g, bonded=True)
+        done = ctx.minimise_sd(i, steps, g["sd_stepsize"], 1)
+        e, f = ctx.calc_forces(i)
+        x = ctx.get_positions(i)
+    assert done == steps == int(g["svsd_done"])
+    assert rel(e["epot"], g["svsd_epot"]) <= 1e-9
+    assert np.abs(x - g["svsd_xyz"].reshape(-1, 3)).max() <= 1e-7
+    assert rms_rel(f, g["svsd_force"]) <= 1e-7
+
+
+def test_softvdw_with_gb_is_unsupported():
+    g = G.load("tri_ala")
+    with pytest.raises(pdgpu.PdgpuError):
+        context_for(g, pdgpu.FP64, components=pdgpu.FF_GB | pdgpu.FF_SOFTVDW)
