@@ -307,6 +307,7 @@ void setup_params(pdgpu_ctx *c) {
     P.vcut = o.vdw_cutoff;
     P.vinner = o.vdw_inner_cutoff;
     P.vinvSw = o.vdw_cutoff > o.vdw_inner_cutoff ? 1.0 / (o.vdw_cutoff - o.vdw_inner_cutoff) : 0.0;
+    P.same_switch = P.inner == P.vinner && P.invSw == P.vinvSw;
     P.s14v = o.vdw14_scaling;
     P.s14e = o.elec14_scaling;
     P.econv_in = econv * (1.0 / o.dielectric_solute);

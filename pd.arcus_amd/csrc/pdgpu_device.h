@@ -55,6 +55,7 @@ struct PairParams {
     double econv_in;       // econv / DielectricSolute
     double econv_out;      // econv / DielectricSolvent
     float econv_in32, econv_out32; // the same rounded once, for the FP32 pair maths
+    int same_switch;       // the vdW switch equals the elec/GB switch (same radii)
     double soft_h;         // FF_SoftVDW Hardness, kcal/mol
     double soft_olap;      // FF_SoftVDW OlapOffset, A
     double soft_cut2;      // FF_SoftVDW VdwCutoff^2
@@ -671,8 +672,10 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
 // front, the others from the back.
 __device__ __forceinline__ int entry_pos(int q, int nb) { return q < nb ? q : kTileCap - 1 - (q - nb); }
 
-struct LaneEnergy {
-    double vdw = 0, elec = 0, pol = 0, vir = 0;
+// Per-lane energy sums over one tile, in the pass's precision (a lane adds
+// at most 64 terms before the tile's sums are reduced in FP64).
+template <typename R> struct LaneEnergy {
+    R vdw = 0, elec = 0, pol = 0, vir = 0;
     unsigned int pairs = 0;
 };
 
@@ -712,12 +715,12 @@ template <typename R> struct InstSink {
         }
     }
     // wave-uniform: call from every lane of the wavefront
-    __device__ __forceinline__ void take(const DevState &S, int next, const LaneEnergy &le) {
+    __device__ __forceinline__ void take(const DevState &S, int next, const LaneEnergy<R> &le) {
         to(S, next);
-        energy(0, wave_sum(le.vdw));
-        energy(1, wave_sum(le.elec));
-        energy(2, wave_sum(le.pol));
-        energy(3, wave_sum(le.vir));
+        energy(0, wave_sum((double)le.vdw));
+        energy(1, wave_sum((double)le.elec));
+        energy(2, wave_sum((double)le.pol));
+        energy(3, wave_sum((double)le.vir));
         n[kCPairs] += wave_sum(le.pairs);
     }
     __device__ __forceinline__ void flush(const DevState &S) {
@@ -952,7 +955,7 @@ template <typename R> struct PairAtom { R x, y, z, q, rad, eps, a; };
 // adds dE/d(a) contributions to dai / daj (MODE 0) and energies to E.
 template <typename R, int MODE>
 __device__ __forceinline__ R pair_term(const PairParams &P, const PairAtom<R> &A, const PairAtom<R> &B, int bo,
-                                       R r2, LaneEnergy &E, R &dai, R &daj) {
+                                       R r2, LaneEnergy<R> &E, R &dai, R &daj) {
     const R inner = (R)P.inner, invSw = (R)P.invSw;
     const R vcut = (R)P.vcut, vinner = (R)P.vinner, vinvSw = (R)P.vinvSw;
     const R premul = (R)P.premul;
@@ -992,7 +995,7 @@ __device__ __forceinline__ R pair_term(const PairParams &P, const PairAtom<R> &A
             dedal *= Sw;
             epol *= Sw;
             g = dedr * (R)2 * rinv;
-            E.pol += 2.0 * (double)epol;
+            E.pol += (R)2 * epol;
             dai = dedal * B.a;
             daj = dedal * A.a;
         }
@@ -1006,12 +1009,17 @@ __device__ __forceinline__ R pair_term(const PairParams &P, const PairAtom<R> &A
         const R Ap = Bp * Bp;
         const R eps = lj ? s14v * A.eps * B.eps : (R)0;
         R vS, vd;
-        switch_fn_clamped<R>(r, vinner, vinvSw, vS, vd);
+        if (P.same_switch) { // identical inputs give identical values
+            vS = Sw;
+            vd = dS;
+        } else {
+            switch_fn_clamped<R>(r, vinner, vinvSw, vS, vd);
+        }
         R vdw_pot = ((R)2 * eps) * ((R)0.5 * Ap - Bp);
         const R vdw_force = vS * (-((R)12 * eps * rinv) * (Ap - Bp)) + vdw_pot * vd;
         vdw_pot *= vS;
-        E.vdw += (double)vdw_pot;
-        E.elec += (double)elec_pot;
+        E.vdw += vdw_pot;
+        E.elec += elec_pot;
         g += (vdw_force + elec_force) * rinv;
     } else {
         // FF_NonBonded energy path, bond-order scale tables {0,0,0,s14,1,1,1,1}
@@ -1057,10 +1065,10 @@ __device__ __forceinline__ R pair_term(const PairParams &P, const PairAtom<R> &A
                 elec_pot *= (rinv - (R)P.fs_eshift);
             }
         }
-        E.vdw += (double)vdw_pot;
-        E.elec += (double)elec_pot;
+        E.vdw += vdw_pot;
+        E.elec += elec_pot;
         const R fm = vdw_force + elec_force;
-        E.vir += (double)(r * fm);
+        E.vir += r * fm;
         g = fm * rinv;
         // exact pairwise GB energy, no threshold branch
         if (r <= (R)P.cut) {
@@ -1069,7 +1077,7 @@ __device__ __forceinline__ R pair_term(const PairParams &P, const PairAtom<R> &A
             const R s2 = r * r;
             const R aiaj = A.a * B.a;
             const R fgb = Prec<R>::rsqrt(s2 + aiaj * Prec<R>::expo(-s2 * (R)0.25 * Prec<R>::rcp(aiaj)));
-            E.pol += (double)(premul * (R)2 * Sw * qiqj * fgb);
+            E.pol += premul * (R)2 * Sw * qiqj * fgb;
         }
     }
     return g;
@@ -1117,7 +1125,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                 }
                 R fjx = 0, fjy = 0, fjz = 0, daj_acc = 0;
                 double fix = 0, fiy = 0, fiz = 0, dai_acc = 0;
-                LaneEnergy E;
+                LaneEnergy<R> E;
                 const int ldir = (rot_i(l) - l) & 63;
                 const int steps = diag ? 32 : kBlk;
                 for (int k = 1; k <= steps; ++k) {
@@ -1185,7 +1193,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
             sj[w][l] = {pj.x, pj.y, pj.z, pj.q, Rj, ej, ld_born<R>(S, b)};
             for (int k = 0; k < 4; ++k) { fi[w][k][l] = 0; fj[w][k][l] = 0; }
             wave_lds_sync();
-            LaneEnergy E;
+            LaneEnergy<R> E;
             for (int q0 = 0; q0 < ntot; q0 += kBlk) {
                 const int q = q0 + l;
                 const int e = e_nx;
@@ -1447,7 +1455,7 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
             const int tslot = S.tile_slot[t];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             int *st = S.inst_status + inst;
-            LaneEnergy E;
+            LaneEnergy<R> E;
             if (diag || cnt.x < 0) {
                 const DiagBits D = diag_bits(S, tslot);
                 const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
@@ -1471,7 +1479,7 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
                         const R gx = g * dx, gy = g * dy, gz = g * dz;
                         fix += (double)gx; fiy += (double)gy; fiz += (double)gz;
                         fjx -= gx; fjy -= gy; fjz -= gz;
-                        E.vdw += (double)e;
+                        E.vdw += e;
                     }
                 }
                 if (diag) {
@@ -1526,7 +1534,7 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
                         atomicAdd(&fj[w][0][jj], (double)-gx);
                         atomicAdd(&fj[w][1][jj], (double)-gy);
                         atomicAdd(&fj[w][2][jj], (double)-gz);
-                        E.vdw += (double)en;
+                        E.vdw += en;
                     }
                 }
             }
