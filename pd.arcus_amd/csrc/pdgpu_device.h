@@ -614,13 +614,16 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
         const int cend = min(t_end, c + tpw);
         for (int t = c; t < cend; ++t) {
             // the tile record and its candidate slot come first; the slot's
-            // count and first candidate chunk (every slot holds kTileCap
+            // count and first 256 candidates (every slot holds kTileCap
             // entries, and tile_slot only ever names allocated slots) are
-            // requested together with the block's records
+            // requested together with the block's records. A lane reads four
+            // consecutive 2-byte candidates per 8-byte load, so a tile of up
+            // to 256 candidates is one round trip.
             const int2 tw = S.tiles[t];
             const long long slot = S.tile_slot[t];
-            const uint16_t *C = S.cand + slot * kTileCap;
-            int e_nx = (int)C[l]; // candidates are fetched one chunk ahead
+            const unsigned long long *C4 =
+                reinterpret_cast<const unsigned long long *>(S.cand + slot * kTileCap);
+            unsigned long long v_nx = C4[l];
             const int n = S.cand_n[slot];
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
@@ -637,7 +640,6 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
                 if (l == 0) S.tile_cnt[t] = make_int2(-1, -1);
                 continue;
             }
-            if (l >= n) e_nx = 0;
             uint16_t *L = S.plist + (size_t)t * kTileCap;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
@@ -646,21 +648,36 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
             sj[w][l] = {pj.x, pj.y, pj.z};
             wave_lds_sync();
             int nb = 0, no = 0;
-            for (int q0 = 0; q0 < n; q0 += kBlk) {
-                const int q = q0 + l;
-                const bool in = q < n;
-                const int e = e_nx;
-                e_nx = q + kBlk < n ? (int)C[q + kBlk] : 0;
-                const PosR<R> A = si[w][e & 63], B = sj[w][(e >> 6) & 63];
-                const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
-                const R r2 = dx * dx + dy * dy + dz * dz;
-                const bool born = in && ((e >> 12) & 7) > 2 && r2 <= still2;
-                const bool oth = in && !born && r2 <= cut2;
-                const unsigned long long mb = __ballot(born), mo = __ballot(oth);
-                if (born) L[nb + (int)__popcll(mb & below)] = (uint16_t)e;
-                if (oth) L[kTileCap - 1 - (no + (int)__popcll(mo & below))] = (uint16_t)e;
-                nb += (int)__popcll(mb);
-                no += (int)__popcll(mo);
+            // chunks of 256 candidates: lane l loads 4l .. 4l+3 (q0 + 255 <
+            // kTileCap whenever q0 < n, so every load stays in the slot) and
+            // step k hands lane l candidate q0 + 64k + l through a lane
+            // permute, so entries are taken in candidate order: within 64
+            // consecutive list entries (one rotation step of the
+            // enumeration) no i or j atom repeats, which keeps the later
+            // passes' LDS sums free of same-address conflicts
+            for (int q0 = 0; q0 < n; q0 += 4 * kBlk) {
+                const unsigned long long v = v_nx;
+                if (q0 + 4 * kBlk < n) v_nx = C4[(q0 >> 2) + kBlk + l];
+                const unsigned int vlo = (unsigned int)v, vhi = (unsigned int)(v >> 32);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int src = 16 * k + (l >> 2);
+                    const unsigned int lo = __shfl(vlo, src, 64), hi = __shfl(vhi, src, 64);
+                    const unsigned int word = (l & 2) ? hi : lo;
+                    const int e = (int)((word >> (16 * (l & 1))) & 0xFFFFu);
+                    const int q = q0 + kBlk * k + l;
+                    const bool in = q < n;
+                    const PosR<R> A = si[w][e & 63], B = sj[w][(e >> 6) & 63];
+                    const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    const bool born = in && ((e >> 12) & 7) > 2 && r2 <= still2;
+                    const bool oth = in && !born && r2 <= cut2;
+                    const unsigned long long mb = __ballot(born), mo = __ballot(oth);
+                    if (born) L[nb + (int)__popcll(mb & below)] = (uint16_t)e;
+                    if (oth) L[kTileCap - 1 - (no + (int)__popcll(mo & below))] = (uint16_t)e;
+                    nb += (int)__popcll(mb);
+                    no += (int)__popcll(mo);
+                }
             }
             if (l == 0) S.tile_cnt[t] = make_int2(nb, no);
             wave_lds_sync();
