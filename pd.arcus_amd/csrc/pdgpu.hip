@@ -566,9 +566,11 @@ int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, in
     return PDGPU_OK;
 }
 
-// One evaluation: MODE 0 = calcForces, MODE 1 = calcEnergies.
+// One evaluation: MODE 0 = calcForces, MODE 1 = calcEnergies. Without
+// finalize the caller launches a kernel that finalises the forces itself
+// (the velocity-Verlet kick).
 template <typename R>
-int launch_eval(pdgpu_ctx *c, int id, int mode) {
+int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true) {
     const DevState S = c->S();
     const Span sp = span_of(c, id);
     const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk;
@@ -578,9 +580,12 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
     const bool soft = c->opt.components & PDGPU_FF_SOFTVDW;
     {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
-        hipLaunchKernelGGL(k_inst_clear, dim3((ns + 255) / 256), dim3(256), 0, c->stream, S, sp.s0, sp.s1);
-        if (gb) hipLaunchKernelGGL(k_prepare<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
-        else hipLaunchKernelGGL(k_clear_grad<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+        const int nt = std::max(na, ns);
+        if (gb)
+            hipLaunchKernelGGL(k_prepare<R>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, S, a0, a1, sp.s0, sp.s1);
+        else
+            hipLaunchKernelGGL(k_clear_grad<R>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, S, a0, a1, sp.s0,
+                               sp.s1);
         if (soft && c->mixed) // block-relative FP32 positions (k_prepare's job when GB runs)
             hipLaunchKernelGGL(k_xyzq32, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
     }
@@ -605,7 +610,7 @@ int launch_eval(pdgpu_ctx *c, int id, int mode) {
         ProfScope ps(c, PDGPU_KERNEL_BONDED);
         hipLaunchKernelGGL(k_bonded<R>, dim3((nterms + 255) / 256), dim3(256), 0, c->stream, S, B);
     }
-    {
+    if (finalize) {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
         hipLaunchKernelGGL(k_finalize<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
     }

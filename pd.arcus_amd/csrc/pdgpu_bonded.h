@@ -260,8 +260,10 @@ __global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
 // Clears the force accumulators where the GB kernels (which clear them in
 // k_born_final) do not run.
 template <typename R>
-__global__ void k_clear_grad(DevState S, int atom_begin, int atom_end) {
-    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_clear_grad(DevState S, int atom_begin, int atom_end, int slot_begin, int slot_end) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot_begin + t < slot_end) inst_clear(S, slot_begin + t); // the per-instance records too
+    const int a = atom_begin + t;
     if (a >= atom_end) return;
     acc_store<R>(S.acc_fx, a, 0.0, 1.0);
     acc_store<R>(S.acc_fy, a, 0.0, 1.0);

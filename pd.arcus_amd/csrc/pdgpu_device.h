@@ -239,14 +239,13 @@ __device__ __forceinline__ int wave_global_id() {
 }
 
 // ------------------------------------------------------ per-instance setup
-__global__ void k_inst_clear(DevState S, int slot_begin, int slot_end) {
-    const int s = slot_begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= slot_end) return;
+__device__ __forceinline__ void inst_clear(const DevState &S, int s) {
     unsigned long long *e = reinterpret_cast<unsigned long long *>(S.inst_energy) + (size_t)s * kENum;
     for (int k = 0; k < kENum; ++k) e[k] = 0;
     for (int k = 0; k < kCNum; ++k) S.inst_count[(size_t)s * kCNum + k] = 0;
     S.inst_status[s] = 0;
 }
+
 
 // --------------------------------------------------- neighbour-list build
 // Bounding box and reference centre of each block (one wave per block).
@@ -404,8 +403,10 @@ __global__ void k_xyzq32(DevState S, int atom_begin, int atom_end) {
 }
 
 template <typename R>
-__global__ void k_prepare(DevState S, int atom_begin, int atom_end) {
-    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_prepare(DevState S, int atom_begin, int atom_end, int slot_begin, int slot_end) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot_begin + t < slot_end) inst_clear(S, slot_begin + t); // the per-instance records too
+    const int a = atom_begin + t;
     if (a >= atom_end) return;
     acc_store<R>(S.acc_g, a, 0.0, 1.0);
     if constexpr (Prec<R>::fixed) store_xyzq32(S, a);

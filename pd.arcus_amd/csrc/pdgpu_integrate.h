@@ -225,6 +225,42 @@ __global__ void k_vv_pd(DevState S, InstMD *md, const double *minv, double *vx, 
     }
 }
 
+// k_finalize and the closing half kick of a velocity-Verlet step in one
+// pass; with next != 0 also the following step's opening half kick and
+// drift (the same arithmetic as the kernels in sequence).
+template <typename R>
+__global__ void k_finalize_kick(DevState S, InstMD *md, const double *minv, double *vx, double *vy, double *vz,
+                                double *px, double *py, double *pz, int atom_begin, int atom_end, double dt,
+                                double f_to_n, int want_ekin, int next) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = a / kBlk;
+    const bool in = a < atom_end;
+    const int inst = in ? S.blk_inst[b] : -1;
+    const bool v = in && inst >= 0 && (a % kBlk) < S.blk_n[b];
+    double ek = 0;
+    if (v) {
+        const double fx = -acc_read<R>(S.acc_fx, a, kForceScale);
+        const double fy = -acc_read<R>(S.acc_fy, a, kForceScale);
+        const double fz = -acc_read<R>(S.acc_fz, a, kForceScale);
+        S.fx[a] = fx; S.fy[a] = fy; S.fz[a] = fz;
+        S.deda_out[a] = acc_read<R>(S.acc_deda, a, kForceScale);
+        if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicOr(S.inst_status + inst, kStatNonFinite);
+        const double h = 0.5 * dt * minv[a] * f_to_n;
+        double ux = vx[a] + fx * h, uy = vy[a] + fy * h, uz = vz[a] + fz * h;
+        if (minv[a] > 0) ek = 0.5 / minv[a] * (ux * ux + uy * uy + uz * uz);
+        if (next) {
+            ux += fx * h; uy += fy * h; uz += fz * h;
+            const double s = dt / 1e-10;
+            px[a] += ux * s; py[a] += uy * s; pz[a] += uz * s;
+        }
+        vx[a] = ux; vy[a] = uy; vz[a] = uz;
+    }
+    if (want_ekin) {
+        ek = wave_sum(ek);
+        if (lane_id() == 0 && in && inst >= 0 && ek != 0.0) unsafeAtomicAdd(&md[inst].ekin, ek);
+    }
+}
+
 __global__ void k_ekin_clear(InstMD *md, int s0, int s1) {
     const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (s < s1) md[s].ekin = 0;
