@@ -950,7 +950,6 @@ __global__ void k_born_final(DevState S, PairParams P, int atom_begin, int atom_
         acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEPolSelf, tot, kEnergyScale, S.inst_status + inst);
 }
 
-// Energies of one lane over one tile.
 // Switching function S and dS/dr of PD (forcefields/gbff.cpp:1605-1614).
 template <typename R>
 __device__ __forceinline__ void switch_fn(R r, R inner, R invSw, R &S, R &dS) {
