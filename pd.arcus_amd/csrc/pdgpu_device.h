@@ -167,6 +167,17 @@ template <> struct Prec<float> {
 };
 
 // Accumulate v into an 8-byte slot: fixed point (mixed) or double (FP64).
+// Round-to-nearest-even double -> int64 (what __double2ll_rn gives). Below
+// 2^51 in magnitude, adding 1.5 * 2^52 leaves the rounded integer in the
+// low mantissa bits, two instructions instead of the general sequence.
+__device__ __forceinline__ long long to_fixed(double s) {
+    if (fabs(s) < 0x1p51) {
+        const double m = 0x1.8p52;
+        return __double_as_longlong(s + m) - __double_as_longlong(m);
+    }
+    return __double2ll_rn(s);
+}
+
 template <typename R>
 __device__ __forceinline__ void acc_add(void *base, size_t idx, double v, double scale, int *status) {
     if constexpr (Prec<R>::fixed) {
@@ -175,7 +186,7 @@ __device__ __forceinline__ void acc_add(void *base, size_t idx, double v, double
             atomicOr(status, (s == s) ? kStatOverflow : kStatNonFinite);
             return;
         }
-        const long long iv = __double2ll_rn(s);
+        const long long iv = to_fixed(s);
         if (iv != 0) atomicAdd(reinterpret_cast<unsigned long long *>(base) + idx, (unsigned long long)iv);
     } else {
         if (!(fabs(v) < 1e300)) atomicOr(status, kStatNonFinite);
@@ -196,7 +207,7 @@ template <typename R>
 __device__ __forceinline__ void acc_store(void *base, size_t idx, double v, double scale) {
     if constexpr (Prec<R>::fixed) {
         const double s = v * scale;
-        reinterpret_cast<long long *>(base)[idx] = (fabs(s) < kFixedLimit) ? __double2ll_rn(s) : 0;
+        reinterpret_cast<long long *>(base)[idx] = (fabs(s) < kFixedLimit) ? to_fixed(s) : 0;
     } else {
         reinterpret_cast<double *>(base)[idx] = v;
     }
@@ -726,7 +737,7 @@ template <typename R> struct InstSink {
                 bad |= (sv == sv) ? kStatOverflow : kStatNonFinite;
                 return;
             }
-            e[k] += __double2ll_rn(sv);
+            e[k] += to_fixed(sv);
         } else {
             if (!(fabs(v) < 1e300)) bad |= kStatNonFinite;
             e[k] += v;
