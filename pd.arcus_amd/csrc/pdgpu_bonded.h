@@ -1,12 +1,14 @@
 // pdgpu_bonded.h -- FF_Bonded on the device (forcefields/ffbonded.cpp:809-1590):
 // harmonic bonds, harmonic angles, Fourier / harmonic dihedrals for
 // torsions and impropers. One work item per term; every term reads its
-// 2-4 atoms' FP64 positions and adds its gradients to the same per-atom
-// accumulators as the pair kernels (64-bit fixed point in the mixed path,
-// FP64 in the FP64 path), so bonded and GB forces come out of one
-// k_finalize. The arithmetic is FP64 in both precisions. (Summing a
-// workgroup's gradients in an LDS window first measured no faster than the
-// direct atomics: zeroing and flushing the window costs what it saves.)
+// 2-4 atoms' FP64 positions, and the gradients of a workgroup's terms are
+// summed in LDS before they reach the same per-atom accumulators as the pair
+// kernels (64-bit fixed point in the mixed path, FP64 in the FP64 path), so
+// bonded and GB forces come out of one k_finalize. The arithmetic is FP64 in
+// both precisions. (One global atomic per term and component instead of the
+// LDS window measured several times slower once the terms are off their
+// minima; at a freshly built geometry most bond and angle gradients round to
+// zero and skip their atomics, which hides the cost.)
 #pragma once
 
 #include "pdgpu_device.h"
@@ -45,16 +47,41 @@ __device__ __forceinline__ D3 d3_lin(double s, const D3 &a, double t, const D3 &
 }
 __device__ __forceinline__ D3 d3_at(const DevState &S, int a) { return {S.px[a], S.py[a], S.pz[a]}; }
 
-// Gradient sink: one atomic per component into the per-atom accumulators.
-// In the mixed path every contribution is rounded to 64-bit fixed point on
-// its own, so the sums are exact integers whatever the order.
+// Gradient sink of one workgroup. Terms are stored instance by instance in
+// atom order, so the 256 terms of a workgroup touch a narrow range of atoms:
+// their gradients are summed in an LDS window of kWin atoms starting at the
+// smallest atom index of the workgroup, and the window is added to the
+// global accumulators once at the end. Atoms beyond the window go straight
+// to the global accumulators. In the mixed path every contribution is
+// rounded to 64-bit fixed point before it is added, so the sums are exact
+// integers whatever the grouping.
+constexpr int kWin = 1024;
+
 template <typename R> struct GradSink {
+    unsigned long long (*win)[kWin]; // [3][kWin]: int64 fixed point (mixed) or double bits (FP64)
+    int amin;
     const DevState *S;
 
     __device__ void add(int a, const D3 &g, int *st) const {
-        acc_add<R>(S->acc_fx, a, g.x, kForceScale, st);
-        acc_add<R>(S->acc_fy, a, g.y, kForceScale, st);
-        acc_add<R>(S->acc_fz, a, g.z, kForceScale, st);
+        const double v[3] = {g.x, g.y, g.z};
+        const int k = a - amin;
+        if (k >= 0 && k < kWin) {
+            for (int c = 0; c < 3; ++c) {
+                if constexpr (Prec<R>::fixed) {
+                    const double sc = v[c] * kForceScale;
+                    if (!(fabs(sc) < kFixedLimit)) { atomicOr(st, (sc == sc) ? kStatOverflow : kStatNonFinite); continue; }
+                    const long long iv = to_fixed(sc);
+                    if (iv != 0) atomicAdd(&win[c][k], (unsigned long long)iv);
+                } else {
+                    if (!(fabs(v[c]) < 1e300)) atomicOr(st, kStatNonFinite);
+                    if (v[c] != 0.0) atomicAdd(reinterpret_cast<double *>(&win[c][k]), v[c]);
+                }
+            }
+        } else {
+            acc_add<R>(S->acc_fx, a, g.x, kForceScale, st);
+            acc_add<R>(S->acc_fy, a, g.y, kForceScale, st);
+            acc_add<R>(S->acc_fz, a, g.z, kForceScale, st);
+        }
     }
 };
 
@@ -141,6 +168,8 @@ __device__ double dihedral_term(const DevState &S, const GradSink<R> &G, const i
 // [d0,d1) dihedrals, one thread per record.
 template <typename R>
 __global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
+    __shared__ unsigned long long win[3][kWin];
+    __shared__ int s_amin;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int nb = B.b1 - B.b0, na = B.a1 - B.a0, nd = B.d1 - B.d0;
     // decode this thread's record
@@ -164,7 +193,13 @@ __global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
             kind = 2 + impr; inst = (meta & 0xFFFFFF) - 1; q = B.dh_idx[r];
         }
     }
-    const GradSink<R> G{&S};
+    // LDS window at the workgroup's smallest atom index
+    if (threadIdx.x == 0) s_amin = 0x7FFFFFFF;
+    for (int k = threadIdx.x; k < 3 * kWin; k += blockDim.x) (&win[0][0])[k] = 0ull;
+    __syncthreads();
+    if (kind >= 0) atomicMin(&s_amin, min(min(q.x, q.y), min(q.z, q.w)));
+    __syncthreads();
+    const GradSink<R> G{win, s_amin, &S};
     int slot = kEBond;
     double e = 0, vir = 0;
     if (kind == 0) {
@@ -208,6 +243,20 @@ __global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
     }
     add_term_energy<R>(S, kind >= 0, inst, slot, e);
     add_term_energy<R>(S, kind == 0, inst, kEVirial, vir);
+    __syncthreads();
+    // flush the window (only atoms that received something)
+    for (int k = threadIdx.x; k < kWin; k += blockDim.x) {
+        for (int c = 0; c < 3; ++c) {
+            const unsigned long long v = win[c][k];
+            if (v == 0ull) continue;
+            void *base = c == 0 ? S.acc_fx : c == 1 ? S.acc_fy : S.acc_fz;
+            if constexpr (Prec<R>::fixed) {
+                atomicAdd(reinterpret_cast<unsigned long long *>(base) + (s_amin + k), v);
+            } else {
+                unsafeAtomicAdd(reinterpret_cast<double *>(base) + (s_amin + k), __longlong_as_double((long long)v));
+            }
+        }
+    }
 }
 
 // Clears the force accumulators where the GB kernels (which clear them in
