@@ -276,6 +276,34 @@ int pdgpu_md_vv(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t up
 int pdgpu_md_vv_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
                    pdgpu_energies *energies, double *ekin);
 
+/* MolecularDynamics::runcore with Integrator = Langevin and
+ * LangevinOnHydrogens (PD's default; protocols/md.cpp:906-1043 update,
+ * :209-233 step loop): each step refreshes the list when due, evaluates
+ * forces, then (after the first step) v += (1 - c0/c1) f dt/m / (gamma dt),
+ * takes ekin, and (before the last step) moves p by dt (c1 v + c2 f dt/m)
+ * + sigma_r n1 and sets v = c0 v + c0 c2 f dt/m / c1 + sigma_v (crv n1 +
+ * sqrt(1 - crv^2) n2), with PD's coefficients for gamma dt (exponential
+ * form above 0.05, its Taylor series below; below 1e-5 the VelocityVerlet
+ * update, as PD does). temperature in K (constant), friction gamma in 1/s
+ * (PD's FricCoeff, default 1e13). Normal deviates: noise == NULL draws
+ * them on the device from Philox4x32-10 keyed by seed over (atom,
+ * instance, step counter of the context); a single id may instead pass
+ * noise = [steps][natoms][6] doubles (n1x, n1y, n1z, n2x, n2y, n2z), e.g.
+ * the reference's libc rand() stream, for a step-for-step reproduction of
+ * a PD run. */
+int pdgpu_md_langevin_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
+                         double temperature, double friction, uint64_t seed, const double *noise,
+                         pdgpu_energies *energies, double *ekin);
+
+/* MolecularDynamics::runcore with Integrator = Beeman (PD's default
+ * integrator; protocols/md.cpp:840-904), no thermostat: after the first
+ * step v += 2 f dt/6m, ekin taken, and before the last step p += dt (v +
+ * (4 f - f_old) dt/6m), v += (5 f - f_old) dt/6m, f_old = f. The old
+ * forces (WorkSpace::old in PD) are kept per atom between calls and are
+ * zero when an instance is added. */
+int pdgpu_md_beeman_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
+                       pdgpu_energies *energies, double *ekin);
+
 /* Grid-shape autotuner (no reference counterpart): times the tile kernels
  * of a force evaluation of id (or PDGPU_ALL) reps times at each of several
  * wavefronts-per-CU settings and keeps the fastest per kernel for later

@@ -29,6 +29,14 @@
  *                        protocols/md.cpp:788-838)
  *   orc_md_vv_pd         PD's own MolecularDynamics runs with the VelocityVerlet
  *                        integrator (compiled reference, tests/golden md_*)
+ *   orc_langevin_coef    exp-form and Taylor-form coefficients agree where both
+ *                        hold; gamma*dt -> 0 limits (c0, c1, c2 -> 1, 1, 1/2)
+ *   orc_langevin_noise   PD's nrand over libc rand() in the reference's call
+ *                        order; the Langevin golden run uses the same stream
+ *   orc_md_langevin_pd   PD's own MolecularDynamics runs with the Langevin
+ *                        integrator (compiled reference, tests/golden lg_*)
+ *   orc_md_beeman_pd     PD's own MolecularDynamics runs with the Beeman
+ *                        integrator (compiled reference, tests/golden bm_*)
  */
 #include "pd_oracle.h"
 #include <math.h>
@@ -948,6 +956,186 @@ int orc_md_vv_pd(const orc_system *s, double *xyz, double *vel, const orc_params
                 vel[3 * i + c] += dv[c];
             }
         }
+        if (trace_epot) trace_epot[step] = e.epot;
+        if (trace_ekin) trace_ekin[step] = ekin;
+        if (!isfinite(e.epot)) { rc = -1; break; }
+    }
+    free(nl.row_start);
+    free(nl.entries);
+    free(f);
+    return rc;
+}
+
+/* ---- A11 Langevin integrator: protocols/md.cpp:906-1043 ---- */
+
+/* The coefficients of applyForces_LangevinIntegration for gamma*dt
+ * (md.cpp:950-985): exact exponential forms above 0.05, the reference's
+ * Taylor series in between. Returns 0 below 1e-5, where the reference runs
+ * its VelocityVerlet update instead. */
+int orc_langevin_coef(double gamma, double dt, double *c0, double *c1, double *c2, double *varv, double *varr,
+                      double *crv) {
+    const double gt = gamma * dt;
+    if (gt < 0.00001) return 0;
+    if (gt > 0.05) {
+        *c0 = exp(-gt);
+        *c1 = (1 - *c0) / gt;
+        *c2 = (1 - *c1) / gt;
+        const double eg = exp(-gt);
+        *varv = 1.0 - sqr(eg);
+        *varr = 2.0 * gt - 3.0 + (4.0 - eg) * eg;
+        *crv = sqr(1.0 - eg) / sqrt(*varv * *varr);
+        return 1;
+    }
+    const double g2 = gt * gt, g3 = gt * g2, g4 = g2 * g2, g5 = g2 * g3, g6 = g3 * g3, g7 = g3 * g4, g8 = g4 * g4,
+                 g9 = g4 * g5;
+    *c0 = 1 - gt + 0.5 * g2 - g3 / 6.0 + g4 / 24.0 - g5 / 120.0 + g6 / 720.0 - g7 / 5040.0 + g8 / 40320.0 -
+          g9 / 362880.0;
+    *c1 = 1 - (0.5 * gt - g2 / 6.0 + g3 / 24.0 - g4 / 120.0 + g5 / 720.0 - g6 / 5040.0 + g7 / 40320.0 -
+               g8 / 362880.0);
+    *c2 = 0.5 - gt / 6.0 + g2 / 24.0 - g3 / 120.0 + g4 / 720.0 - g5 / 5040.0 + g6 / 40320.0 - g7 / 362880.0;
+    *varr = 2.0 * g3 / 3.0 - g4 / 2.0 + 7.0 * g5 / 30.0 - g6 / 12.0 + 31.0 * g7 / 1260.0 - g8 / 160.0 +
+            127.0 * g9 / 90720.0;
+    *varv = 2.0 * gt - 2.0 * g2 + 4.0 * g3 / 3.0 - 2.0 * g4 / 3.0 + 4.0 * g5 / 15.0 - 4.0 * g6 / 45.0 +
+            8.0 * g7 / 315.0 - 2.0 * g8 / 315.0 + 4.0 * g9 / 2835.0;
+    *crv = sqrt(3.0) * (0.5 - 3.0 * gt / 16.0 - 17.0 * g2 / 1280.0 + 17.0 * g3 / 6144.0 + 40967.0 * g4 / 34406400.0 -
+                        57203.0 * g5 / 275251200.0 - 1429487.0 * g6 / 13212057600.0);
+    return 2;
+}
+
+/* The normal deviates the reference draws in a Langevin run: per step
+ * (0..steps-2) and atom, nrand(n1x, n2y), nrand(n2x, n1z), nrand(n1y, n2z)
+ * (md.cpp:1017-1019), each nrand being two frand() = (1 + rand()) /
+ * (RAND_MAX + 1) through Box-Muller at -2 pi (maths/maths.h:138-152,
+ * maths/maths.cpp:256-261), after srand(seed). Written per step and atom as
+ * (n1x, n1y, n1z, n2x, n2y, n2z); the last step's slots are zero. */
+void orc_langevin_noise(unsigned seed, int natoms, int steps, double *out) {
+    srand(seed);
+    const double two_pi = 2.0 * kPI;
+    for (int st = 0; st < steps; ++st) {
+        for (int i = 0; i < natoms; ++i) {
+            double *o = out + ((size_t)st * natoms + i) * 6;
+            if (st == steps - 1) {
+                for (int k = 0; k < 6; ++k) o[k] = 0.0;
+                continue;
+            }
+            double pr[6];
+            for (int q = 0; q < 3; ++q) {
+                const double z1 = (1.0 + (double)rand()) / ((double)RAND_MAX + 1.0);
+                const double z2 = (1.0 + (double)rand()) / ((double)RAND_MAX + 1.0);
+                const double l = sqrt(-2.0 * log(z1));
+                pr[2 * q] = l * cos(-two_pi * z2);
+                pr[2 * q + 1] = l * sin(-two_pi * z2);
+            }
+            /* call order: (n1x, n2y), (n2x, n1z), (n1y, n2z) */
+            o[0] = pr[0]; o[4] = pr[1];
+            o[3] = pr[2]; o[2] = pr[3];
+            o[1] = pr[4]; o[5] = pr[5];
+        }
+    }
+}
+
+/* MolecularDynamics::run_core with Integrator = Langevin and
+ * LangevinOnHydrogens (PD's default): per step, list refresh, forces, then
+ * (step > 0) v += (1 - c0/c1) f dt/m / (gamma dt); ekin; (step < steps-1)
+ * dr = dt (c1 v + c2 f dt/m) + sigma_r n1, v = c0 v + c0 c2 f dt/m / c1 +
+ * sigma_v (crv n1 + sqrt(1 - crv^2) n2) with sigma_v^2 = kB T/m varv and
+ * sigma_r^2 = kB T/m varr / gamma^2 (md.cpp:1003-1041). noise as written
+ * by orc_langevin_noise. Below gamma dt = 1e-5 the VelocityVerlet update. */
+int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p, int steps, double dt,
+                       int update_nlist, double temperature, double gamma, const double *noise,
+                       double *trace_epot, double *trace_ekin) {
+    double c0, c1, c2, varv, varr, crv;
+    if (!orc_langevin_coef(gamma, dt, &c0, &c1, &c2, &varv, &varr, &crv))
+        return orc_md_vv_pd(s, xyz, vel, p, steps, dt, update_nlist, trace_epot, trace_ekin);
+    const double gt = gamma * dt;
+    const int n = s->n;
+    orc_nlist nl;
+    nl.row_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    nl.cap = (int64_t)n * 64 + 1024;
+    nl.entries = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nl.cap);
+    double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    orc_energies e;
+    int rc = steps;
+    for (int step = 0; step < steps; ++step) {
+        if (update_nlist > 0 && step % update_nlist == 0) build_list(s, xyz, p, &nl);
+        eval_forces(s, xyz, &nl, p, f, &e);
+        if (step > 0)
+            for (int i = 0; i < n; ++i) {
+                const double tinvmass = dt / s->mass[i];
+                for (int c = 0; c < 3; ++c) vel[3 * i + c] += (1.0 - c0 / c1) * f[3 * i + c] * tinvmass / gt;
+            }
+        double ekin = 0;
+        for (int i = 0; i < n; ++i)
+            ekin += 0.5 * s->mass[i] * sqr(sqrt(sqr(vel[3 * i]) + sqr(vel[3 * i + 1]) + sqr(vel[3 * i + 2])));
+        if (step < steps - 1)
+            for (int i = 0; i < n; ++i) {
+                const double tinvmass = dt / s->mass[i];
+                const double kTdivm = kKB * temperature / s->mass[i];
+                const double sv = sqrt(kTdivm * varv), sr = sqrt(kTdivm * varr) / gamma;
+                const double *nz = noise + ((size_t)step * n + i) * 6;
+                for (int c = 0; c < 3; ++c) {
+                    const double n1 = nz[c], n2 = nz[3 + c];
+                    const double dlv = sv * (crv * n1 + sqrt(1.0 - sqr(crv)) * n2);
+                    const double dlr = sr * n1;
+                    double dr = dt * (c1 * vel[3 * i + c] + c2 * f[3 * i + c] * tinvmass) + dlr;
+                    dr *= 1 / kAngstrom;
+                    const double dv = (c0 * c2) * f[3 * i + c] * tinvmass / c1 + dlv;
+                    xyz[3 * i + c] += dr;
+                    vel[3 * i + c] *= c0;
+                    vel[3 * i + c] += dv;
+                }
+            }
+        if (trace_epot) trace_epot[step] = e.epot;
+        if (trace_ekin) trace_ekin[step] = ekin;
+        if (!isfinite(e.epot)) { rc = -1; break; }
+    }
+    free(nl.row_start);
+    free(nl.entries);
+    free(f);
+    return rc;
+}
+
+/* ---- A11 Beeman integrator: protocols/md.cpp:840-904 ---- */
+
+/* MolecularDynamics::run_core with Integrator = Beeman (PD's default
+ * integrator), no thermostat: per step, list refresh, forces, then
+ * (step > 0) v += 2 f dt/6m; ekin; (step < steps-1) p += dt (v + (4 f -
+ * f_old) dt/6m) / A, v += (5 f - f_old) dt/6m, f_old = f. f_old (N,
+ * natoms*3) is the workspace's old forces and is updated in place. */
+int orc_md_beeman_pd(const orc_system *s, double *xyz, double *vel, double *f_old, const orc_params *p, int steps,
+                     double dt, int update_nlist, double *trace_epot, double *trace_ekin) {
+    const int n = s->n;
+    orc_nlist nl;
+    nl.row_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    nl.cap = (int64_t)n * 64 + 1024;
+    nl.entries = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)nl.cap);
+    double *f = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    orc_energies e;
+    int rc = steps;
+    for (int step = 0; step < steps; ++step) {
+        if (update_nlist > 0 && step % update_nlist == 0) build_list(s, xyz, p, &nl);
+        eval_forces(s, xyz, &nl, p, f, &e);
+        if (step > 0)
+            for (int i = 0; i < n; ++i) {
+                const double tinvmass = dt / (6.0 * s->mass[i]);
+                for (int c = 0; c < 3; ++c) vel[3 * i + c] += (2.0 * f[3 * i + c]) * tinvmass;
+            }
+        double ekin = 0;
+        for (int i = 0; i < n; ++i)
+            ekin += 0.5 * s->mass[i] * sqr(sqrt(sqr(vel[3 * i]) + sqr(vel[3 * i + 1]) + sqr(vel[3 * i + 2])));
+        if (step < steps - 1)
+            for (int i = 0; i < n; ++i) {
+                const double tinvmass = dt / (6.0 * s->mass[i]);
+                for (int c = 0; c < 3; ++c) {
+                    const double fc = f[3 * i + c], fo = f_old[3 * i + c];
+                    double dr = dt * (vel[3 * i + c] + (4.0 * fc - fo) * tinvmass);
+                    dr *= 1 / kAngstrom;
+                    const double dv = (5.0 * fc - fo) * tinvmass;
+                    xyz[3 * i + c] += dr;
+                    vel[3 * i + c] += dv;
+                    f_old[3 * i + c] = fc;
+                }
+            }
         if (trace_epot) trace_epot[step] = e.epot;
         if (trace_ekin) trace_ekin[step] = ekin;
         if (!isfinite(e.epot)) { rc = -1; break; }

@@ -179,6 +179,28 @@ int orc_md_vv(const orc_system *s, double *xyz, double *vel, const orc_params *p
 int orc_md_vv_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p,
                  int steps, double dt, int update_nlist, double *trace_epot, double *trace_ekin);
 
+/* A11 Langevin (protocols/md.cpp:906-1043): coefficients for gamma [1/s]
+ * and dt [s]; returns 0 (VelocityVerlet), 1 (exponential form) or 2
+ * (Taylor form). */
+int orc_langevin_coef(double gamma, double dt, double *c0, double *c1, double *c2, double *varv, double *varr,
+                      double *crv);
+
+/* The reference's normal deviates after srand(seed): [steps][natoms][6] as
+ * (n1x, n1y, n1z, n2x, n2y, n2z), last step zero. */
+void orc_langevin_noise(unsigned seed, int natoms, int steps, double *out);
+
+/* MolecularDynamics::run_core, Integrator = Langevin (LangevinOnHydrogens),
+ * constant target temperature [K], friction gamma [1/s], noise as from
+ * orc_langevin_noise. */
+int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p, int steps, double dt,
+                       int update_nlist, double temperature, double gamma, const double *noise,
+                       double *trace_epot, double *trace_ekin);
+
+/* MolecularDynamics::run_core, Integrator = Beeman, no thermostat; f_old
+ * (N, natoms*3) in/out. */
+int orc_md_beeman_pd(const orc_system *s, double *xyz, double *vel, double *f_old, const orc_params *p, int steps,
+                     double dt, int update_nlist, double *trace_epot, double *trace_ekin);
+
 #ifdef __cplusplus
 }
 #endif

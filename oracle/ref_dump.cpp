@@ -449,71 +449,98 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
         r.s("svsd_done", (double)done);
     }
     if (c.md_steps > 0) {
-        // PD's MolecularDynamics with the VelocityVerlet integrator on
-        // FF_Bonded + GB, after 100 steepest-descent steps (the built chain
-        // can hold clashes), from velocities drawn from a SplitMix64 stream
-        // (uniform in +-sqrt(3 kB T / m) per component, T = 300 K)
-        System sys(ffps);
-        sys.add(NewProtein(ffps, pd_grammar(c.seq)));
-        WorkSpace ws(sys);
-        ws.nlist().setPadding(c.padding);
-        Forcefield ff(ws);
-        FF_Bonded bonds(ws);
-        GBProbe gb(ws);
-        configure(gb, c);
-        ff.add(bonds);
-        ff.add(gb);
-        ff.printEnergySummary();
-        {
-            Protocol::Minimisation mini(ff);
-            mini.Algorithm = Protocol::Minimisation::SteepestDescent;
-            mini.Steps = 100;
-            mini.UpdateScr = 0;
-            mini.OutputLevel = Verbosity::Silent;
-            mini.run();
-        }
-        const int n = (int)ws.atom.size();
-        std::vector<double> v0, x0;
-        uint64_t z = 0x5EEDULL;
-        for (int i = 0; i < n; ++i) {
-            const double vmax = sqrt(3.0 * PhysicsConst::kB * 300.0 / ws.atom[i].mass);
-            double u[3];
-            for (int k = 0; k < 3; ++k) {
-                z += 0x9E3779B97F4A7C15ULL;
-                uint64_t w = z;
-                w = (w ^ (w >> 30)) * 0xBF58476D1CE4E5B9ULL;
-                w = (w ^ (w >> 27)) * 0x94D049BB133111EBULL;
-                w ^= w >> 31;
-                u[k] = (2.0 * ((double)(w >> 11) * (1.0 / 9007199254740992.0)) - 1.0) * vmax;
+        // PD's MolecularDynamics on FF_Bonded + GB, after 100 steepest-descent
+        // steps (the built chain can hold clashes), from velocities drawn from
+        // a SplitMix64 stream (uniform in +-sqrt(3 kB T / m) per component,
+        // T = 300 K). One run per integrator from the same start: md_*
+        // VelocityVerlet, lg_* Langevin (PD's defaults: 10/ps friction, 300 K,
+        // LangevinOnHydrogens; libc rand() after srand(lg_seed)), bm_* Beeman
+        // (the workspace's old forces zeroed first).
+        for (int integ = 0; integ < 3; ++integ) {
+            System sys(ffps);
+            sys.add(NewProtein(ffps, pd_grammar(c.seq)));
+            WorkSpace ws(sys);
+            ws.nlist().setPadding(c.padding);
+            Forcefield ff(ws);
+            FF_Bonded bonds(ws);
+            GBProbe gb(ws);
+            configure(gb, c);
+            ff.add(bonds);
+            ff.add(gb);
+            ff.printEnergySummary();
+            {
+                Protocol::Minimisation mini(ff);
+                mini.Algorithm = Protocol::Minimisation::SteepestDescent;
+                mini.Steps = 100;
+                mini.UpdateScr = 0;
+                mini.OutputLevel = Verbosity::Silent;
+                mini.run();
             }
-            ws.cur.atom[i].v.setTo(u[0], u[1], u[2]);
-            v0.push_back(u[0]); v0.push_back(u[1]); v0.push_back(u[2]);
-            x0.push_back(ws.cur.atom[i].p.x); x0.push_back(ws.cur.atom[i].p.y); x0.push_back(ws.cur.atom[i].p.z);
+            const int n = (int)ws.atom.size();
+            std::vector<double> v0, x0;
+            uint64_t z = 0x5EEDULL;
+            for (int i = 0; i < n; ++i) {
+                const double vmax = sqrt(3.0 * PhysicsConst::kB * 300.0 / ws.atom[i].mass);
+                double u[3];
+                for (int k = 0; k < 3; ++k) {
+                    z += 0x9E3779B97F4A7C15ULL;
+                    uint64_t w = z;
+                    w = (w ^ (w >> 30)) * 0xBF58476D1CE4E5B9ULL;
+                    w = (w ^ (w >> 27)) * 0x94D049BB133111EBULL;
+                    w ^= w >> 31;
+                    u[k] = (2.0 * ((double)(w >> 11) * (1.0 / 9007199254740992.0)) - 1.0) * vmax;
+                }
+                ws.cur.atom[i].v.setTo(u[0], u[1], u[2]);
+                v0.push_back(u[0]); v0.push_back(u[1]); v0.push_back(u[2]);
+                x0.push_back(ws.cur.atom[i].p.x); x0.push_back(ws.cur.atom[i].p.y); x0.push_back(ws.cur.atom[i].p.z);
+            }
+            Protocol::MolecularDynamics md(ff);
+            const char *tag = "md";
+            md.Integrator = Protocol::MolecularDynamics::VelocityVerlet;
+            const unsigned lg_seed = 1234u;
+            if (integ == 1) {
+                tag = "lg";
+                md.Integrator = Protocol::MolecularDynamics::Langevin;
+            } else if (integ == 2) {
+                tag = "bm";
+                md.Integrator = Protocol::MolecularDynamics::Beeman;
+                for (int i = 0; i < n; ++i) ws.old.atom[i].f.setTo(0, 0, 0);
+            }
+            md.Steps = c.md_steps;
+            md.Timestep = 1e-15;
+            md.RandVel = false;
+            md.UpdateNList = 10;
+            md.UpdateScr = 0;
+            md.UpdateTra = 0;
+            md.OutputLevel = Verbosity::Silent;
+            if (integ == 1) srand(lg_seed);
+            md.run();
+            std::vector<double> xyz, vel;
+            for (int i = 0; i < n; ++i) {
+                xyz.push_back(ws.cur.atom[i].p.x); xyz.push_back(ws.cur.atom[i].p.y); xyz.push_back(ws.cur.atom[i].p.z);
+                vel.push_back(ws.cur.atom[i].v.x); vel.push_back(ws.cur.atom[i].v.y); vel.push_back(ws.cur.atom[i].v.z);
+            }
+            const std::string t(tag);
+            if (integ == 0) {
+                r.d("md_xyz0", x0);
+                r.d("md_v0", v0);
+                r.s("md_steps", (double)c.md_steps);
+                r.s("md_dt", md.Timestep);
+                r.s("md_update_nlist", (double)md.UpdateNList);
+            } else {
+                // the same start as the VelocityVerlet run (checked by the tests)
+                r.d(t + "_xyz0", x0);
+            }
+            r.d(t + "_xyz", xyz);
+            r.d(t + "_v", vel);
+            r.s(t + "_epot", ws.ene.epot);
+            r.s(t + "_ekin", ws.ene.ekin);
+            if (integ == 1) {
+                r.s("lg_seed", (double)lg_seed);
+                r.s("lg_temp", 300.0);
+                r.s("lg_fric", md.FricCoeff);
+            }
         }
-        Protocol::MolecularDynamics md(ff);
-        md.Integrator = Protocol::MolecularDynamics::VelocityVerlet;
-        md.Steps = c.md_steps;
-        md.Timestep = 1e-15;
-        md.RandVel = false;
-        md.UpdateNList = 10;
-        md.UpdateScr = 0;
-        md.UpdateTra = 0;
-        md.OutputLevel = Verbosity::Silent;
-        md.run();
-        std::vector<double> xyz, vel;
-        for (int i = 0; i < n; ++i) {
-            xyz.push_back(ws.cur.atom[i].p.x); xyz.push_back(ws.cur.atom[i].p.y); xyz.push_back(ws.cur.atom[i].p.z);
-            vel.push_back(ws.cur.atom[i].v.x); vel.push_back(ws.cur.atom[i].v.y); vel.push_back(ws.cur.atom[i].v.z);
-        }
-        r.d("md_xyz0", x0);
-        r.d("md_v0", v0);
-        r.d("md_xyz", xyz);
-        r.d("md_v", vel);
-        r.s("md_epot", ws.ene.epot);
-        r.s("md_ekin", ws.ene.ekin);
-        r.s("md_steps", (double)c.md_steps);
-        r.s("md_dt", md.Timestep);
-        r.s("md_update_nlist", (double)md.UpdateNList);
     }
 }
 

@@ -23,6 +23,7 @@ constexpr double kKcal2J = 4184.0;
 constexpr double kNa = 6.0221418E23;
 constexpr double kAngstrom = 1E-10;
 constexpr double kPI = 3.14159265358979323846;
+constexpr double kBoltzmann = 1.38065E-23;        // PhysicsConst::kB, J/K (forcefields/forcefield.cpp:18-38)
 const double kJPerKcal = kKcal2J / kNa;          // J/molecule per kcal/mol
 const double kForceToN = kJPerKcal / kAngstrom;  // N per kcal/mol/A
 
@@ -124,6 +125,8 @@ struct pdgpu_ctx {
     DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, row_count, row_off, tiles, tile_slot;
     // per instance
     DevBuf any_need, inst_need, inst_moved, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
+    DevBuf fox, foy, foz; // Beeman's old forces (N), per atom
+    DevBuf rng_ctr;       // Langevin device-noise step counter
     DevBuf band, sparse;
     DevBuf plist, tile_cnt, overflow;
     DevBuf cand, cand_n, inst_cslot; // candidate pairs of the last rebuild, per tile slot
@@ -258,7 +261,7 @@ int grow_atoms(pdgpu_ctx *c, int64_t blocks) {
     const size_t na = (size_t)blocks * kBlk;
     DevBuf *d8[] = {&c->px, &c->py, &c->pz, &c->q, &c->t123, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy,
                     &c->acc_fz, &c->fx, &c->fy, &c->fz, &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv,
-                    &c->opx, &c->opy, &c->opz, &c->rx, &c->ry, &c->rz};
+                    &c->opx, &c->opy, &c->opz, &c->rx, &c->ry, &c->rz, &c->fox, &c->foy, &c->foz};
     for (DevBuf *b : d8) TRY(resize(c, *b, na, 8));
     TRY(resize(c, c->lj64, na, 16));
     TRY(resize(c, c->gb64, na, 16));
@@ -829,6 +832,8 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     if (o->precision != PDGPU_FP64 && o->precision != PDGPU_MIXED) return PDGPU_ERR_INVALID;
     if ((o->components & ~(PDGPU_FF_ALL | PDGPU_FF_SOFTVDW)) != 0) return PDGPU_ERR_INVALID;
     if ((o->components & PDGPU_FF_GB) && (o->components & PDGPU_FF_SOFTVDW)) return PDGPU_ERR_UNSUPPORTED;
+    // instance slots are stored in 24-bit fields (term records, noise counters)
+    if (o->max_instances >= (1 << 24)) return PDGPU_ERR_INVALID;
     if (!(o->cutoff > 0) || !(o->vdw_cutoff > 0) || o->inner_cutoff > o->cutoff ||
         o->vdw_inner_cutoff > o->vdw_cutoff || !(o->p5 > 0) || !(o->dielectric_solute > 0) ||
         !(o->dielectric_solvent > 0) || o->nlist_padding < 0)
@@ -875,6 +880,7 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     if (rc == PDGPU_OK) rc = resize(c, c->tiles, 4096, 8);
     if (rc == PDGPU_OK) rc = resize(c, c->tile_slot, 4096, 4);
     if (rc == PDGPU_OK) rc = resize(c, c->overflow, 1, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->rng_ctr, 1, 8);
     if (rc != PDGPU_OK) {
         pdgpu_ctx_destroy(c);
         return rc;
@@ -904,7 +910,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
                      &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
                      &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot, &c->rx, &c->ry, &c->rz,
-                     &c->inst_moved, &c->any_need};
+                     &c->inst_moved, &c->any_need, &c->fox, &c->foy, &c->foz, &c->rng_ctr};
     for (DevBuf *b : all) release(*b);
     for (auto &g : c->graphs)
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
@@ -1044,7 +1050,7 @@ extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t
         HIPCHECK(c, hipMemcpyAsync((float2 *)c->gb32.p + a0, gb32.data(), na * 8, hipMemcpyHostToDevice, s));
     }
     std::vector<double> zero(na, 0.0);
-    DevBuf *zs[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz};
+    DevBuf *zs[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->fox, &c->foy, &c->foz};
     for (DevBuf *b : zs) HIPCHECK(c, hipMemcpyAsync((double *)b->p + a0, zero.data(), na * 8, hipMemcpyHostToDevice, s));
     // per block
     std::vector<int> bi(nb, slot), bn(nb), bf(nb, (int)b0);

@@ -261,6 +261,164 @@ __global__ void k_finalize_kick(DevState S, InstMD *md, const double *minv, doub
     }
 }
 
+// ------------------------------------------ Langevin (protocols/md.cpp:906-1043)
+// Per-call constants of PD's Langevin update (host-computed with the
+// reference's formulas, md.cpp:950-985).
+struct LangevinCoef {
+    double c0, c1, c2; // velocity decay and force weights
+    double varv, varr; // variances of the velocity and position increments, in units of kB T / m
+    double crv, scrv;  // their correlation and sqrt(1 - crv^2)
+    double gamma, gt;  // friction [1/s] and gamma * dt
+    double kT;         // kB * T [J]
+};
+
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ten rounds of
+// two 32x32->64 multiplies with the published multipliers and Weyl key
+// increments.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k) {
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k.x += 0x9E3779B9u;
+            k.y += 0xBB67AE85u;
+        }
+        const unsigned lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+        const unsigned lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    }
+    return c;
+}
+
+// 53-bit uniform in (0, 1], as PD's frand() excludes 0 (maths/maths.cpp:256)
+__device__ __forceinline__ double uniform53(unsigned hi, unsigned lo) {
+    return ((double)((((unsigned long long)hi << 32) | lo) >> 11) + 1.0) * 0x1p-53;
+}
+
+// The six standard normal deviates of one atom and step, (n1x, n1y, n1z,
+// n2x, n2y, n2z), by Box-Muller as PD's nrand (maths/maths.h:138-152) over
+// the counter (atom, instance | pair << 24, step) and key seed.
+__device__ __forceinline__ void langevin_normals(uint2 key, unsigned atom, unsigned slot, unsigned long long step,
+                                                 double n[6]) {
+    for (int q = 0; q < 3; ++q) {
+        const uint4 r = philox4x32_10(make_uint4(atom, slot | ((unsigned)q << 24), (unsigned)step,
+                                                 (unsigned)(step >> 32)),
+                                      key);
+        const double u1 = uniform53(r.x, r.y), u2 = uniform53(r.z, r.w);
+        const double l = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincospi(2.0 * u2, &sn, &cs);
+        n[q] = l * cs;
+        n[3 + q] = l * sn;
+    }
+}
+
+// One Langevin update (LangevinOnHydrogens), after the step's forces: with
+// first == 0 the velocity is finished with the new forces, v += (1 -
+// c0/c1) f dt/m / (gamma dt); the kinetic energy is taken; with last == 0
+// positions and velocities advance: dr = dt (c1 v + c2 f dt/m) + sigma_r
+// n1, v = c0 v + c0 c2 f dt/m / c1 + sigma_v (crv n1 + sqrt(1 - crv^2) n2)
+// (md.cpp:998-1041). noise, when given, holds the step's deviates per
+// atom of the instance as (n1x, n1y, n1z, n2x, n2y, n2z); otherwise they
+// come from langevin_normals at the step counter *ctr.
+__global__ void k_langevin_pd(DevState S, InstMD *md, const double *minv, double *vx, double *vy, double *vz,
+                              double *px, double *py, double *pz, int atom_begin, int atom_end, double dt,
+                              double f_to_n, LangevinCoef L, int first, int last, int want_ekin,
+                              const double *noise, const unsigned long long *ctr, uint2 key) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = a / kBlk;
+    const bool in = a < atom_end;
+    const int inst = in ? S.blk_inst[b] : -1;
+    const bool v = in && inst >= 0 && (a % kBlk) < S.blk_n[b];
+    double ek = 0;
+    if (v) {
+        const double im = minv[a], tm = dt * im;
+        const double f[3] = {S.fx[a] * f_to_n, S.fy[a] * f_to_n, S.fz[a] * f_to_n};
+        double u[3] = {vx[a], vy[a], vz[a]};
+        if (!first)
+            for (int c = 0; c < 3; ++c) u[c] += (1.0 - L.c0 / L.c1) * f[c] * tm / L.gt;
+        if (im > 0) {
+            const double m = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            ek = 0.5 / im * (m * m);
+        }
+        if (!last) {
+            const int local = a - S.blk_first[b] * kBlk;
+            double n[6];
+            if (noise) {
+                for (int k = 0; k < 6; ++k) n[k] = noise[(size_t)local * 6 + k];
+            } else {
+                langevin_normals(key, (unsigned)local, (unsigned)inst, *ctr, n);
+            }
+            const double kTdivm = L.kT * im;
+            const double sv = sqrt(kTdivm * L.varv), sr = sqrt(kTdivm * L.varr) / L.gamma;
+            double p[3] = {px[a], py[a], pz[a]};
+            for (int c = 0; c < 3; ++c) {
+                const double n1 = n[c], n2 = n[3 + c];
+                const double dlv = sv * (L.crv * n1 + L.scrv * n2), dlr = sr * n1;
+                double dr = dt * (L.c1 * u[c] + L.c2 * f[c] * tm) + dlr;
+                dr *= 1.0 / 1e-10;
+                const double dv = (L.c0 * L.c2) * f[c] * tm / L.c1 + dlv;
+                p[c] += dr;
+                u[c] *= L.c0;
+                u[c] += dv;
+            }
+            px[a] = p[0]; py[a] = p[1]; pz[a] = p[2];
+        }
+        vx[a] = u[0]; vy[a] = u[1]; vz[a] = u[2];
+    }
+    if (want_ekin) {
+        ek = wave_sum(ek);
+        if (lane_id() == 0 && in && inst >= 0 && ek != 0.0) unsafeAtomicAdd(&md[inst].ekin, ek);
+    }
+}
+
+__global__ void k_counter_step(unsigned long long *ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr += 1;
+}
+
+// ---------------------------------------------- Beeman (protocols/md.cpp:840-904)
+// One Beeman update after the step's forces: with first == 0 v += 2 f
+// dt/6m; the kinetic energy is taken; with last == 0 p += dt (v + (4 f -
+// f_old) dt/6m) / A, v += (5 f - f_old) dt/6m and f_old = f. f_old (N) is
+// the workspace's old forces, kept per atom between calls.
+__global__ void k_beeman_pd(DevState S, InstMD *md, const double *minv, double *vx, double *vy, double *vz,
+                            double *px, double *py, double *pz, double *ox, double *oy, double *oz, int atom_begin,
+                            int atom_end, double dt, double f_to_n, int first, int last, int want_ekin) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = a / kBlk;
+    const bool in = a < atom_end;
+    const int inst = in ? S.blk_inst[b] : -1;
+    const bool v = in && inst >= 0 && (a % kBlk) < S.blk_n[b];
+    double ek = 0;
+    if (v) {
+        const double im = minv[a], tm = dt * im / 6.0;
+        const double f[3] = {S.fx[a] * f_to_n, S.fy[a] * f_to_n, S.fz[a] * f_to_n};
+        double u[3] = {vx[a], vy[a], vz[a]};
+        if (!first)
+            for (int c = 0; c < 3; ++c) u[c] += (2.0 * f[c]) * tm;
+        if (im > 0) {
+            const double m = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            ek = 0.5 / im * (m * m);
+        }
+        if (!last) {
+            const double fo[3] = {ox[a], oy[a], oz[a]};
+            double p[3] = {px[a], py[a], pz[a]};
+            for (int c = 0; c < 3; ++c) {
+                double dr = dt * (u[c] + (4.0 * f[c] - fo[c]) * tm);
+                dr *= 1.0 / 1e-10;
+                p[c] += dr;
+                u[c] += (5.0 * f[c] - fo[c]) * tm;
+            }
+            px[a] = p[0]; py[a] = p[1]; pz[a] = p[2];
+            ox[a] = f[0]; oy[a] = f[1]; oz[a] = f[2];
+        }
+        vx[a] = u[0]; vy[a] = u[1]; vz[a] = u[2];
+    }
+    if (want_ekin) {
+        ek = wave_sum(ek);
+        if (lane_id() == 0 && in && inst >= 0 && ek != 0.0) unsafeAtomicAdd(&md[inst].ekin, ek);
+    }
+}
+
 __global__ void k_ekin_clear(InstMD *md, int s0, int s1) {
     const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (s < s1) md[s].ekin = 0;

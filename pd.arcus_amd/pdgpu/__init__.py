@@ -46,7 +46,8 @@ EXPORTS = (
     "pdgpu_minimise_sd", "pdgpu_set_velocities", "pdgpu_get_velocities", "pdgpu_md_vv",
     "pdgpu_bondorder_build", "pdgpu_fixed_terms", "pdgpu_instance_bytes", "pdgpu_bytes_free",
     "pdgpu_profile", "pdgpu_profile_read", "pdgpu_sd_start", "pdgpu_sd_advance", "pdgpu_sd_finish",
-    "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune", "pdgpu_md_vv_pd",
+    "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune", "pdgpu_md_vv_pd", "pdgpu_md_langevin_pd",
+    "pdgpu_md_beeman_pd",
 )
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
@@ -133,6 +134,10 @@ def _load():
         "pdgpu_minimise_sd": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double, _ip]),
         "pdgpu_md_vv": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, P(Energies), _dp]),
         "pdgpu_md_vv_pd": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, P(Energies), _dp]),
+        "pdgpu_md_langevin_pd": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double,
+                                           C.c_double, C.c_uint64, _dp, P(Energies), _dp]),
+        "pdgpu_md_beeman_pd": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, P(Energies),
+                                         _dp]),
         "pdgpu_bondorder_build": (C.c_int, [C.c_int32, _ip, C.c_int32, _ip, _ip, C.c_int32, _ip, _ip, C.c_int32,
                                             _ip, _ip, _ip, _bp, _ip, _ip, _ip, _ip]),
         "pdgpu_fixed_terms": (C.c_int, [C.c_int32, _dp, C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _ip, _dp,
@@ -435,6 +440,39 @@ class Context:
         self._check(lib.pdgpu_md_vv_pd(self.h, iid, steps, dt, update_nlist, C.byref(e), _p(ek, _dp)), "md_vv_pd")
         return e.as_dict(), float(ek[0])
 
+
+    def _md_call(self, name, fn, iid):
+        if iid == PDGPU_ALL:
+            n = self.opt.max_instances if self.opt.max_instances > 0 else 8192
+            arr = (Energies * n)()
+            ek = np.zeros(n)
+            self._check(fn(arr, _p(ek, _dp)), name)
+            return {i: arr[i].as_dict() for i in self._natoms}, {i: float(ek[i]) for i in self._natoms}
+        e = Energies()
+        ek = np.zeros(1)
+        self._check(fn(C.byref(e), _p(ek, _dp)), name)
+        return e.as_dict(), float(ek[0])
+
+    def md_langevin_pd(self, iid, steps, dt, temperature=300.0, friction=1e13, seed=1, noise=None,
+                       update_nlist=10):
+        """PD's MolecularDynamics with its Langevin integrator
+        (protocols/md.cpp:906-1043). noise: None (device Philox stream keyed
+        by seed) or [steps][natoms][6] deviates (n1x, n1y, n1z, n2x, n2y,
+        n2z) for a single instance. Returns like md_vv."""
+        nz = None
+        if noise is not None:
+            nz = np.ascontiguousarray(noise, dtype=np.float64)
+            if iid == PDGPU_ALL or nz.size != steps * self._natoms[iid] * 6:
+                raise ValueError("noise must be [steps][natoms][6] for one instance")
+        return self._md_call("md_langevin_pd", lambda e, k: lib.pdgpu_md_langevin_pd(
+            self.h, iid, steps, dt, update_nlist, temperature, friction, seed,
+            _p(nz, _dp) if nz is not None else None, e, k), iid)
+
+    def md_beeman_pd(self, iid, steps, dt, update_nlist=10):
+        """PD's MolecularDynamics with its Beeman integrator
+        (protocols/md.cpp:840-904). Returns like md_vv."""
+        return self._md_call("md_beeman_pd", lambda e, k: lib.pdgpu_md_beeman_pd(
+            self.h, iid, steps, dt, update_nlist, e, k), iid)
     def profile(self, enable=True):
         self._check(lib.pdgpu_profile(self.h, int(enable)), "profile")
 

@@ -91,6 +91,16 @@ lib.orc_softvdw_forces.argtypes = [C.POINTER(System), _dp, C.POINTER(NList), C.c
 lib.orc_md_vv_pd.restype = C.c_int
 lib.orc_md_vv_pd.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                              _dp, _dp]
+lib.orc_langevin_coef.restype = C.c_int
+lib.orc_langevin_coef.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
+lib.orc_langevin_noise.restype = None
+lib.orc_langevin_noise.argtypes = [C.c_uint, C.c_int, C.c_int, _dp]
+lib.orc_md_langevin_pd.restype = C.c_int
+lib.orc_md_langevin_pd.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
+                                   C.c_double, C.c_double, _dp, _dp, _dp]
+lib.orc_md_beeman_pd.restype = C.c_int
+lib.orc_md_beeman_pd.argtypes = [C.POINTER(System), _dp, _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
+                                 _dp, _dp]
 lib.orc_md_vv.restype = C.c_int
 lib.orc_md_vv.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                           _dp, _dp]
@@ -289,6 +299,44 @@ def md_vv_pd(sysm, xyz, vel, params, steps, dt, update_nlist=10):
     rc = lib.orc_md_vv_pd(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), C.byref(params), steps, dt,
                           update_nlist, _ptr(te, _dp), _ptr(tk, _dp))
     return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk
+
+
+def langevin_coef(gamma, dt):
+    """(branch, c0, c1, c2, varv, varr, crv) of PD's Langevin update."""
+    v = [C.c_double() for _ in range(6)]
+    br = lib.orc_langevin_coef(gamma, dt, *[C.byref(x) for x in v])
+    return (br,) + tuple(x.value for x in v)
+
+
+def langevin_noise(seed, natoms, steps):
+    """The reference's normal deviates after srand(seed), [steps][natoms][6]."""
+    out = np.zeros((steps, natoms, 6))
+    lib.orc_langevin_noise(seed, natoms, steps, _ptr(out, _dp))
+    return out
+
+
+def md_langevin_pd(sysm, xyz, vel, params, steps, dt, temperature, gamma, noise, update_nlist=10):
+    """PD's MolecularDynamics with its Langevin integrator."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float64).copy()
+    noise = np.ascontiguousarray(noise, dtype=np.float64)
+    te = np.zeros(steps)
+    tk = np.zeros(steps)
+    rc = lib.orc_md_langevin_pd(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), C.byref(params), steps, dt,
+                                update_nlist, temperature, gamma, _ptr(noise, _dp), _ptr(te, _dp), _ptr(tk, _dp))
+    return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk
+
+
+def md_beeman_pd(sysm, xyz, vel, params, steps, dt, f_old=None, update_nlist=10):
+    """PD's MolecularDynamics with its Beeman integrator; f_old (N) in/out."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float64).copy()
+    fo = np.zeros(xyz.size) if f_old is None else np.ascontiguousarray(f_old, dtype=np.float64).ravel().copy()
+    te = np.zeros(steps)
+    tk = np.zeros(steps)
+    rc = lib.orc_md_beeman_pd(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), _ptr(fo, _dp), C.byref(params),
+                              steps, dt, update_nlist, _ptr(te, _dp), _ptr(tk, _dp))
+    return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), fo.reshape(-1, 3), te, tk
 
 
 def softvdw_forces(sysm, xyz, rs, ent, hardness=12.0, olap=0.6, vdw_cutoff=15.0):

@@ -1,0 +1,159 @@
+"""The oracle's Langevin and Beeman restatements (protocols/md.cpp:840-1043)
+pinned against the compiled reference's MolecularDynamics runs (FF_Bonded +
+GB, golden lg_* / bm_*), and the Langevin coefficients against what the
+mathematics fixes: the reference's Taylor series against the exact
+coefficients of the Ornstein-Uhlenbeck update (60-digit arithmetic), its
+exponential branch, and the gamma*dt -> 0 limits. One of PD's series, the
+velocity-position correlation crv, is off at first order (low by gt/4); the
+test pins that size."""
+import decimal
+import math
+
+import numpy as np
+import pytest
+
+import golden_io as G
+import oracle_lib as O
+
+CASES = G.case_names()
+LG_CASES = [n for n in CASES if "lg_xyz" in G.load(n)]
+BM_CASES = [n for n in CASES if "bm_xyz" in G.load(n)]
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def _system(d):
+    return O.OSystem(d["charge"], d["radius"], d["epsilon"], d["gb_radius"], d["vj"], d["terms123"], d["mass"],
+                     d["mid"], d["band"], d["sp_i"], d["sp_j"], d["sp_o"], bonded=O.OBonded(d))
+
+
+def exact_coef(gt):
+    """c0, c1, c2, varv, varr, crv of the exact Langevin update, evaluated
+    in 60-digit decimal arithmetic (the exponential forms cancel badly in
+    double precision for small gamma*dt)."""
+    with decimal.localcontext() as ctx:
+        ctx.prec = 60
+        g = decimal.Decimal(repr(gt))
+        e = (-g).exp()
+        c0 = e
+        c1 = (1 - e) / g
+        c2 = (1 - c1) / g
+        varv = 1 - e * e
+        varr = 2 * g - 3 + (4 - e) * e
+        crv = (1 - e) ** 2 / (varv * varr).sqrt()
+        return tuple(float(x) for x in (c0, c1, c2, varv, varr, crv))
+
+
+@pytest.mark.parametrize("gt", [0.001, 0.01, 0.03, 0.049])
+def test_taylor_coefficients_match_the_exponential_forms(gt):
+    br, *tay = O.langevin_coef(gt / 1e-15, 1e-15)
+    assert br == 2
+    ex = exact_coef(gt)
+    for a, b, name in zip(tay[:5], ex[:5], ("c0", "c1", "c2", "varv", "varr")):
+        # the series stop at gt^9 (c0, varv, varr), gt^8 (c1), gt^7 (c2)
+        assert rel(a, b) <= 1e-12, name
+    # PD's series for the velocity-position correlation starts
+    # sqrt(3) (1/2 - 3 gt/16) where the exact expansion is sqrt(3) (1/2 -
+    # gt/16) (md.cpp:981): its crv is low by gt/4 relative. The oracle and
+    # the device keep PD's series.
+    dev = (ex[5] - tay[5]) / ex[5]
+    assert abs(dev / (gt / 4) - 1) < 0.01
+
+
+@pytest.mark.parametrize("gt", [0.06, 0.5, 3.0])
+def test_exponential_branch(gt):
+    br, *c = O.langevin_coef(gt / 1e-15, 1e-15)
+    assert br == 1
+    for a, b in zip(c, exact_coef(gt)):
+        # varr = 2gt - 3 + (4 - e)e cancels to O(gt^3) in double precision
+        assert rel(a, b) <= 1e-11
+
+
+def test_small_friction_is_velocity_verlet():
+    assert O.langevin_coef(1e9, 1e-15)[0] == 0
+
+
+def test_coefficients_small_friction_limit():
+    br, c0, c1, c2, varv, varr, crv = O.langevin_coef(1e9 * 10, 1e-15)  # gamma dt = 1e-5
+    assert br == 2
+    assert abs(c0 - 1) < 2e-5 and abs(c1 - 1) < 1e-5 and abs(c2 - 0.5) < 1e-5
+    assert abs(varv / 2e-5 - 1) < 1e-4  # varv ~ 2 gamma dt
+    assert abs(crv - math.sqrt(3) / 2) < 1e-5
+
+
+def test_noise_is_standard_normal_and_in_call_order():
+    n = O.langevin_noise(77, 500, 21)
+    assert np.all(n[-1] == 0)
+    x = n[:-1].ravel()
+    assert abs(x.mean()) < 0.01 and abs(x.std() - 1) < 0.01
+    # the first three Box-Muller pairs of the stream, re-drawn by hand
+    a = O.langevin_noise(77, 1, 2)[0, 0]
+    b = O.langevin_noise(77, 2, 2)[0, 0]
+    assert np.array_equal(a, b)  # atom 0 of step 0 is the same six numbers
+    c = O.langevin_noise(78, 1, 2)[0, 0]
+    assert not np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("name", LG_CASES)
+def test_pd_langevin_matches_reference(name):
+    """PD's MolecularDynamics with Integrator = Langevin (PD's defaults:
+    friction 10/ps, 300 K, LangevinOnHydrogens), from the same start, with
+    the reference's own libc rand() stream."""
+    d = G.load(name)
+    assert np.array_equal(d["lg_xyz0"], d["md_xyz0"])
+    s = _system(d)
+    p = O.default_params(**d["param_dict"])
+    steps = int(d["md_steps"])
+    noise = O.langevin_noise(int(d["lg_seed"]), len(d["mass"]), steps)
+    rc, x, v, te, tk = O.md_langevin_pd(s, d["md_xyz0"], d["md_v0"], p, steps, float(d["md_dt"]),
+                                        float(d["lg_temp"]), float(d["lg_fric"]), noise,
+                                        int(d["md_update_nlist"]))
+    assert rc == steps
+    assert np.abs(x - d["lg_xyz"]).max() <= 1e-9
+    assert np.abs(v - d["lg_v"]).max() <= 1e-9 * np.abs(d["lg_v"]).max()
+    assert rel(te[-1], d["lg_epot"]) <= 1e-10
+    assert rel(tk[-1], d["lg_ekin"]) <= 1e-10
+    # the run differs from VelocityVerlet: the noise and the friction act
+    assert np.abs(x - d["md_xyz"]).max() > 1e-4
+
+
+@pytest.mark.parametrize("name", BM_CASES)
+def test_pd_beeman_matches_reference(name):
+    """PD's MolecularDynamics with Integrator = Beeman (its default), old
+    forces zero at the start."""
+    d = G.load(name)
+    assert np.array_equal(d["bm_xyz0"], d["md_xyz0"])
+    s = _system(d)
+    p = O.default_params(**d["param_dict"])
+    steps = int(d["md_steps"])
+    rc, x, v, fo, te, tk = O.md_beeman_pd(s, d["md_xyz0"], d["md_v0"], p, steps, float(d["md_dt"]),
+                                          update_nlist=int(d["md_update_nlist"]))
+    assert rc == steps
+    assert np.abs(x - d["bm_xyz"]).max() <= 1e-9
+    assert np.abs(v - d["bm_v"]).max() <= 1e-9 * np.abs(d["bm_v"]).max()
+    assert rel(te[-1], d["bm_epot"]) <= 1e-10
+    assert rel(tk[-1], d["bm_ekin"]) <= 1e-10
+    assert np.any(fo != 0)
+
+
+def test_beeman_free_particle_and_constant_force():
+    """Closed forms: Beeman with a constant force F from f_old = F moves a
+    particle exactly as x = x0 + v0 t + F t^2 / 2m and v = v0 + F t / m
+    (the scheme is exact for constant acceleration)."""
+    # one isolated atom: no pair terms, no bonded terms -> zero force; a
+    # free particle then moves in a straight line
+    d = G.load("gly_ala")
+    s = O.OSystem(d["charge"][:1] * 0, d["radius"][:1], d["epsilon"][:1], d["gb_radius"][:1], d["vj"][:1],
+                  d["terms123"][:1], d["mass"][:1], 1, np.zeros(0, np.int8), np.zeros(0, np.int32),
+                  np.zeros(0, np.int32), np.zeros(0, np.int32))
+    p = O.default_params(**d["param_dict"])
+    x0 = np.zeros((1, 3))
+    v0 = np.array([[100.0, -50.0, 25.0]])  # m/s
+    steps, dt = 11, 1e-15
+    rc, x, v, fo, te, tk = O.md_beeman_pd(s, x0, v0, p, steps, dt)
+    assert rc == steps
+    # ten moves (the last step does not move), each dt * v / A
+    assert np.allclose(x, x0 + 10 * dt * v0 / 1e-10, rtol=1e-13, atol=0)
+    assert np.array_equal(v, v0)
