@@ -304,6 +304,34 @@ int pdgpu_md_langevin_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, i
 int pdgpu_md_beeman_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
                        pdgpu_energies *energies, double *ekin);
 
+/* Result hand-off formats. A function returning int64_t gives the byte
+ * count of its output (or a negative pdgpu_status) and writes only when cap
+ * is large enough (text outputs also get a terminating NUL, so they need
+ * cap > length).
+ *
+ * SnapShot::writeMIME (workspace/snapshot.cpp:316-348) of arrays: "!PSP
+ * name", PD's 6-bit coding (tools/io.cpp:79-163) of a 128-byte header
+ * (natoms, epot, four zero status words; PD leaves the remaining 100 bytes
+ * uninitialised, here they are zero) followed by p (A), v (m/s), f (N) per
+ * atom, then "!ENDPSP". */
+int64_t pdgpu_mime_encode(const double *xyz, const double *vel, const double *force, int32_t natoms,
+                          double epot, const char *name, char *out, int64_t cap);
+
+/* WorkSpace::save().writeMIME(name) (workspace/workspace.cpp:165-172) of
+ * an instance: its device positions, velocities, the forces and total
+ * energy of its last evaluation, gathered in one copy. */
+int64_t pdgpu_snapshot_mime(pdgpu_ctx *ctx, int32_t id, const char *name, char *out, int64_t cap);
+
+/* OutTra_NAMD's DCD trajectory (fileio/psfdcd.cpp:104-247): the 276-byte
+ * header write_DCD_header writes (its title blocks zero-padded where PD
+ * writes uninitialised stack bytes), and one append_DCD_step frame of an
+ * instance (x, y, z as float records), converted on the device and copied
+ * once. pdgpu_dcd_create / pdgpu_dcd_append do the same on a file. */
+int64_t pdgpu_dcd_header(int32_t natoms, uint8_t *out, int64_t cap);
+int64_t pdgpu_dcd_frame(pdgpu_ctx *ctx, int32_t id, uint8_t *out, int64_t cap);
+int pdgpu_dcd_create(const char *path, int32_t natoms);
+int pdgpu_dcd_append(pdgpu_ctx *ctx, int32_t id, const char *path);
+
 /* Grid-shape autotuner (no reference counterpart): times the tile kernels
  * of a force evaluation of id (or PDGPU_ALL) reps times at each of several
  * wavefronts-per-CU settings and keeps the fastest per kernel for later

@@ -22,6 +22,7 @@
 #include "forcefields/ffsoftvdw.h"
 #include "protocols/minimise.h"
 #include "protocols/md.h"
+#include "fileio/psfdcd.h"
 
 #include <stdint.h>
 #include <cstdio>
@@ -123,6 +124,17 @@ void dump_bonded_terms(Rec &r, const BondedProbe &b) {
 }
 
 void dump_forces(Rec &r, WorkSpace &ws, const std::string &tag);
+
+// The bytes of a file the reference wrote.
+std::vector<int8_t> slurp(const std::string &path) {
+    std::vector<int8_t> v;
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return v;
+    int ch;
+    while ((ch = fgetc(f)) != EOF) v.push_back((int8_t)ch);
+    fclose(f);
+    return v;
+}
 
 void dump_bonded_eval(Rec &r, WorkSpace &ws, const BondedProbe &b, const std::string &tag) {
     dump_forces(r, ws, tag + "_force");
@@ -539,6 +551,33 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
                 r.s("lg_seed", (double)lg_seed);
                 r.s("lg_temp", 300.0);
                 r.s("lg_fric", md.FricCoeff);
+            }
+            if (integ == 0) {
+                // PD's result hand-off formats on the final VelocityVerlet
+                // state: WorkSpace::save() as SnapShot MIME text
+                // (workspace/snapshot.cpp:316-348) after a force evaluation,
+                // and a NAMD DCD trajectory (fileio/psfdcd.cpp) of three
+                // frames: the start, the end, and the end shifted by 0.1 A
+                ff.calcForces();
+                const std::string stem = outdir + "/" + c.name + "_io";
+                SnapShot snap = ws.save();
+                snap.writeMIME(stem + ".mime", c.name, "w");
+                r.b("io_mime", slurp(stem + ".mime"));
+                dump_forces(r, ws, "io_force");
+                r.s("io_epot", ws.ene.epot);
+                std::vector<double> fx;
+                IO::OutTra_NAMD tra(stem, ws);
+                for (int frame = 0; frame < 3; ++frame) {
+                    for (int i = 0; i < n; ++i) {
+                        const double s0 = frame == 2 ? 0.1 : 0.0;
+                        if (frame == 0) ws.cur.atom[i].p.setTo(x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]);
+                        else ws.cur.atom[i].p.setTo(xyz[3 * i] + s0, xyz[3 * i + 1] - s0, xyz[3 * i + 2] + 2 * s0);
+                        fx.push_back(ws.cur.atom[i].p.x); fx.push_back(ws.cur.atom[i].p.y); fx.push_back(ws.cur.atom[i].p.z);
+                    }
+                    tra.append();
+                }
+                r.d("io_dcd_xyz", fx);
+                r.b("io_dcd", slurp(stem + ".dcd"));
             }
         }
     }

@@ -127,6 +127,7 @@ struct pdgpu_ctx {
     DevBuf any_need, inst_need, inst_moved, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf fox, foy, foz; // Beeman's old forces (N), per atom
     DevBuf rng_ctr;       // Langevin device-noise step counter
+    DevBuf stage;         // packing buffer of the output formats
     DevBuf band, sparse;
     DevBuf plist, tile_cnt, overflow;
     DevBuf cand, cand_n, inst_cslot; // candidate pairs of the last rebuild, per tile slot
@@ -910,7 +911,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
                      &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
                      &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot, &c->rx, &c->ry, &c->rz,
-                     &c->inst_moved, &c->any_need, &c->fox, &c->foy, &c->foz, &c->rng_ctr};
+                     &c->inst_moved, &c->any_need, &c->fox, &c->foy, &c->foz, &c->rng_ctr, &c->stage};
     for (DevBuf *b : all) release(*b);
     for (auto &g : c->graphs)
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
@@ -1419,3 +1420,5 @@ hipStream_t stream(pdgpu_ctx *c) { return c->stream; }
 }
 
 #include "pdgpu_integrate.inc"
+
+#include "pdgpu_io.inc"

@@ -47,7 +47,8 @@ EXPORTS = (
     "pdgpu_bondorder_build", "pdgpu_fixed_terms", "pdgpu_instance_bytes", "pdgpu_bytes_free",
     "pdgpu_profile", "pdgpu_profile_read", "pdgpu_sd_start", "pdgpu_sd_advance", "pdgpu_sd_finish",
     "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune", "pdgpu_md_vv_pd", "pdgpu_md_langevin_pd",
-    "pdgpu_md_beeman_pd",
+    "pdgpu_md_beeman_pd", "pdgpu_mime_encode", "pdgpu_snapshot_mime", "pdgpu_dcd_header", "pdgpu_dcd_frame",
+    "pdgpu_dcd_create", "pdgpu_dcd_append",
 )
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
@@ -55,6 +56,28 @@ FF_BONDED = FF_BONDS | FF_ANGLES | FF_TORSIONS | FF_IMPROPERS
 FF_SOFTVDW = 32  # FF_SoftVDW soft-core repulsion (not in FF_ALL; not with FF_GB)
 FF_ALL = FF_GB | FF_BONDED
 KERNEL_GROUPS = ("nlist", "atom", "born", "pair", "chain", "integrate", "enum", "bonded")
+
+
+def mime_encode(xyz, vel, force, epot, name=""):
+    """PD's SnapShot MIME text of the given arrays (A, m/s, N; epot in
+    J/molecule), as pdgpu_mime_encode writes it."""
+    x, v, f = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (xyz, vel, force))
+    n = len(x) // 3
+    nb = name.encode()
+    need = lib.pdgpu_mime_encode(_p(x, _dp), _p(v, _dp), _p(f, _dp), n, epot, nb, None, 0)
+    if need < 0:
+        raise PdgpuError(need, "mime_encode")
+    buf = C.create_string_buffer(need + 1)
+    lib.pdgpu_mime_encode(_p(x, _dp), _p(v, _dp), _p(f, _dp), n, epot, nb, buf, need + 1)
+    return buf.raw[:need].decode()
+
+
+def dcd_header(natoms):
+    """The DCD header PD's OutTra_NAMD writes for natoms atoms (bytes)."""
+    need = lib.pdgpu_dcd_header(natoms, None, 0)
+    buf = C.create_string_buffer(need)
+    lib.pdgpu_dcd_header(natoms, buf, need)
+    return buf.raw
 
 
 class PdgpuError(RuntimeError):
@@ -138,6 +161,13 @@ def _load():
                                            C.c_double, C.c_uint64, _dp, P(Energies), _dp]),
         "pdgpu_md_beeman_pd": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32, P(Energies),
                                          _dp]),
+        "pdgpu_mime_encode": (C.c_int64, [_dp, _dp, _dp, C.c_int32, C.c_double, C.c_char_p, C.c_char_p,
+                                          C.c_int64]),
+        "pdgpu_snapshot_mime": (C.c_int64, [C.c_void_p, C.c_int32, C.c_char_p, C.c_char_p, C.c_int64]),
+        "pdgpu_dcd_header": (C.c_int64, [C.c_int32, C.c_char_p, C.c_int64]),
+        "pdgpu_dcd_frame": (C.c_int64, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int64]),
+        "pdgpu_dcd_create": (C.c_int, [C.c_char_p, C.c_int32]),
+        "pdgpu_dcd_append": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
         "pdgpu_bondorder_build": (C.c_int, [C.c_int32, _ip, C.c_int32, _ip, _ip, C.c_int32, _ip, _ip, C.c_int32,
                                             _ip, _ip, _ip, _bp, _ip, _ip, _ip, _ip]),
         "pdgpu_fixed_terms": (C.c_int, [C.c_int32, _dp, C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _ip, _dp,
@@ -473,6 +503,35 @@ class Context:
         (protocols/md.cpp:840-904). Returns like md_vv."""
         return self._md_call("md_beeman_pd", lambda e, k: lib.pdgpu_md_beeman_pd(
             self.h, iid, steps, dt, update_nlist, e, k), iid)
+    def snapshot_mime(self, iid, name=""):
+        """WorkSpace::save().writeMIME(name) of instance iid (text)."""
+        nb = name.encode()
+        need = lib.pdgpu_snapshot_mime(self.h, iid, nb, None, 0)
+        if need < 0:
+            self._check(int(need), "snapshot_mime")
+        buf = C.create_string_buffer(need + 1)
+        got = lib.pdgpu_snapshot_mime(self.h, iid, nb, buf, need + 1)
+        if got != need:
+            self._check(int(got) if got < 0 else PDGPU_ERR_HIP, "snapshot_mime")
+        return buf.raw[:need].decode()
+
+    def dcd_frame(self, iid):
+        """One DCD frame of instance iid's positions (bytes)."""
+        need = lib.pdgpu_dcd_frame(self.h, iid, None, 0)
+        if need < 0:
+            self._check(int(need), "dcd_frame")
+        buf = C.create_string_buffer(need)
+        got = lib.pdgpu_dcd_frame(self.h, iid, buf, need)
+        if got != need:
+            self._check(int(got) if got < 0 else PDGPU_ERR_HIP, "dcd_frame")
+        return buf.raw
+
+    def dcd_create(self, path, iid):
+        self._check(lib.pdgpu_dcd_create(str(path).encode(), self._natoms[iid]), "dcd_create")
+
+    def dcd_append(self, path, iid):
+        self._check(lib.pdgpu_dcd_append(self.h, iid, str(path).encode()), "dcd_append")
+
     def profile(self, enable=True):
         self._check(lib.pdgpu_profile(self.h, int(enable)), "profile")
 

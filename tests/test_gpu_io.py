@@ -1,0 +1,57 @@
+"""Output formats written from device state through the C ABI:
+pdgpu_dcd_create / pdgpu_dcd_append and pdgpu_dcd_frame (positions
+converted to float on the device) against the DCD file the compiled
+reference's OutTra_NAMD wrote for the same three frames (every byte except
+PD's uninitialised title bytes), and pdgpu_snapshot_mime against
+WorkSpace::save().writeMIME (positions and velocities bit-exact, forces and
+energy to the FP64 path's bounds: 1e-9 force RMS, 1e-10 energy)."""
+import numpy as np
+import pytest
+
+import golden_io as G
+import pdgpu
+from gpu_helpers import context_for, rel, rms_rel
+from test_io_formats import IO_CASES, TITLE_GARBAGE, decode_6bit, split_mime
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", IO_CASES)
+def test_dcd_file_matches_reference(name, tmp_path):
+    g = G.load(name)
+    n = len(g["mass"])
+    frames = g["io_dcd_xyz"].reshape(3, n, 3)
+    path = tmp_path / "t.dcd"
+    with context_for(g, pdgpu.MIXED) as ctx:
+        i = ctx.add_system(g, bonded=True)
+        ctx.dcd_create(path, i)
+        for fr in frames:
+            ctx.set_positions(i, fr)
+            ctx.dcd_append(path, i)
+        one = ctx.dcd_frame(i)
+    ours = path.read_bytes()
+    ref = g["io_dcd"].view(np.uint8).tobytes()
+    assert len(ours) == len(ref)
+    keep = np.ones(len(ref), bool)
+    keep[TITLE_GARBAGE] = False
+    assert np.array_equal(np.frombuffer(ours, np.uint8)[keep], np.frombuffer(ref, np.uint8)[keep])
+    assert one == ours[-len(one):]
+
+
+@pytest.mark.parametrize("name", IO_CASES)
+def test_snapshot_mime_of_device_state(name):
+    g = G.load(name)
+    n = len(g["mass"])
+    with context_for(g, pdgpu.FP64) as ctx:
+        i = ctx.add_system(g, xyz=g["md_xyz"], bonded=True)
+        ctx.set_velocities(i, g["md_v"])
+        ctx.calc_forces(i)
+        text = ctx.snapshot_mime(i, name)
+    nm, body = split_mime(text)
+    assert nm == name
+    rec = decode_6bit(body)
+    assert np.frombuffer(rec[0:4], np.int32)[0] == n
+    assert rel(np.frombuffer(rec[4:12], np.float64)[0], float(g["io_epot"])) <= 1e-10
+    a = np.frombuffer(rec[128:128 + 72 * n], np.float64).reshape(n, 9)
+    assert np.array_equal(a[:, :3], g["md_xyz"]) and np.array_equal(a[:, 3:6], g["md_v"])
+    assert rms_rel(a[:, 6:], g["io_force"]) <= 1e-9
