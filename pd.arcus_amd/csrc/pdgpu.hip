@@ -96,6 +96,34 @@ struct Instance {
     std::vector<double> mass;
 };
 
+// One destination of a staged upload: bytes from blob offset src (or
+// zeros) to the device address dst.
+struct UpSec {
+    unsigned long long dst;
+    long long src, bytes, zero;
+};
+
+// Scatters the sections of an uploaded blob: grid (sections, parts), each
+// part copying a strided share of its section in the widest words that the
+// section's alignment allows.
+__global__ void k_scatter(const unsigned char *blob, const UpSec *secs) {
+    const UpSec u = secs[blockIdx.x];
+    const long long tid = (long long)blockIdx.y * blockDim.x + threadIdx.x, nth = (long long)gridDim.y * blockDim.x;
+    const unsigned long long al = u.dst | (unsigned long long)u.src | (unsigned long long)u.bytes;
+    if ((al & 7) == 0) {
+        unsigned long long *d = reinterpret_cast<unsigned long long *>(u.dst);
+        const unsigned long long *q = reinterpret_cast<const unsigned long long *>(blob + u.src);
+        for (long long k = tid; k < u.bytes / 8; k += nth) d[k] = u.zero ? 0ull : q[k];
+    } else if ((al & 3) == 0) {
+        unsigned int *d = reinterpret_cast<unsigned int *>(u.dst);
+        const unsigned int *q = reinterpret_cast<const unsigned int *>(blob + u.src);
+        for (long long k = tid; k < u.bytes / 4; k += nth) d[k] = u.zero ? 0u : q[k];
+    } else {
+        unsigned char *d = reinterpret_cast<unsigned char *>(u.dst);
+        for (long long k = tid; k < u.bytes; k += nth) d[k] = u.zero ? (unsigned char)0 : blob[u.src + k];
+    }
+}
+
 } // namespace
 
 struct pdgpu_ctx {
@@ -140,6 +168,15 @@ struct pdgpu_ctx {
     // pinned host staging for per-instance results (energies, counts, status)
     void *h_stage = nullptr;
     size_t h_stage_bytes = 0;
+    // staged uploads: host data of an instance (topology, term records,
+    // positions) is gathered into one blob, copied once through pinned
+    // memory and scattered on the device by k_scatter
+    std::vector<unsigned char> up_blob;
+    std::vector<UpSec> up_secs;
+    void *h_up = nullptr;
+    size_t h_up_bytes = 0;
+    hipEvent_t up_done = nullptr;
+    DevBuf upl;
 
     // captured step graphs, keyed by loop kind and instance, each with the
     // launch arguments it was captured with
@@ -257,6 +294,51 @@ void release(DevBuf &b) {
         int r_ = (x);                   \
         if (r_ != PDGPU_OK) return r_;  \
     } while (0)
+
+// Staged upload: up_put / up_zero queue sections (the host bytes are
+// copied into the blob at once), up_commit sends the blob in one copy and
+// scatters it in one kernel, in stream order; the host does not wait (the
+// pinned buffer is reused only after the previous upload's event).
+void up_put(pdgpu_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (bytes == 0) return;
+    const size_t off = (c->up_blob.size() + 15) & ~(size_t)15;
+    c->up_blob.resize(off + bytes);
+    std::memcpy(c->up_blob.data() + off, src, bytes);
+    c->up_secs.push_back({(unsigned long long)dst, (long long)off, (long long)bytes, 0});
+}
+
+void up_zero(pdgpu_ctx *c, void *dst, size_t bytes) {
+    if (bytes) c->up_secs.push_back({(unsigned long long)dst, 0, (long long)bytes, 1});
+}
+
+int up_commit(pdgpu_ctx *c) {
+    if (c->up_secs.empty()) return PDGPU_OK;
+    const size_t toff = (c->up_blob.size() + 15) & ~(size_t)15;
+    const size_t total = toff + c->up_secs.size() * sizeof(UpSec);
+    c->up_blob.resize(total);
+    std::memcpy(c->up_blob.data() + toff, c->up_secs.data(), c->up_secs.size() * sizeof(UpSec));
+    const int nsec = (int)c->up_secs.size();
+    c->up_secs.clear();
+    if (c->up_done) HIPCHECK(c, hipEventSynchronize(c->up_done));
+    if (c->h_up_bytes < total) {
+        if (c->h_up) (void)hipHostFree(c->h_up);
+        c->h_up = nullptr;
+        c->h_up_bytes = 0;
+        const size_t cap = std::max<size_t>(total, 1 << 20);
+        HIPCHECK(c, hipHostMalloc(&c->h_up, cap, hipHostMallocDefault));
+        c->h_up_bytes = cap;
+    }
+    std::memcpy(c->h_up, c->up_blob.data(), total);
+    c->up_blob.clear();
+    TRY(resize(c, c->upl, total, 1));
+    HIPCHECK(c, hipMemcpyAsync(c->upl.p, c->h_up, total, hipMemcpyHostToDevice, c->stream));
+    const unsigned char *blob = (const unsigned char *)c->upl.p;
+    hipLaunchKernelGGL(k_scatter, dim3(nsec, 8), dim3(256), 0, c->stream, blob, (const UpSec *)(blob + toff));
+    HIPCHECK(c, hipGetLastError());
+    if (!c->up_done) HIPCHECK(c, hipEventCreateWithFlags(&c->up_done, hipEventDisableTiming));
+    HIPCHECK(c, hipEventRecord(c->up_done, c->stream));
+    return PDGPU_OK;
+}
 
 int grow_atoms(pdgpu_ctx *c, int64_t blocks) {
     const size_t na = (size_t)blocks * kBlk;
@@ -715,11 +797,10 @@ int scatter3(pdgpu_ctx *c, int id, DevBuf &x, DevBuf &y, DevBuf &z, const double
         hy[i] = in[3 * i + 1] * scale;
         hz[i] = in[3 * i + 2] * scale;
     }
-    HIPCHECK(c, hipMemcpyAsync((double *)x.p + a0, hx.data(), I.natoms * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync((double *)y.p + a0, hy.data(), I.natoms * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipMemcpyAsync((double *)z.p + a0, hz.data(), I.natoms * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    return PDGPU_OK;
+    up_put(c, (double *)x.p + a0, hx.data(), I.natoms * 8);
+    up_put(c, (double *)y.p + a0, hy.data(), I.natoms * 8);
+    up_put(c, (double *)z.p + a0, hz.data(), I.natoms * 8);
+    return up_commit(c);
 }
 
 // Reserves n records in one FF_Bonded record family (bufs / esz: its
@@ -911,13 +992,15 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
                      &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
                      &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot, &c->rx, &c->ry, &c->rz,
-                     &c->inst_moved, &c->any_need, &c->fox, &c->foy, &c->foz, &c->rng_ctr, &c->stage};
+                     &c->inst_moved, &c->any_need, &c->fox, &c->foy, &c->foz, &c->rng_ctr, &c->stage, &c->upl};
     for (DevBuf *b : all) release(*b);
     for (auto &g : c->graphs)
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : c->prof_pool) (void)hipEventDestroy(e);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->h_up) (void)hipHostFree(c->h_up);
+    if (c->up_done) (void)hipEventDestroy(c->up_done);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1035,57 +1118,54 @@ extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t
         t123[i] = t->terms123[i];
         minv[i] = (t->mass && t->mass[i] > 0) ? 1.0 / t->mass[i] : 0.0;
     }
-    hipStream_t s = c->stream;
-    HIPCHECK(c, hipMemcpyAsync((double *)c->q.p + a0, q.data(), na * 8, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((double *)c->t123.p + a0, t123.data(), na * 8, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((double *)c->minv.p + a0, minv.data(), na * 8, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((double2 *)c->lj64.p + a0, lj.data(), na * 16, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((double2 *)c->gb64.p + a0, gb.data(), na * 16, hipMemcpyHostToDevice, s));
-    std::vector<float2> lj32(na), gb32(na);
+    // one staged upload of everything below (no host wait)
+    up_put(c, (double *)c->q.p + a0, q.data(), na * 8);
+    up_put(c, (double *)c->t123.p + a0, t123.data(), na * 8);
+    up_put(c, (double *)c->minv.p + a0, minv.data(), na * 8);
+    up_put(c, (double2 *)c->lj64.p + a0, lj.data(), na * 16);
+    up_put(c, (double2 *)c->gb64.p + a0, gb.data(), na * 16);
     if (c->mixed) {
+        std::vector<float2> lj32(na), gb32(na);
         for (size_t i = 0; i < na; ++i) {
             lj32[i] = make_float2((float)lj[i].x, (float)lj[i].y);
             gb32[i] = make_float2((float)gb[i].x, (float)gb[i].y);
         }
-        HIPCHECK(c, hipMemcpyAsync((float2 *)c->lj32.p + a0, lj32.data(), na * 8, hipMemcpyHostToDevice, s));
-        HIPCHECK(c, hipMemcpyAsync((float2 *)c->gb32.p + a0, gb32.data(), na * 8, hipMemcpyHostToDevice, s));
+        up_put(c, (float2 *)c->lj32.p + a0, lj32.data(), na * 8);
+        up_put(c, (float2 *)c->gb32.p + a0, gb32.data(), na * 8);
     }
-    std::vector<double> zero(na, 0.0);
     DevBuf *zs[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->fox, &c->foy, &c->foz};
-    for (DevBuf *b : zs) HIPCHECK(c, hipMemcpyAsync((double *)b->p + a0, zero.data(), na * 8, hipMemcpyHostToDevice, s));
+    for (DevBuf *b : zs) up_zero(c, (double *)b->p + a0, na * 8);
     // per block
     std::vector<int> bi(nb, slot), bn(nb), bf(nb, (int)b0);
     for (int k = 0; k < nb; ++k) bn[k] = std::min(kBlk, n - k * kBlk);
-    HIPCHECK(c, hipMemcpyAsync((int *)c->blk_inst.p + b0, bi.data(), nb * 4, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((int *)c->blk_n.p + b0, bn.data(), nb * 4, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((int *)c->blk_first.p + b0, bf.data(), nb * 4, hipMemcpyHostToDevice, s));
+    up_put(c, (int *)c->blk_inst.p + b0, bi.data(), nb * 4);
+    up_put(c, (int *)c->blk_n.p + b0, bn.data(), nb * 4);
+    up_put(c, (int *)c->blk_first.p + b0, bf.data(), nb * 4);
     for (int k = 0; k < nb; ++k) {
         c->h_blk_inst[b0 + k] = slot;
         c->h_blk_n[b0 + k] = bn[k];
         c->h_blk_first[b0 + k] = (int)b0;
     }
     // bond orders
-    if (bcopy > 0)
-        HIPCHECK(c, hipMemcpyAsync((int8_t *)c->band.p + boff, t->band, bcopy, hipMemcpyHostToDevice, s));
+    if (bcopy > 0) up_put(c, (int8_t *)c->band.p + boff, t->band, bcopy);
     std::vector<int4> sp(t->nsparse);
     for (int k = 0; k < t->nsparse; ++k) {
         int i = t->sparse_i[k], j = t->sparse_j[k];
         if (i > j) std::swap(i, j);
         sp[k] = make_int4(i, j, t->sparse_order[k], 0);
     }
-    if (t->nsparse > 0)
-        HIPCHECK(c, hipMemcpyAsync((int4 *)c->sparse.p + soff, sp.data(), sp.size() * 16, hipMemcpyHostToDevice, s));
+    if (t->nsparse > 0) up_put(c, (int4 *)c->sparse.p + soff, sp.data(), sp.size() * 16);
     const int mid = t->max_index_delta;
     const long long boffl = boff;
     const int soffi = (int)soff, sn = t->nsparse;
-    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_mid.p + slot, &mid, 4, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_nat.p + slot, &n, 4, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((long long *)c->inst_band_off.p + slot, &boffl, 8, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_sp_off.p + slot, &soffi, 4, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_sp_n.p + slot, &sn, 4, hipMemcpyHostToDevice, s));
+    up_put(c, (int *)c->inst_mid.p + slot, &mid, 4);
+    up_put(c, (int *)c->inst_nat.p + slot, &n, 4);
+    up_put(c, (long long *)c->inst_band_off.p + slot, &boffl, 8);
+    up_put(c, (int *)c->inst_sp_off.p + slot, &soffi, 4);
+    up_put(c, (int *)c->inst_sp_n.p + slot, &sn, 4);
     const long long csl = csoff;
-    HIPCHECK(c, hipMemcpyAsync((long long *)c->inst_cslot.p + slot, &csl, 8, hipMemcpyHostToDevice, s));
-    HIPCHECK(c, hipStreamSynchronize(s));
+    up_put(c, (long long *)c->inst_cslot.p + slot, &csl, 8);
+    TRY(up_commit(c));
 
     Instance &I = c->inst[slot];
     I = Instance();
@@ -1110,15 +1190,11 @@ extern "C" int pdgpu_instance_remove(pdgpu_ctx *c, int32_t id) {
     if (!c || !valid_id(c, id)) return PDGPU_ERR_INVALID;
     (void)hipSetDevice(c->device);
     Instance &I = c->inst[id];
-    std::vector<int> neg(I.nblocks, -1), zero(I.nblocks, 0);
-    HIPCHECK(c, hipMemcpyAsync((int *)c->blk_inst.p + I.block0, neg.data(), I.nblocks * 4, hipMemcpyHostToDevice,
-                               c->stream));
-    HIPCHECK(c, hipMemcpyAsync((int *)c->row_count.p + I.block0, zero.data(), I.nblocks * 4, hipMemcpyHostToDevice,
-                               c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    const int zero1 = 0;
-    HIPCHECK(c, hipMemcpyAsync((int *)c->inst_nat.p + id, &zero1, 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<int> neg(I.nblocks, -1);
+    up_put(c, (int *)c->blk_inst.p + I.block0, neg.data(), I.nblocks * 4);
+    up_zero(c, (int *)c->row_count.p + I.block0, I.nblocks * 4);
+    up_zero(c, (int *)c->inst_nat.p + id, 4);
+    TRY(up_commit(c));
     TRY(bonded_release(c, I));
     c->csr.release(I.cs_off, I.cs_len);
     for (int k = 0; k < I.nblocks; ++k) c->h_blk_inst[I.block0 + k] = -1;
@@ -1211,24 +1287,22 @@ extern "C" int pdgpu_bonded_set(pdgpu_ctx *c, int32_t id, const pdgpu_bonded *b)
         dn[k] = make_double4(ww[0], ww[1], ww[2], ww[3]);
         dg[k] = make_double4(gg[0], gg[1], gg[2], gg[3]);
     }
-    hipStream_t s = c->stream;
     if (nb) {
-        HIPCHECK(c, hipMemcpyAsync((int4 *)c->bd_idx.p + I.bd_off, bi.data(), nb * 16, hipMemcpyHostToDevice, s));
-        HIPCHECK(c, hipMemcpyAsync((double2 *)c->bd_par.p + I.bd_off, bp.data(), nb * 16, hipMemcpyHostToDevice, s));
+        up_put(c, (int4 *)c->bd_idx.p + I.bd_off, bi.data(), nb * 16);
+        up_put(c, (double2 *)c->bd_par.p + I.bd_off, bp.data(), nb * 16);
     }
     if (na) {
-        HIPCHECK(c, hipMemcpyAsync((int4 *)c->an_idx.p + I.an_off, ai.data(), na * 16, hipMemcpyHostToDevice, s));
-        HIPCHECK(c, hipMemcpyAsync((double2 *)c->an_par.p + I.an_off, ap.data(), na * 16, hipMemcpyHostToDevice, s));
+        up_put(c, (int4 *)c->an_idx.p + I.an_off, ai.data(), na * 16);
+        up_put(c, (double2 *)c->an_par.p + I.an_off, ap.data(), na * 16);
     }
     if (nd) {
-        HIPCHECK(c, hipMemcpyAsync((int4 *)c->dh_idx.p + I.dh_off, di.data(), nd * 16, hipMemcpyHostToDevice, s));
-        HIPCHECK(c, hipMemcpyAsync((double4 *)c->dh_vn.p + I.dh_off, dv.data(), nd * 32, hipMemcpyHostToDevice, s));
-        HIPCHECK(c, hipMemcpyAsync((double4 *)c->dh_n.p + I.dh_off, dn.data(), nd * 32, hipMemcpyHostToDevice, s));
-        HIPCHECK(c, hipMemcpyAsync((double4 *)c->dh_gamma.p + I.dh_off, dg.data(), nd * 32, hipMemcpyHostToDevice, s));
-        HIPCHECK(c, hipMemcpyAsync((int *)c->dh_meta.p + I.dh_off, dm.data(), nd * 4, hipMemcpyHostToDevice, s));
+        up_put(c, (int4 *)c->dh_idx.p + I.dh_off, di.data(), nd * 16);
+        up_put(c, (double4 *)c->dh_vn.p + I.dh_off, dv.data(), nd * 32);
+        up_put(c, (double4 *)c->dh_n.p + I.dh_off, dn.data(), nd * 32);
+        up_put(c, (double4 *)c->dh_gamma.p + I.dh_off, dg.data(), nd * 32);
+        up_put(c, (int *)c->dh_meta.p + I.dh_off, dm.data(), nd * 4);
     }
-    HIPCHECK(c, hipStreamSynchronize(s));
-    return PDGPU_OK;
+    return up_commit(c);
 }
 
 extern "C" int64_t pdgpu_bonded_bytes(int32_t nbonds, int32_t nangles, int32_t ntorsions, int32_t nimpropers) {
