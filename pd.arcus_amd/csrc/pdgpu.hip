@@ -656,7 +656,7 @@ int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, in
 // finalize the caller launches a kernel that finalises the forces itself
 // (the velocity-Verlet kick).
 template <typename R>
-int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true) {
+int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2 = nullptr, int f2_stride = 0) {
     const DevState S = c->S();
     const Span sp = span_of(c, id);
     const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk;
@@ -698,7 +698,8 @@ int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true) {
     }
     if (finalize) {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
-        hipLaunchKernelGGL(k_finalize<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
+        hipLaunchKernelGGL(k_finalize<R>, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1, f2,
+                           f2_stride);
     }
     HIPCHECK(c, hipGetLastError());
     return PDGPU_OK;

@@ -1586,18 +1586,27 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
 
 // Forces f = -dE/dx in kcal/mol/A (double), dE/da for export.
 template <typename R>
-__global__ void k_finalize(DevState S, int atom_begin, int atom_end) {
+// With f2 given, also adds each instance's sum of |f|^2 to f2[inst *
+// f2_stride] (one wavefront is one block of one instance, atom_begin being
+// block-aligned).
+__global__ void k_finalize(DevState S, int atom_begin, int atom_end, double *f2, int f2_stride) {
     const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= atom_end) return;
     const int b = a / kBlk;
-    const int inst = S.blk_inst[b];
-    if (inst < 0 || (a % kBlk) >= S.blk_n[b]) return;
-    const double fx = -acc_read<R>(S.acc_fx, a, kForceScale);
-    const double fy = -acc_read<R>(S.acc_fy, a, kForceScale);
-    const double fz = -acc_read<R>(S.acc_fz, a, kForceScale);
-    S.fx[a] = fx; S.fy[a] = fy; S.fz[a] = fz;
-    S.deda_out[a] = acc_read<R>(S.acc_deda, a, kForceScale);
-    if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicOr(S.inst_status + inst, kStatNonFinite);
+    const int inst = a < atom_end ? S.blk_inst[b] : -1;
+    double sq = 0;
+    if (inst >= 0 && (a % kBlk) < S.blk_n[b]) {
+        const double fx = -acc_read<R>(S.acc_fx, a, kForceScale);
+        const double fy = -acc_read<R>(S.acc_fy, a, kForceScale);
+        const double fz = -acc_read<R>(S.acc_fz, a, kForceScale);
+        S.fx[a] = fx; S.fy[a] = fy; S.fz[a] = fz;
+        S.deda_out[a] = acc_read<R>(S.acc_deda, a, kForceScale);
+        if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicOr(S.inst_status + inst, kStatNonFinite);
+        sq = fx * fx + fy * fy + fz * fz;
+    }
+    if (f2) { // uniform over the launch
+        sq = wave_sum(sq);
+        if (lane_id() == 0 && inst >= 0) unsafeAtomicAdd(f2 + (size_t)inst * f2_stride, sq);
+    }
 }
 
 // ------------------------------------------------------ PD-format export

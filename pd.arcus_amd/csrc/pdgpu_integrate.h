@@ -61,35 +61,28 @@ __global__ void k_sd_start(InstMD *md, int s, int max_steps, double step_size, d
 // With update_nlist < 0 an instance rebuilds when k_moved has flagged it
 // (the Verlet-skin criterion, an extension PD does not have); the flags are
 // cleared here for the next step.
-__global__ void k_sd_need(const InstMD *md, int *need, int *moved, int *any, int s0, int s1) {
-    const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= s1) return;
-    const InstMD &m = md[s];
-    const bool sched = m.update_nlist > 0 && (m.step % m.update_nlist) == 0;
-    const bool skin = m.update_nlist < 0 && moved[s] != 0;
-    const bool n = m.running && !m.done && (sched || skin);
-    need[s] = n ? 1 : 0;
-    moved[s] = 0;
-    if (n) atomicOr(any, 1);
+// One workgroup: the flags per instance, and *any = whether any is set
+// (written, not or-ed, so no reset is needed between steps).
+__global__ void __launch_bounds__(1024) k_sd_need(const InstMD *md, int *need, int *moved, int *any, int s0,
+                                                  int s1) {
+    int found = 0;
+    for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+        const InstMD &m = md[s];
+        const bool sched = m.update_nlist > 0 && (m.step % m.update_nlist) == 0;
+        const bool skin = m.update_nlist < 0 && moved[s] != 0;
+        const bool n = m.running && !m.done && (sched || skin);
+        need[s] = n ? 1 : 0;
+        moved[s] = 0;
+        found |= n ? 1 : 0;
+    }
+    found = __syncthreads_or(found);
+    if (threadIdx.x == 0) *any = found;
 }
 
-__global__ void k_flag_reset(int *p) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
-}
-
-// sum of |f|^2 per instance (one wave per block)
-__global__ void k_force_sq(DevState S, InstMD *md, int blk_begin, int blk_end) {
-    const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    if (b >= blk_end) return;
-    const int inst = S.blk_inst[b];
-    if (inst < 0) return;
-    const int l = lane_id();
-    const size_t a = (size_t)b * kBlk + l;
-    double f2 = 0;
-    if (l < S.blk_n[b]) f2 = S.fx[a] * S.fx[a] + S.fy[a] * S.fy[a] + S.fz[a] * S.fz[a];
-    f2 = wave_sum(f2);
-    if (l == 0 && md[inst].running && !md[inst].done) unsafeAtomicAdd(&md[inst].f2, f2);
-}
+// The per-instance |f|^2 sums (InstMD::f2) are added by k_finalize: an SD
+// step passes it &md[0].f2 with this stride.
+static_assert(sizeof(InstMD) % sizeof(double) == 0, "InstMD is read with a stride in doubles");
+constexpr int kInstMDStride = (int)(sizeof(InstMD) / sizeof(double));
 
 // PD's accept / reject rule (protocols/minimise.cpp:181-235), one thread per
 // instance, on the energy just evaluated.
