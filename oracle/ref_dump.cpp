@@ -467,8 +467,9 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
         // T = 300 K). One run per integrator from the same start: md_*
         // VelocityVerlet, lg_* Langevin (PD's defaults: 10/ps friction, 300 K,
         // LangevinOnHydrogens; libc rand() after srand(lg_seed)), bm_* Beeman
-        // (the workspace's old forces zeroed first).
-        for (int integ = 0; integ < 3; ++integ) {
+        // (the workspace's old forces zeroed first), lx_* Langevin at 100/ps
+        // (gamma dt = 0.1: the exponential form of the coefficients).
+        for (int integ = 0; integ < 4; ++integ) {
             System sys(ffps);
             sys.add(NewProtein(ffps, pd_grammar(c.seq)));
             WorkSpace ws(sys);
@@ -509,10 +510,11 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
             Protocol::MolecularDynamics md(ff);
             const char *tag = "md";
             md.Integrator = Protocol::MolecularDynamics::VelocityVerlet;
-            const unsigned lg_seed = 1234u;
-            if (integ == 1) {
-                tag = "lg";
+            const unsigned lg_seed = integ == 3 ? 4321u : 1234u;
+            if (integ == 1 || integ == 3) {
+                tag = integ == 1 ? "lg" : "lx";
                 md.Integrator = Protocol::MolecularDynamics::Langevin;
+                if (integ == 3) md.FricCoeff = 1e14;
             } else if (integ == 2) {
                 tag = "bm";
                 md.Integrator = Protocol::MolecularDynamics::Beeman;
@@ -525,7 +527,7 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
             md.UpdateScr = 0;
             md.UpdateTra = 0;
             md.OutputLevel = Verbosity::Silent;
-            if (integ == 1) srand(lg_seed);
+            if (integ == 1 || integ == 3) srand(lg_seed);
             md.run();
             std::vector<double> xyz, vel;
             for (int i = 0; i < n; ++i) {
@@ -547,10 +549,10 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
             r.d(t + "_v", vel);
             r.s(t + "_epot", ws.ene.epot);
             r.s(t + "_ekin", ws.ene.ekin);
-            if (integ == 1) {
-                r.s("lg_seed", (double)lg_seed);
-                r.s("lg_temp", 300.0);
-                r.s("lg_fric", md.FricCoeff);
+            if (integ == 1 || integ == 3) {
+                r.s(t + "_seed", (double)lg_seed);
+                r.s(t + "_temp", 300.0);
+                r.s(t + "_fric", md.FricCoeff);
             }
             if (integ == 0) {
                 // PD's result hand-off formats on the final VelocityVerlet

@@ -28,25 +28,28 @@ BM_CASES = [c for c in CASES if "bm_xyz" in G.load(c)]
 TOL = {pdgpu.FP64: (1e-8, 1e-7, 1e-9), pdgpu.MIXED: (1e-3, 1e-2, 1e-4)}
 
 
+@pytest.mark.parametrize("tag", ["lg", "lx"])
 @pytest.mark.parametrize("name", LG_CASES)
 @pytest.mark.parametrize("precision", [pdgpu.FP64, pdgpu.MIXED])
-def test_pd_langevin_matches_reference(name, precision):
+def test_pd_langevin_matches_reference(name, precision, tag):
+    """lg: PD's default friction (Taylor coefficients); lx: 100/ps
+    (exponential coefficients)."""
     g = G.load(name)
     steps = int(g["md_steps"])
     tx, tv, te = TOL[precision]
-    noise = O.langevin_noise(int(g["lg_seed"]), len(g["mass"]), steps)
+    noise = O.langevin_noise(int(g[tag + "_seed"]), len(g["mass"]), steps)
     with context_for(g, precision) as ctx:
         i = ctx.add_system(g, xyz=g["md_xyz0"], bonded=True)
         ctx.set_velocities(i, g["md_v0"].reshape(-1, 3))
-        e, ek = ctx.md_langevin_pd(i, steps, float(g["md_dt"]), float(g["lg_temp"]), float(g["lg_fric"]),
+        e, ek = ctx.md_langevin_pd(i, steps, float(g["md_dt"]), float(g[tag + "_temp"]), float(g[tag + "_fric"]),
                                    noise=noise, update_nlist=int(g["md_update_nlist"]))
         assert e["status"] == 0
         x = ctx.get_positions(i)
         v = ctx.get_velocities(i)
-    assert np.abs(x - g["lg_xyz"]).max() <= tx
-    assert np.abs(v - g["lg_v"]).max() <= tv * np.abs(g["lg_v"]).max()
-    assert rel(e["epot"], g["lg_epot"]) <= te
-    assert rel(ek, g["lg_ekin"]) <= te
+    assert np.abs(x - g[tag + "_xyz"]).max() <= tx
+    assert np.abs(v - g[tag + "_v"]).max() <= tv * np.abs(g[tag + "_v"]).max()
+    assert rel(e["epot"], g[tag + "_epot"]) <= te
+    assert rel(ek, g[tag + "_ekin"]) <= te
 
 
 @pytest.mark.parametrize("name", BM_CASES)

@@ -96,25 +96,28 @@ def test_noise_is_standard_normal_and_in_call_order():
     assert not np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("tag", ["lg", "lx"])
 @pytest.mark.parametrize("name", LG_CASES)
-def test_pd_langevin_matches_reference(name):
-    """PD's MolecularDynamics with Integrator = Langevin (PD's defaults:
-    friction 10/ps, 300 K, LangevinOnHydrogens), from the same start, with
-    the reference's own libc rand() stream."""
+def test_pd_langevin_matches_reference(name, tag):
+    """PD's MolecularDynamics with Integrator = Langevin (300 K,
+    LangevinOnHydrogens; lg: PD's default friction 10/ps, the Taylor
+    coefficients; lx: 100/ps, the exponential ones), from the same start,
+    with the reference's own libc rand() stream."""
     d = G.load(name)
-    assert np.array_equal(d["lg_xyz0"], d["md_xyz0"])
+    assert np.array_equal(d[tag + "_xyz0"], d["md_xyz0"])
     s = _system(d)
     p = O.default_params(**d["param_dict"])
     steps = int(d["md_steps"])
-    noise = O.langevin_noise(int(d["lg_seed"]), len(d["mass"]), steps)
+    assert O.langevin_coef(float(d[tag + "_fric"]), float(d["md_dt"]))[0] == (2 if tag == "lg" else 1)
+    noise = O.langevin_noise(int(d[tag + "_seed"]), len(d["mass"]), steps)
     rc, x, v, te, tk = O.md_langevin_pd(s, d["md_xyz0"], d["md_v0"], p, steps, float(d["md_dt"]),
-                                        float(d["lg_temp"]), float(d["lg_fric"]), noise,
+                                        float(d[tag + "_temp"]), float(d[tag + "_fric"]), noise,
                                         int(d["md_update_nlist"]))
     assert rc == steps
-    assert np.abs(x - d["lg_xyz"]).max() <= 1e-9
-    assert np.abs(v - d["lg_v"]).max() <= 1e-9 * np.abs(d["lg_v"]).max()
-    assert rel(te[-1], d["lg_epot"]) <= 1e-10
-    assert rel(tk[-1], d["lg_ekin"]) <= 1e-10
+    assert np.abs(x - d[tag + "_xyz"]).max() <= 1e-9
+    assert np.abs(v - d[tag + "_v"]).max() <= 1e-9 * np.abs(d[tag + "_v"]).max()
+    assert rel(te[-1], d[tag + "_epot"]) <= 1e-10
+    assert rel(tk[-1], d[tag + "_ekin"]) <= 1e-10
     # the run differs from VelocityVerlet: the noise and the friction act
     assert np.abs(x - d["md_xyz"]).max() > 1e-4
 
