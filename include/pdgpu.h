@@ -333,6 +333,16 @@ int64_t pdgpu_dcd_frame(pdgpu_ctx *ctx, int32_t id, uint8_t *out, int64_t cap);
 int pdgpu_dcd_create(const char *path, int32_t natoms);
 int pdgpu_dcd_append(pdgpu_ctx *ctx, int32_t id, const char *path);
 
+/* MolecularDynamics::UpdateTra with an OutTra_NAMD in the workspace's
+ * trajectory container (protocols/md.cpp:247, ProtocolBase::every): with
+ * path non-NULL, the PD MD calls (pdgpu_md_vv_pd, _langevin_pd,
+ * _beeman_pd) append a DCD frame of instance id after every step whose
+ * count (from 0 in each call) is a positive multiple of `every`; the file
+ * gets its header with the first frame. Frames are packed on the device
+ * and written in batches, at the latest when the MD call returns. path ==
+ * NULL detaches; removing the instance detaches too. */
+int pdgpu_traj_dcd(pdgpu_ctx *ctx, int32_t id, const char *path, int32_t every);
+
 /* Grid-shape autotuner (no reference counterpart): times the tile kernels
  * of a force evaluation of id (or PDGPU_ALL) reps times at each of several
  * wavefronts-per-CU settings and keeps the fastest per kernel for later

@@ -528,7 +528,15 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
             md.UpdateTra = 0;
             md.OutputLevel = Verbosity::Silent;
             if (integ == 1 || integ == 3) srand(lg_seed);
+            // the VelocityVerlet run also writes a NAMD trajectory every 10
+            // steps through the workspace's trajectory container (md.cpp:247)
+            IO::OutTra_NAMD tra_md(outdir + "/" + c.name + "_mdtra", ws);
+            if (integ == 0) {
+                ws.outtra.add(tra_md);
+                md.UpdateTra = 10;
+            }
             md.run();
+            if (integ == 0) r.b("md_dcd", slurp(outdir + "/" + c.name + "_mdtra.dcd"));
             std::vector<double> xyz, vel;
             for (int i = 0; i < n; ++i) {
                 xyz.push_back(ws.cur.atom[i].p.x); xyz.push_back(ws.cur.atom[i].p.y); xyz.push_back(ws.cur.atom[i].p.z);

@@ -156,6 +156,16 @@ struct pdgpu_ctx {
     DevBuf fox, foy, foz; // Beeman's old forces (N), per atom
     DevBuf rng_ctr;       // Langevin device-noise step counter
     DevBuf stage;         // packing buffer of the output formats
+    // DCD trajectories attached to instances (pdgpu_traj_dcd): frames are
+    // packed on the device into a ring and written in batches
+    struct TrajSink {
+        std::string path;
+        int every = 0;
+        bool created = false;
+        int frames = 0, cap = 0;
+        DevBuf ring;
+    };
+    std::map<int, TrajSink> traj;
     DevBuf band, sparse;
     DevBuf plist, tile_cnt, overflow;
     DevBuf cand, cand_n, inst_cslot; // candidate pairs of the last rebuild, per tile slot
@@ -995,6 +1005,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot, &c->rx, &c->ry, &c->rz,
                      &c->inst_moved, &c->any_need, &c->fox, &c->foy, &c->foz, &c->rng_ctr, &c->stage, &c->upl};
     for (DevBuf *b : all) release(*b);
+    for (auto &t : c->traj) release(t.second.ring);
     for (auto &g : c->graphs)
         if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     for (auto &r : c->prof_recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -1197,6 +1208,12 @@ extern "C" int pdgpu_instance_remove(pdgpu_ctx *c, int32_t id) {
     up_zero(c, (int *)c->inst_nat.p + id, 4);
     TRY(up_commit(c));
     TRY(bonded_release(c, I));
+    auto tj = c->traj.find(id);
+    if (tj != c->traj.end()) { // frames are written at the end of every MD call
+        c->used -= (int64_t)tj->second.ring.bytes;
+        release(tj->second.ring);
+        c->traj.erase(tj);
+    }
     c->csr.release(I.cs_off, I.cs_len);
     for (int k = 0; k < I.nblocks; ++k) c->h_blk_inst[I.block0 + k] = -1;
     c->blocks.release(I.block0, I.nblocks);
@@ -1494,6 +1511,6 @@ DevState state(pdgpu_ctx *c) { return c->S(); }
 hipStream_t stream(pdgpu_ctx *c) { return c->stream; }
 }
 
-#include "pdgpu_integrate.inc"
-
 #include "pdgpu_io.inc"
+
+#include "pdgpu_integrate.inc"

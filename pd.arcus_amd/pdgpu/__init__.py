@@ -48,7 +48,7 @@ EXPORTS = (
     "pdgpu_profile", "pdgpu_profile_read", "pdgpu_sd_start", "pdgpu_sd_advance", "pdgpu_sd_finish",
     "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune", "pdgpu_md_vv_pd", "pdgpu_md_langevin_pd",
     "pdgpu_md_beeman_pd", "pdgpu_mime_encode", "pdgpu_snapshot_mime", "pdgpu_dcd_header", "pdgpu_dcd_frame",
-    "pdgpu_dcd_create", "pdgpu_dcd_append",
+    "pdgpu_dcd_create", "pdgpu_dcd_append", "pdgpu_traj_dcd",
 )
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
@@ -168,6 +168,7 @@ def _load():
         "pdgpu_dcd_frame": (C.c_int64, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int64]),
         "pdgpu_dcd_create": (C.c_int, [C.c_char_p, C.c_int32]),
         "pdgpu_dcd_append": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
+        "pdgpu_traj_dcd": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
         "pdgpu_bondorder_build": (C.c_int, [C.c_int32, _ip, C.c_int32, _ip, _ip, C.c_int32, _ip, _ip, C.c_int32,
                                             _ip, _ip, _ip, _bp, _ip, _ip, _ip, _ip]),
         "pdgpu_fixed_terms": (C.c_int, [C.c_int32, _dp, C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _ip, _dp,
@@ -531,6 +532,12 @@ class Context:
 
     def dcd_append(self, path, iid):
         self._check(lib.pdgpu_dcd_append(self.h, iid, str(path).encode()), "dcd_append")
+
+    def traj_dcd(self, iid, path, every):
+        """Attach (path, every > 0) or detach (path None) a DCD trajectory
+        written by the PD MD calls every `every` steps."""
+        self._check(lib.pdgpu_traj_dcd(self.h, iid, None if path is None else str(path).encode(), int(every)),
+                    "traj_dcd")
 
     def profile(self, enable=True):
         self._check(lib.pdgpu_profile(self.h, int(enable)), "profile")

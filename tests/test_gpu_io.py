@@ -55,3 +55,34 @@ def test_snapshot_mime_of_device_state(name):
     a = np.frombuffer(rec[128:128 + 72 * n], np.float64).reshape(n, 9)
     assert np.array_equal(a[:, :3], g["md_xyz"]) and np.array_equal(a[:, 3:6], g["md_v"])
     assert rms_rel(a[:, 6:], g["io_force"]) <= 1e-9
+
+
+@pytest.mark.parametrize("name", [c for c in IO_CASES if "md_dcd" in G.load(c)])
+def test_md_trajectory_matches_reference(name, tmp_path):
+    """PD's VelocityVerlet run with UpdateTra = 10 and an OutTra_NAMD in the
+    workspace: the device run writes the same frames (steps 10, 20, ...),
+    to FP64 rounding of the positions (the float frames then agree to a few
+    ulp); header bytes as in the DCD file test."""
+    g = G.load(name)
+    n = len(g["mass"])
+    steps = int(g["md_steps"])
+    path = tmp_path / "md.dcd"
+    with context_for(g, pdgpu.FP64) as ctx:
+        i = ctx.add_system(g, xyz=g["md_xyz0"], bonded=True)
+        ctx.set_velocities(i, g["md_v0"].reshape(-1, 3))
+        ctx.traj_dcd(i, path, 10)
+        ctx.md_vv_pd(i, steps, float(g["md_dt"]), int(g["md_update_nlist"]))
+        ctx.traj_dcd(i, None, 0)
+    ours = path.read_bytes()
+    ref = g["md_dcd"].view(np.uint8).tobytes()
+    frames = (steps - 1) // 10
+    assert len(ours) == len(ref) == 276 + frames * 3 * (8 + 4 * n)
+    keep = np.ones(276, bool)
+    keep[TITLE_GARBAGE] = False
+    assert np.array_equal(np.frombuffer(ours[:276], np.uint8)[keep], np.frombuffer(ref[:276], np.uint8)[keep])
+    a = np.frombuffer(ours[276:], np.uint8).reshape(frames, 3, 8 + 4 * n)
+    b = np.frombuffer(ref[276:], np.uint8).reshape(frames, 3, 8 + 4 * n)
+    assert np.array_equal(a[:, :, :4], b[:, :, :4]) and np.array_equal(a[:, :, -4:], b[:, :, -4:])
+    fa = a[:, :, 4:-4].copy().view(np.float32)
+    fb = b[:, :, 4:-4].copy().view(np.float32)
+    assert np.abs(fa - fb).max() <= 1e-6 * max(1.0, np.abs(fb).max())
