@@ -335,13 +335,13 @@ def run_serving(args, pdgpu, mols):
     out = {"workload": f"{nreq} minimisation requests per rate, sizes log-uniform 20-400 residues "
                        f"(mean {int(np.mean([m['n'] for m in mols]))} atoms), steepest descent until force "
                        "RMS < 0.1 kcal/mol/A or 1000 steps, 10 A cutoff + Still GB + bonded, mixed precision",
-           "memory_budget_bytes": args.serve_budget, "steps_per_tick": 10,
+           "memory_budget_bytes": args.serve_budget, "steps_per_tick": 25,
            "list_update": "rebuild per instance when an atom moved > padding/2", "rates": []}
     for rate in args.serve_rates:
         arrivals = serve.poisson_arrivals(rate, nreq, seed=int(rate))
         reqs = [serve.Request(k, m, float(t)) for k, (m, t) in enumerate(zip(mols, arrivals))]
         srv = serve.MinimisationServer(opts, memory_budget=args.serve_budget, max_steps=1000, grms_tol=0.1,
-                                       steps_per_tick=10, keep_xyz=False)
+                                       steps_per_tick=25, keep_xyz=False)
         t0 = time.perf_counter()
         done = srv.run(reqs)
         wall = time.perf_counter() - t0
