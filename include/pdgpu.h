@@ -86,7 +86,9 @@ typedef struct pdgpu_options {
                                   /* candidate pairs are evaluated densely (lane   */
                                   /* rotation, no lists); 0: 3584 x (list radius / */
                                   /* cutoff)^3, i.e. ~7/8 of the slots in cutoff   */
-    int32_t reserved;
+    int32_t langevin_on_hydrogens; /* LangevinOnHydrogens (protocols/md.h);      */
+                                  /* default 1; 0: hydrogens flagged with        */
+                                  /* pdgpu_set_hydrogens move by plain kicks     */
     /* FF_SoftVDW (forcefields/ffsoftvdw.h:39-47), used with PDGPU_FF_SOFTVDW  */
     double softvdw_hardness;      /* Hardness, kcal/mol (default 12)               */
     double softvdw_olap_offset;   /* OlapOffset, A (default 0.6)                   */
@@ -292,6 +294,13 @@ int pdgpu_md_vv_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t
  * noise = [steps][natoms][6] doubles (n1x, n1y, n1z, n2x, n2y, n2z), e.g.
  * the reference's libc rand() stream, for a step-for-step reproduction of
  * a PD run. */
+/* Particle::isHydrogen flags of an instance's atoms (natoms bytes, 0/1;
+ * system/fundamentals.h:223), used by the Langevin update when
+ * langevin_on_hydrogens is 0 (applyForces_LangevinIntegration_NonHydrogens,
+ * protocols/md.cpp:1049-1195: hydrogens take PD's plain half kicks and draw
+ * no deviates). Zero when an instance is added. */
+int pdgpu_set_hydrogens(pdgpu_ctx *ctx, int32_t id, const uint8_t *is_hydrogen);
+
 int pdgpu_md_langevin_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
                          double temperature, double friction, uint64_t seed, const double *noise,
                          pdgpu_energies *energies, double *ekin);

@@ -1007,14 +1007,17 @@ int orc_langevin_coef(double gamma, double dt, double *c0, double *c1, double *c
  * (md.cpp:1017-1019), each nrand being two frand() = (1 + rand()) /
  * (RAND_MAX + 1) through Box-Muller at -2 pi (maths/maths.h:138-152,
  * maths/maths.cpp:256-261), after srand(seed). Written per step and atom as
- * (n1x, n1y, n1z, n2x, n2y, n2z); the last step's slots are zero. */
-void orc_langevin_noise(unsigned seed, int natoms, int steps, double *out) {
+ * (n1x, n1y, n1z, n2x, n2y, n2z); the last step's slots are zero, and so
+ * are those of atoms with skip[i] set (hydrogens when LangevinOnHydrogens
+ * is off: applyForces_LangevinIntegration_NonHydrogens draws nothing for
+ * them, md.cpp:1160-1190). */
+void orc_langevin_noise(unsigned seed, int natoms, int steps, const int8_t *skip, double *out) {
     srand(seed);
     const double two_pi = 2.0 * kPI;
     for (int st = 0; st < steps; ++st) {
         for (int i = 0; i < natoms; ++i) {
             double *o = out + ((size_t)st * natoms + i) * 6;
-            if (st == steps - 1) {
+            if (st == steps - 1 || (skip && skip[i])) {
                 for (int k = 0; k < 6; ++k) o[k] = 0.0;
                 continue;
             }
@@ -1034,8 +1037,10 @@ void orc_langevin_noise(unsigned seed, int natoms, int steps, double *out) {
     }
 }
 
-/* MolecularDynamics::run_core with Integrator = Langevin and
- * LangevinOnHydrogens (PD's default): per step, list refresh, forces, then
+/* MolecularDynamics::run_core with Integrator = Langevin: with hydrogen ==
+ * NULL LangevinOnHydrogens (PD's default), otherwise atoms with hydrogen[i]
+ * set move by PD's plain half kicks (applyForces_LangevinIntegration_
+ * NonHydrogens, md.cpp:1049-1195). Per step, list refresh, forces, then
  * (step > 0) v += (1 - c0/c1) f dt/m / (gamma dt); ekin; (step < steps-1)
  * dr = dt (c1 v + c2 f dt/m) + sigma_r n1, v = c0 v + c0 c2 f dt/m / c1 +
  * sigma_v (crv n1 + sqrt(1 - crv^2) n2) with sigma_v^2 = kB T/m varv and
@@ -1043,7 +1048,7 @@ void orc_langevin_noise(unsigned seed, int natoms, int steps, double *out) {
  * by orc_langevin_noise. Below gamma dt = 1e-5 the VelocityVerlet update. */
 int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p, int steps, double dt,
                        int update_nlist, double temperature, double gamma, const double *noise,
-                       double *trace_epot, double *trace_ekin) {
+                       const int8_t *hydrogen, double *trace_epot, double *trace_ekin) {
     double c0, c1, c2, varv, varr, crv;
     if (!orc_langevin_coef(gamma, dt, &c0, &c1, &c2, &varv, &varr, &crv))
         return orc_md_vv_pd(s, xyz, vel, p, steps, dt, update_nlist, trace_epot, trace_ekin);
@@ -1061,6 +1066,11 @@ int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_
         eval_forces(s, xyz, &nl, p, f, &e);
         if (step > 0)
             for (int i = 0; i < n; ++i) {
+                if (hydrogen && hydrogen[i]) { /* plain VV half kick (md.cpp:1125-1136) */
+                    const double th = 0.5 * dt / s->mass[i];
+                    for (int c = 0; c < 3; ++c) vel[3 * i + c] += f[3 * i + c] * th;
+                    continue;
+                }
                 const double tinvmass = dt / s->mass[i];
                 for (int c = 0; c < 3; ++c) vel[3 * i + c] += (1.0 - c0 / c1) * f[3 * i + c] * tinvmass / gt;
             }
@@ -1069,6 +1079,17 @@ int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_
             ekin += 0.5 * s->mass[i] * sqr(sqrt(sqr(vel[3 * i]) + sqr(vel[3 * i + 1]) + sqr(vel[3 * i + 2])));
         if (step < steps - 1)
             for (int i = 0; i < n; ++i) {
+                if (hydrogen && hydrogen[i]) { /* md.cpp:1163-1180 */
+                    const double th = 0.5 * dt / s->mass[i];
+                    for (int c = 0; c < 3; ++c) {
+                        const double dv = f[3 * i + c] * th;
+                        double dr = dt * (vel[3 * i + c] + dv);
+                        dr *= 1 / kAngstrom;
+                        xyz[3 * i + c] += dr;
+                        vel[3 * i + c] += dv;
+                    }
+                    continue;
+                }
                 const double tinvmass = dt / s->mass[i];
                 const double kTdivm = kKB * temperature / s->mass[i];
                 const double sv = sqrt(kTdivm * varv), sr = sqrt(kTdivm * varr) / gamma;

@@ -186,15 +186,15 @@ int orc_langevin_coef(double gamma, double dt, double *c0, double *c1, double *c
                       double *crv);
 
 /* The reference's normal deviates after srand(seed): [steps][natoms][6] as
- * (n1x, n1y, n1z, n2x, n2y, n2z), last step zero. */
-void orc_langevin_noise(unsigned seed, int natoms, int steps, double *out);
+ * (n1x, n1y, n1z, n2x, n2y, n2z), last step and skipped atoms zero. */
+void orc_langevin_noise(unsigned seed, int natoms, int steps, const int8_t *skip, double *out);
 
-/* MolecularDynamics::run_core, Integrator = Langevin (LangevinOnHydrogens),
- * constant target temperature [K], friction gamma [1/s], noise as from
- * orc_langevin_noise. */
+/* MolecularDynamics::run_core, Integrator = Langevin, constant target
+ * temperature [K], friction gamma [1/s], noise as from orc_langevin_noise;
+ * hydrogen NULL: LangevinOnHydrogens, else those atoms on plain kicks. */
 int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p, int steps, double dt,
                        int update_nlist, double temperature, double gamma, const double *noise,
-                       double *trace_epot, double *trace_ekin);
+                       const int8_t *hydrogen, double *trace_epot, double *trace_ekin);
 
 /* MolecularDynamics::run_core, Integrator = Beeman, no thermostat; f_old
  * (N, natoms*3) in/out. */

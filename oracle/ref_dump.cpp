@@ -468,8 +468,10 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
         // VelocityVerlet, lg_* Langevin (PD's defaults: 10/ps friction, 300 K,
         // LangevinOnHydrogens; libc rand() after srand(lg_seed)), bm_* Beeman
         // (the workspace's old forces zeroed first), lx_* Langevin at 100/ps
-        // (gamma dt = 0.1: the exponential form of the coefficients).
-        for (int integ = 0; integ < 4; ++integ) {
+        // (gamma dt = 0.1: the exponential form of the coefficients), lh_*
+        // Langevin with LangevinOnHydrogens = false (hydrogens on plain
+        // kicks; their flags as is_h).
+        for (int integ = 0; integ < 5; ++integ) {
             System sys(ffps);
             sys.add(NewProtein(ffps, pd_grammar(c.seq)));
             WorkSpace ws(sys);
@@ -510,11 +512,12 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
             Protocol::MolecularDynamics md(ff);
             const char *tag = "md";
             md.Integrator = Protocol::MolecularDynamics::VelocityVerlet;
-            const unsigned lg_seed = integ == 3 ? 4321u : 1234u;
-            if (integ == 1 || integ == 3) {
-                tag = integ == 1 ? "lg" : "lx";
+            const unsigned lg_seed = integ == 3 ? 4321u : integ == 4 ? 999u : 1234u;
+            if (integ == 1 || integ == 3 || integ == 4) {
+                tag = integ == 1 ? "lg" : integ == 3 ? "lx" : "lh";
                 md.Integrator = Protocol::MolecularDynamics::Langevin;
                 if (integ == 3) md.FricCoeff = 1e14;
+                if (integ == 4) md.LangevinOnHydrogens = false;
             } else if (integ == 2) {
                 tag = "bm";
                 md.Integrator = Protocol::MolecularDynamics::Beeman;
@@ -527,7 +530,7 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
             md.UpdateScr = 0;
             md.UpdateTra = 0;
             md.OutputLevel = Verbosity::Silent;
-            if (integ == 1 || integ == 3) srand(lg_seed);
+            if (integ == 1 || integ == 3 || integ == 4) srand(lg_seed);
             // the VelocityVerlet run also writes a NAMD trajectory every 10
             // steps through the workspace's trajectory container (md.cpp:247)
             IO::OutTra_NAMD tra_md(outdir + "/" + c.name + "_mdtra", ws);
@@ -557,7 +560,12 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
             r.d(t + "_v", vel);
             r.s(t + "_epot", ws.ene.epot);
             r.s(t + "_ekin", ws.ene.ekin);
-            if (integ == 1 || integ == 3) {
+            if (integ == 4) {
+                std::vector<int8_t> h;
+                for (int i = 0; i < n; ++i) h.push_back(ws.atom[i].isHydrogen() ? 1 : 0);
+                r.b("is_h", h);
+            }
+            if (integ == 1 || integ == 3 || integ == 4) {
                 r.s(t + "_seed", (double)lg_seed);
                 r.s(t + "_temp", 300.0);
                 r.s(t + "_fric", md.FricCoeff);

@@ -154,6 +154,7 @@ struct pdgpu_ctx {
     // per instance
     DevBuf any_need, inst_need, inst_moved, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf fox, foy, foz; // Beeman's old forces (N), per atom
+    DevBuf is_h;          // Particle::isHydrogen per atom (Langevin without hydrogens)
     DevBuf rng_ctr;       // Langevin device-noise step counter
     DevBuf stage;         // packing buffer of the output formats
     // DCD trajectories attached to instances (pdgpu_traj_dcd): frames are
@@ -356,6 +357,7 @@ int grow_atoms(pdgpu_ctx *c, int64_t blocks) {
                     &c->acc_fz, &c->fx, &c->fy, &c->fz, &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv,
                     &c->opx, &c->opy, &c->opz, &c->rx, &c->ry, &c->rz, &c->fox, &c->foy, &c->foz};
     for (DevBuf *b : d8) TRY(resize(c, *b, na, 8));
+    TRY(resize(c, c->is_h, na, 1));
     TRY(resize(c, c->lj64, na, 16));
     TRY(resize(c, c->gb64, na, 16));
     TRY(resize(c, c->born, na, c->mixed ? 4 : 8));
@@ -1003,7 +1005,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
                      &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
                      &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot, &c->rx, &c->ry, &c->rz,
-                     &c->inst_moved, &c->any_need, &c->fox, &c->foy, &c->foz, &c->rng_ctr, &c->stage, &c->upl};
+                     &c->inst_moved, &c->any_need, &c->fox, &c->foy, &c->foz, &c->rng_ctr, &c->stage, &c->upl, &c->is_h};
     for (DevBuf *b : all) release(*b);
     for (auto &t : c->traj) release(t.second.ring);
     for (auto &g : c->graphs)
@@ -1147,6 +1149,7 @@ extern "C" int pdgpu_instance_add(pdgpu_ctx *c, const pdgpu_topology *t, int32_t
     }
     DevBuf *zs[] = {&c->px, &c->py, &c->pz, &c->vx, &c->vy, &c->vz, &c->fox, &c->foy, &c->foz};
     for (DevBuf *b : zs) up_zero(c, (double *)b->p + a0, na * 8);
+    up_zero(c, (uint8_t *)c->is_h.p + a0, na);
     // per block
     std::vector<int> bi(nb, slot), bn(nb), bf(nb, (int)b0);
     for (int k = 0; k < nb; ++k) bn[k] = std::min(kBlk, n - k * kBlk);
@@ -1326,6 +1329,16 @@ extern "C" int pdgpu_bonded_set(pdgpu_ctx *c, int32_t id, const pdgpu_bonded *b)
 extern "C" int64_t pdgpu_bonded_bytes(int32_t nbonds, int32_t nangles, int32_t ntorsions, int32_t nimpropers) {
     if (nbonds < 0 || nangles < 0 || ntorsions < 0 || nimpropers < 0) return -1;
     return 32 * (int64_t)nbonds + 32 * (int64_t)nangles + 116 * ((int64_t)ntorsions + nimpropers);
+}
+
+extern "C" int pdgpu_set_hydrogens(pdgpu_ctx *c, int32_t id, const uint8_t *flags) {
+    if (!c || !valid_id(c, id) || !flags) return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    const Instance &I = c->inst[id];
+    std::vector<uint8_t> h(flags, flags + I.natoms);
+    for (auto &v : h) v = v ? 1 : 0;
+    up_put(c, (uint8_t *)c->is_h.p + (size_t)I.block0 * kBlk, h.data(), h.size());
+    return up_commit(c);
 }
 
 extern "C" int pdgpu_set_positions(pdgpu_ctx *c, int32_t id, const double *xyz) {

@@ -316,7 +316,8 @@ __device__ __forceinline__ void langevin_normals(uint2 key, unsigned atom, unsig
 __global__ void k_langevin_pd(DevState S, InstMD *md, const double *minv, double *vx, double *vy, double *vz,
                               double *px, double *py, double *pz, int atom_begin, int atom_end, double dt,
                               double f_to_n, LangevinCoef L, int first, int last, int want_ekin,
-                              const double *noise, const unsigned long long *ctr, uint2 key) {
+                              const double *noise, const unsigned long long *ctr, uint2 key,
+                              const uint8_t *is_h, int on_h) {
     const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
     const int b = a / kBlk;
     const bool in = a < atom_end;
@@ -327,13 +328,27 @@ __global__ void k_langevin_pd(DevState S, InstMD *md, const double *minv, double
         const double im = minv[a], tm = dt * im;
         const double f[3] = {S.fx[a] * f_to_n, S.fy[a] * f_to_n, S.fz[a] * f_to_n};
         double u[3] = {vx[a], vy[a], vz[a]};
+        // LangevinOnHydrogens off: hydrogens take PD's plain half kicks
+        // and draw no deviates (md.cpp:1125-1180)
+        const bool plain = !on_h && is_h[a];
         if (!first)
-            for (int c = 0; c < 3; ++c) u[c] += (1.0 - L.c0 / L.c1) * f[c] * tm / L.gt;
+            for (int c = 0; c < 3; ++c)
+                u[c] += plain ? f[c] * (0.5 * dt * im) : (1.0 - L.c0 / L.c1) * f[c] * tm / L.gt;
         if (im > 0) {
             const double m = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
             ek = 0.5 / im * (m * m);
         }
-        if (!last) {
+        if (!last && plain) {
+            double p[3] = {px[a], py[a], pz[a]};
+            for (int c = 0; c < 3; ++c) {
+                const double dv = f[c] * (0.5 * dt * im);
+                double dr = dt * (u[c] + dv);
+                dr *= 1.0 / 1e-10;
+                p[c] += dr;
+                u[c] += dv;
+            }
+            px[a] = p[0]; py[a] = p[1]; pz[a] = p[2];
+        } else if (!last) {
             const int local = a - S.blk_first[b] * kBlk;
             double n[6];
             if (noise) {

@@ -42,6 +42,7 @@ extern "C" void pdgpu_options_default(pdgpu_options *o) {
     o->max_instances = 0;
     o->tiles_per_wave = 0;
     o->components = PDGPU_FF_ALL;
+    o->langevin_on_hydrogens = 1; // MolecularDynamics::settodefault (protocols/md.cpp:59)
     o->use_graphs = 0; // measured no faster: graph replay leaves ~10 us between kernel nodes
     // FF_SoftVDW::settodefault (forcefields/ffsoftvdw.h:39-47)
     o->softvdw_hardness = 12.0;

@@ -48,7 +48,7 @@ EXPORTS = (
     "pdgpu_profile", "pdgpu_profile_read", "pdgpu_sd_start", "pdgpu_sd_advance", "pdgpu_sd_finish",
     "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune", "pdgpu_md_vv_pd", "pdgpu_md_langevin_pd",
     "pdgpu_md_beeman_pd", "pdgpu_mime_encode", "pdgpu_snapshot_mime", "pdgpu_dcd_header", "pdgpu_dcd_frame",
-    "pdgpu_dcd_create", "pdgpu_dcd_append", "pdgpu_traj_dcd",
+    "pdgpu_dcd_create", "pdgpu_dcd_append", "pdgpu_traj_dcd", "pdgpu_set_hydrogens",
 )
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
@@ -96,7 +96,7 @@ class Options(C.Structure):
                    "p1", "p2", "p3", "p4", "p5", "nlist_padding")] + \
                [("device_memory_budget", C.c_int64), ("max_instances", C.c_int32), ("components", C.c_int32),
                 ("use_graphs", C.c_int32), ("dense_tile_pairs", C.c_int32),
-                ("reserved", C.c_int32), ("softvdw_hardness", C.c_double), ("softvdw_olap_offset", C.c_double),
+                ("langevin_on_hydrogens", C.c_int32), ("softvdw_hardness", C.c_double), ("softvdw_olap_offset", C.c_double),
                 ("softvdw_cutoff", C.c_double)]
 
 
@@ -169,6 +169,7 @@ def _load():
         "pdgpu_dcd_create": (C.c_int, [C.c_char_p, C.c_int32]),
         "pdgpu_dcd_append": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
         "pdgpu_traj_dcd": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
+        "pdgpu_set_hydrogens": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
         "pdgpu_bondorder_build": (C.c_int, [C.c_int32, _ip, C.c_int32, _ip, _ip, C.c_int32, _ip, _ip, C.c_int32,
                                             _ip, _ip, _ip, _bp, _ip, _ip, _ip, _ip]),
         "pdgpu_fixed_terms": (C.c_int, [C.c_int32, _dp, C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _ip, _dp,
@@ -532,6 +533,14 @@ class Context:
 
     def dcd_append(self, path, iid):
         self._check(lib.pdgpu_dcd_append(self.h, iid, str(path).encode()), "dcd_append")
+
+    def set_hydrogens(self, iid, flags):
+        """Particle::isHydrogen flags of instance iid (used by Langevin when
+        the options' langevin_on_hydrogens is 0)."""
+        f = np.ascontiguousarray(flags, dtype=np.uint8)
+        if f.size != self._natoms[iid]:
+            raise ValueError("one flag per atom")
+        self._check(lib.pdgpu_set_hydrogens(self.h, iid, f.tobytes()), "set_hydrogens")
 
     def traj_dcd(self, iid, path, every):
         """Attach (path, every > 0) or detach (path None) a DCD trajectory

@@ -94,10 +94,10 @@ lib.orc_md_vv_pd.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c
 lib.orc_langevin_coef.restype = C.c_int
 lib.orc_langevin_coef.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
 lib.orc_langevin_noise.restype = None
-lib.orc_langevin_noise.argtypes = [C.c_uint, C.c_int, C.c_int, _dp]
+lib.orc_langevin_noise.argtypes = [C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_int8), _dp]
 lib.orc_md_langevin_pd.restype = C.c_int
 lib.orc_md_langevin_pd.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
-                                   C.c_double, C.c_double, _dp, _dp, _dp]
+                                   C.c_double, C.c_double, _dp, C.POINTER(C.c_int8), _dp, _dp]
 lib.orc_md_beeman_pd.restype = C.c_int
 lib.orc_md_beeman_pd.argtypes = [C.POINTER(System), _dp, _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                                  _dp, _dp]
@@ -308,22 +308,32 @@ def langevin_coef(gamma, dt):
     return (br,) + tuple(x.value for x in v)
 
 
-def langevin_noise(seed, natoms, steps):
-    """The reference's normal deviates after srand(seed), [steps][natoms][6]."""
+def _i8(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int8)
+
+
+def langevin_noise(seed, natoms, steps, skip=None):
+    """The reference's normal deviates after srand(seed), [steps][natoms][6]
+    (atoms with skip set draw none)."""
     out = np.zeros((steps, natoms, 6))
-    lib.orc_langevin_noise(seed, natoms, steps, _ptr(out, _dp))
+    sk = _i8(skip)
+    lib.orc_langevin_noise(seed, natoms, steps, None if sk is None else _ptr(sk, C.POINTER(C.c_int8)),
+                           _ptr(out, _dp))
     return out
 
 
-def md_langevin_pd(sysm, xyz, vel, params, steps, dt, temperature, gamma, noise, update_nlist=10):
-    """PD's MolecularDynamics with its Langevin integrator."""
+def md_langevin_pd(sysm, xyz, vel, params, steps, dt, temperature, gamma, noise, update_nlist=10, hydrogen=None):
+    """PD's MolecularDynamics with its Langevin integrator (hydrogen: the
+    LangevinOnHydrogens = false variant's hydrogen flags)."""
+    hy = _i8(hydrogen)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).copy()
     vel = np.ascontiguousarray(vel, dtype=np.float64).copy()
     noise = np.ascontiguousarray(noise, dtype=np.float64)
     te = np.zeros(steps)
     tk = np.zeros(steps)
     rc = lib.orc_md_langevin_pd(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), C.byref(params), steps, dt,
-                                update_nlist, temperature, gamma, _ptr(noise, _dp), _ptr(te, _dp), _ptr(tk, _dp))
+                                update_nlist, temperature, gamma, _ptr(noise, _dp),
+                                None if hy is None else _ptr(hy, C.POINTER(C.c_int8)), _ptr(te, _dp), _ptr(tk, _dp))
     return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk
 
 

@@ -28,18 +28,22 @@ BM_CASES = [c for c in CASES if "bm_xyz" in G.load(c)]
 TOL = {pdgpu.FP64: (1e-8, 1e-7, 1e-9), pdgpu.MIXED: (1e-3, 1e-2, 1e-4)}
 
 
-@pytest.mark.parametrize("tag", ["lg", "lx"])
+@pytest.mark.parametrize("tag", ["lg", "lx", "lh"])
 @pytest.mark.parametrize("name", LG_CASES)
 @pytest.mark.parametrize("precision", [pdgpu.FP64, pdgpu.MIXED])
 def test_pd_langevin_matches_reference(name, precision, tag):
     """lg: PD's default friction (Taylor coefficients); lx: 100/ps
-    (exponential coefficients)."""
+    (exponential coefficients); lh: LangevinOnHydrogens = false, hydrogens
+    flagged as PD flags them."""
     g = G.load(name)
     steps = int(g["md_steps"])
     tx, tv, te = TOL[precision]
-    noise = O.langevin_noise(int(g[tag + "_seed"]), len(g["mass"]), steps)
-    with context_for(g, precision) as ctx:
+    hyd = g["is_h"] if tag == "lh" else None
+    noise = O.langevin_noise(int(g[tag + "_seed"]), len(g["mass"]), steps, skip=hyd)
+    with context_for(g, precision, langevin_on_hydrogens=0 if tag == "lh" else 1) as ctx:
         i = ctx.add_system(g, xyz=g["md_xyz0"], bonded=True)
+        if hyd is not None:
+            ctx.set_hydrogens(i, hyd)
         ctx.set_velocities(i, g["md_v0"].reshape(-1, 3))
         e, ek = ctx.md_langevin_pd(i, steps, float(g["md_dt"]), float(g[tag + "_temp"]), float(g[tag + "_fric"]),
                                    noise=noise, update_nlist=int(g["md_update_nlist"]))

@@ -96,23 +96,26 @@ def test_noise_is_standard_normal_and_in_call_order():
     assert not np.array_equal(a, c)
 
 
-@pytest.mark.parametrize("tag", ["lg", "lx"])
+@pytest.mark.parametrize("tag", ["lg", "lx", "lh"])
 @pytest.mark.parametrize("name", LG_CASES)
 def test_pd_langevin_matches_reference(name, tag):
-    """PD's MolecularDynamics with Integrator = Langevin (300 K,
-    LangevinOnHydrogens; lg: PD's default friction 10/ps, the Taylor
-    coefficients; lx: 100/ps, the exponential ones), from the same start,
-    with the reference's own libc rand() stream."""
+    """PD's MolecularDynamics with Integrator = Langevin (300 K; lg: PD's
+    default friction 10/ps, the Taylor coefficients; lx: 100/ps, the
+    exponential ones; lh: 10/ps with LangevinOnHydrogens = false), from the
+    same start, with the reference's own libc rand() stream."""
     d = G.load(name)
     assert np.array_equal(d[tag + "_xyz0"], d["md_xyz0"])
     s = _system(d)
     p = O.default_params(**d["param_dict"])
     steps = int(d["md_steps"])
-    assert O.langevin_coef(float(d[tag + "_fric"]), float(d["md_dt"]))[0] == (2 if tag == "lg" else 1)
-    noise = O.langevin_noise(int(d[tag + "_seed"]), len(d["mass"]), steps)
+    assert O.langevin_coef(float(d[tag + "_fric"]), float(d["md_dt"]))[0] == (1 if tag == "lx" else 2)
+    hyd = d["is_h"] if tag == "lh" else None  # lh: LangevinOnHydrogens = false
+    if hyd is not None:
+        assert 0 < hyd.sum() < len(hyd)
+    noise = O.langevin_noise(int(d[tag + "_seed"]), len(d["mass"]), steps, skip=hyd)
     rc, x, v, te, tk = O.md_langevin_pd(s, d["md_xyz0"], d["md_v0"], p, steps, float(d["md_dt"]),
                                         float(d[tag + "_temp"]), float(d[tag + "_fric"]), noise,
-                                        int(d["md_update_nlist"]))
+                                        int(d["md_update_nlist"]), hydrogen=hyd)
     assert rc == steps
     assert np.abs(x - d[tag + "_xyz"]).max() <= 1e-9
     assert np.abs(v - d[tag + "_v"]).max() <= 1e-9 * np.abs(d[tag + "_v"]).max()
