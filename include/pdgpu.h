@@ -294,6 +294,17 @@ int pdgpu_md_vv_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t
  * noise = [steps][natoms][6] doubles (n1x, n1y, n1z, n2x, n2y, n2z), e.g.
  * the reference's libc rand() stream, for a step-for-step reproduction of
  * a PD run. */
+/* MolecularDynamics::Thermostat = Berendsen for the PD MD calls
+ * (pdgpu_md_vv_pd, pdgpu_md_beeman_pd; applyBerendsenThermostat,
+ * protocols/md.cpp:580-597, applied after each step's kinetic energy as
+ * applyThermostat is, md.cpp:816 and :874): every velocity of an instance
+ * times sqrt(1 + dt (3 N kB T / 2 / ekin - 1) / tau). tau in s (PD's
+ * BerendsenTau, default 1e-13), temperature in K; tau = 0 turns it off.
+ * PD also removes the total momenta every 200 steps while a thermostat is
+ * on; that is not built, so such runs are limited to 200 steps
+ * (PDGPU_ERR_UNSUPPORTED beyond). */
+int pdgpu_set_thermostat(pdgpu_ctx *ctx, double tau, double temperature);
+
 /* Particle::isHydrogen flags of an instance's atoms (natoms bytes, 0/1;
  * system/fundamentals.h:223), used by the Langevin update when
  * langevin_on_hydrogens is 0 (applyForces_LangevinIntegration_NonHydrogens,

@@ -922,8 +922,18 @@ int orc_md_vv(const orc_system *s, double *xyz, double *vel, const orc_params *p
  * then adds it again, moves p by dt*(v + f*dt/2m) and adds it a third time.
  * No thermostat or barostat (PD's defaults); momentum removal only acts with
  * a thermostat (md.cpp:246-258). */
+/* MolecularDynamics::applyBerendsenThermostat (md.cpp:580-597): every
+ * velocity times sqrt(1 + dt (Edes / ekin - 1) / tau), Edes = 3 N kB T / 2. */
+static void berendsen(int n, double *vel, double ekin, double dt, double tau, double temperature) {
+    if (!(tau > 0)) return;
+    const double edes = 3.0 * n * kKB * temperature / 2.0;
+    const double chi = sqrt(1.0 + dt * (edes / ekin - 1.0) / tau);
+    for (int k = 0; k < 3 * n; ++k) vel[k] *= chi;
+}
+
 int orc_md_vv_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p,
-                 int steps, double dt, int update_nlist, double *trace_epot, double *trace_ekin) {
+                 int steps, double dt, int update_nlist, double tau, double temperature,
+                 double *trace_epot, double *trace_ekin) {
     const int n = s->n;
     orc_nlist nl;
     nl.row_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
@@ -942,6 +952,7 @@ int orc_md_vv_pd(const orc_system *s, double *xyz, double *vel, const orc_params
         }
         for (int i = 0; i < n; ++i) /* calcKineticEnergy, md.cpp:504-518: 0.5 m |v|^2 via mag() */
             ekin += 0.5 * s->mass[i] * sqr(sqrt(sqr(vel[3 * i]) + sqr(vel[3 * i + 1]) + sqr(vel[3 * i + 2])));
+        berendsen(n, vel, ekin, dt, tau, temperature); /* applyThermostat, md.cpp:816 */
         for (int i = 0; i < n; ++i) {
             const double tinvmass = 0.5 * dt / s->mass[i];
             double dv[3], dr[3];
@@ -1051,7 +1062,7 @@ int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_
                        const int8_t *hydrogen, double *trace_epot, double *trace_ekin) {
     double c0, c1, c2, varv, varr, crv;
     if (!orc_langevin_coef(gamma, dt, &c0, &c1, &c2, &varv, &varr, &crv))
-        return orc_md_vv_pd(s, xyz, vel, p, steps, dt, update_nlist, trace_epot, trace_ekin);
+        return orc_md_vv_pd(s, xyz, vel, p, steps, dt, update_nlist, 0.0, 0.0, trace_epot, trace_ekin);
     const double gt = gamma * dt;
     const int n = s->n;
     orc_nlist nl;
@@ -1124,7 +1135,8 @@ int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_
  * f_old) dt/6m) / A, v += (5 f - f_old) dt/6m, f_old = f. f_old (N,
  * natoms*3) is the workspace's old forces and is updated in place. */
 int orc_md_beeman_pd(const orc_system *s, double *xyz, double *vel, double *f_old, const orc_params *p, int steps,
-                     double dt, int update_nlist, double *trace_epot, double *trace_ekin) {
+                     double dt, int update_nlist, double tau, double temperature, double *trace_epot,
+                     double *trace_ekin) {
     const int n = s->n;
     orc_nlist nl;
     nl.row_start = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
@@ -1144,6 +1156,7 @@ int orc_md_beeman_pd(const orc_system *s, double *xyz, double *vel, double *f_ol
         double ekin = 0;
         for (int i = 0; i < n; ++i)
             ekin += 0.5 * s->mass[i] * sqr(sqrt(sqr(vel[3 * i]) + sqr(vel[3 * i + 1]) + sqr(vel[3 * i + 2])));
+        berendsen(n, vel, ekin, dt, tau, temperature); /* applyThermostat, md.cpp:874 */
         if (step < steps - 1)
             for (int i = 0; i < n; ++i) {
                 const double tinvmass = dt / (6.0 * s->mass[i]);

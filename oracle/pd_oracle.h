@@ -175,9 +175,12 @@ int orc_md_vv(const orc_system *s, double *xyz, double *vel, const orc_params *p
  * (protocols/md.cpp:209-233, 788-838): per step, list refresh when
  * step % update_nlist == 0, forces, then v += f*dt/2m three times with the
  * drift dt*(v + f*dt/2m) before the third. trace_ekin receives ene.ekin of
- * each step (taken after the first addition). */
+ * each step (taken after the first addition). tau > 0: the Berendsen
+ * thermostat (applyBerendsenThermostat, md.cpp:580-597) at temperature K
+ * after the kinetic energy, as PD's applyThermostat (md.cpp:816). */
 int orc_md_vv_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p,
-                 int steps, double dt, int update_nlist, double *trace_epot, double *trace_ekin);
+                 int steps, double dt, int update_nlist, double tau, double temperature,
+                 double *trace_epot, double *trace_ekin);
 
 /* A11 Langevin (protocols/md.cpp:906-1043): coefficients for gamma [1/s]
  * and dt [s]; returns 0 (VelocityVerlet), 1 (exponential form) or 2
@@ -196,10 +199,11 @@ int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_
                        int update_nlist, double temperature, double gamma, const double *noise,
                        const int8_t *hydrogen, double *trace_epot, double *trace_ekin);
 
-/* MolecularDynamics::run_core, Integrator = Beeman, no thermostat; f_old
- * (N, natoms*3) in/out. */
+/* MolecularDynamics::run_core, Integrator = Beeman; f_old (N, natoms*3)
+ * in/out; tau > 0: Berendsen thermostat as for orc_md_vv_pd. */
 int orc_md_beeman_pd(const orc_system *s, double *xyz, double *vel, double *f_old, const orc_params *p, int steps,
-                     double dt, int update_nlist, double *trace_epot, double *trace_ekin);
+                     double dt, int update_nlist, double tau, double temperature, double *trace_epot,
+                     double *trace_ekin);
 
 #ifdef __cplusplus
 }

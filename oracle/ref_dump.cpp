@@ -471,7 +471,10 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
         // (gamma dt = 0.1: the exponential form of the coefficients), lh_*
         // Langevin with LangevinOnHydrogens = false (hydrogens on plain
         // kicks; their flags as is_h).
-        for (int integ = 0; integ < 5; ++integ) {
+        // vb_* / bb_*: VelocityVerlet / Beeman with PD's Berendsen thermostat
+        // (300 K, BerendsenTau 100 fs; fewer than UpdateRemoveTotalMomentum
+        // = 200 steps, so no momentum removal happens).
+        for (int integ = 0; integ < 7; ++integ) {
             System sys(ffps);
             sys.add(NewProtein(ffps, pd_grammar(c.seq)));
             WorkSpace ws(sys);
@@ -518,11 +521,14 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
                 md.Integrator = Protocol::MolecularDynamics::Langevin;
                 if (integ == 3) md.FricCoeff = 1e14;
                 if (integ == 4) md.LangevinOnHydrogens = false;
-            } else if (integ == 2) {
-                tag = "bm";
+            } else if (integ == 2 || integ == 6) {
+                tag = integ == 2 ? "bm" : "bb";
                 md.Integrator = Protocol::MolecularDynamics::Beeman;
                 for (int i = 0; i < n; ++i) ws.old.atom[i].f.setTo(0, 0, 0);
+            } else if (integ == 5) {
+                tag = "vb";
             }
+            if (integ == 5 || integ == 6) md.Thermostat = Protocol::MolecularDynamics::Berendsen;
             md.Steps = c.md_steps;
             md.Timestep = 1e-15;
             md.RandVel = false;
@@ -560,6 +566,7 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
             r.d(t + "_v", vel);
             r.s(t + "_epot", ws.ene.epot);
             r.s(t + "_ekin", ws.ene.ekin);
+            if (integ == 5 || integ == 6) r.s(t + "_tau", md.BerendsenTau);
             if (integ == 4) {
                 std::vector<int8_t> h;
                 for (int i = 0; i < n; ++i) h.push_back(ws.atom[i].isHydrogen() ? 1 : 0);

@@ -155,6 +155,8 @@ struct pdgpu_ctx {
     DevBuf any_need, inst_need, inst_moved, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf fox, foy, foz; // Beeman's old forces (N), per atom
     DevBuf is_h;          // Particle::isHydrogen per atom (Langevin without hydrogens)
+    double thermo_tau = 0, thermo_temp = 300; // Berendsen thermostat of the PD MD calls (tau 0: none)
+    double thermo_dt = 0;                     // time step of the running MD call
     DevBuf rng_ctr;       // Langevin device-noise step counter
     DevBuf stage;         // packing buffer of the output formats
     // DCD trajectories attached to instances (pdgpu_traj_dcd): frames are

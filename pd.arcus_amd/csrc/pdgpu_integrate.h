@@ -427,6 +427,85 @@ __global__ void k_beeman_pd(DevState S, InstMD *md, const double *minv, double *
     }
 }
 
+// ------------------------------ Berendsen thermostat (protocols/md.cpp:580-597)
+// PD's applyThermostat runs after the kinetic energy of a step, so a step
+// with the thermostat splits in two kernels: the velocity update up to the
+// kinetic energy (summed per instance), then every velocity times
+// chi = sqrt(1 + dt (Edes / ekin - 1) / tau), Edes = 3 N kB T / 2, and the
+// rest of the update.
+__device__ __forceinline__ double berendsen_chi(double ekin, int natoms, double dt, double tau, double kT) {
+    const double edes = 3.0 * natoms * kT / 2.0;
+    return sqrt(1.0 + dt * (edes / ekin - 1.0) / tau);
+}
+
+// kind 0: VelocityVerlet as PD writes it (first half kick); kind 1: Beeman
+// (finishing kick 2 f dt/6m after the first step). Adds the kinetic energy.
+__global__ void k_md_thermo_a(DevState S, InstMD *md, const double *minv, double *vx, double *vy, double *vz,
+                              int atom_begin, int atom_end, double dt, double f_to_n, int kind, int first) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = a / kBlk;
+    const bool in = a < atom_end;
+    const int inst = in ? S.blk_inst[b] : -1;
+    const bool v = in && inst >= 0 && (a % kBlk) < S.blk_n[b];
+    double ek = 0;
+    if (v) {
+        const double im = minv[a];
+        const double f[3] = {S.fx[a] * f_to_n, S.fy[a] * f_to_n, S.fz[a] * f_to_n};
+        double u[3] = {vx[a], vy[a], vz[a]};
+        for (int c = 0; c < 3; ++c) {
+            if (kind == 0) u[c] += f[c] * (0.5 * dt * im);
+            else if (!first) u[c] += (2.0 * f[c]) * (dt * im / 6.0);
+        }
+        if (im > 0) {
+            const double m = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            ek = 0.5 / im * (m * m);
+        }
+        vx[a] = u[0]; vy[a] = u[1]; vz[a] = u[2];
+    }
+    ek = wave_sum(ek);
+    if (lane_id() == 0 && in && inst >= 0 && ek != 0.0) unsafeAtomicAdd(&md[inst].ekin, ek);
+}
+
+// chi, then the rest of the step: kind 0 PD's VelocityVerlet (v += f dt/2m,
+// p += dt (v + f dt/2m), v += f dt/2m); kind 1 Beeman's update unless last.
+__global__ void k_md_thermo_b(DevState S, const InstMD *md, const int *natoms, const double *minv, double *vx,
+                              double *vy, double *vz, double *px, double *py, double *pz, double *ox, double *oy,
+                              double *oz, int atom_begin, int atom_end, double dt, double f_to_n, double tau,
+                              double kT, int kind, int last) {
+    const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= atom_end) return;
+    const int b = a / kBlk;
+    const int inst = S.blk_inst[b];
+    if (inst < 0 || (a % kBlk) >= S.blk_n[b]) return;
+    const double chi = berendsen_chi(md[inst].ekin, natoms[inst], dt, tau, kT);
+    const double im = minv[a];
+    const double f[3] = {S.fx[a] * f_to_n, S.fy[a] * f_to_n, S.fz[a] * f_to_n};
+    double u[3] = {vx[a] * chi, vy[a] * chi, vz[a] * chi};
+    double p[3] = {px[a], py[a], pz[a]};
+    if (kind == 0) {
+        for (int c = 0; c < 3; ++c) {
+            const double dv = f[c] * (0.5 * dt * im);
+            u[c] += dv;
+            double dr = dt * (u[c] + dv);
+            dr *= 1.0 / 1e-10;
+            p[c] += dr;
+            u[c] += dv;
+        }
+    } else if (!last) {
+        const double tm = dt * im / 6.0;
+        const double fo[3] = {ox[a], oy[a], oz[a]};
+        for (int c = 0; c < 3; ++c) {
+            double dr = dt * (u[c] + (4.0 * f[c] - fo[c]) * tm);
+            dr *= 1.0 / 1e-10;
+            p[c] += dr;
+            u[c] += (5.0 * f[c] - fo[c]) * tm;
+        }
+        ox[a] = f[0]; oy[a] = f[1]; oz[a] = f[2];
+    }
+    vx[a] = u[0]; vy[a] = u[1]; vz[a] = u[2];
+    px[a] = p[0]; py[a] = p[1]; pz[a] = p[2];
+}
+
 __global__ void k_ekin_clear(InstMD *md, int s0, int s1) {
     const int s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (s < s1) md[s].ekin = 0;

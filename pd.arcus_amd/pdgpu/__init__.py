@@ -49,6 +49,7 @@ EXPORTS = (
     "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune", "pdgpu_md_vv_pd", "pdgpu_md_langevin_pd",
     "pdgpu_md_beeman_pd", "pdgpu_mime_encode", "pdgpu_snapshot_mime", "pdgpu_dcd_header", "pdgpu_dcd_frame",
     "pdgpu_dcd_create", "pdgpu_dcd_append", "pdgpu_traj_dcd", "pdgpu_set_hydrogens",
+    "pdgpu_set_thermostat",
 )
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
@@ -170,6 +171,7 @@ def _load():
         "pdgpu_dcd_append": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
         "pdgpu_traj_dcd": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
         "pdgpu_set_hydrogens": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
+        "pdgpu_set_thermostat": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
         "pdgpu_bondorder_build": (C.c_int, [C.c_int32, _ip, C.c_int32, _ip, _ip, C.c_int32, _ip, _ip, C.c_int32,
                                             _ip, _ip, _ip, _bp, _ip, _ip, _ip, _ip]),
         "pdgpu_fixed_terms": (C.c_int, [C.c_int32, _dp, C.c_int32, _ip, _ip, _dp, C.c_int32, _ip, _ip, _dp,
@@ -533,6 +535,10 @@ class Context:
 
     def dcd_append(self, path, iid):
         self._check(lib.pdgpu_dcd_append(self.h, iid, str(path).encode()), "dcd_append")
+
+    def set_thermostat(self, tau, temperature=300.0):
+        """Berendsen thermostat of the PD MD calls (tau in s; 0: off)."""
+        self._check(lib.pdgpu_set_thermostat(self.h, tau, temperature), "set_thermostat")
 
     def set_hydrogens(self, iid, flags):
         """Particle::isHydrogen flags of instance iid (used by Langevin when

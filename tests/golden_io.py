@@ -24,7 +24,7 @@ def load(name):
     d["xyz"] = d["xyz"].reshape(-1, 3)
     for k in ("force", "en_force", "nb_force", "sd_force", "sd_xyz", "bf_force", "bfp_force", "bfp_xyz",
               "ff_force", "sdf_force", "sdf_xyz", "sv_force", "svsd_force", "svsd_xyz", "md_xyz0", "md_v0",
-              "md_xyz", "md_v", "lg_xyz0", "lg_xyz", "lg_v", "bm_xyz0", "bm_xyz", "bm_v", "io_force", "lx_xyz0", "lx_xyz", "lx_v", "lh_xyz0", "lh_xyz", "lh_v"):
+              "md_xyz", "md_v", "lg_xyz0", "lg_xyz", "lg_v", "bm_xyz0", "bm_xyz", "bm_v", "io_force", "lx_xyz0", "lx_xyz", "lx_v", "lh_xyz0", "lh_xyz", "lh_v", "vb_xyz0", "vb_xyz", "vb_v", "bb_xyz0", "bb_xyz", "bb_v"):
         if k in d:
             d[k] = d[k].reshape(-1, 3)
     d["mid"] = int(d["mid"][0])

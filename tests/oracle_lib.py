@@ -90,7 +90,7 @@ lib.orc_softvdw_forces.argtypes = [C.POINTER(System), _dp, C.POINTER(NList), C.c
                                    _dp, C.POINTER(Energies)]
 lib.orc_md_vv_pd.restype = C.c_int
 lib.orc_md_vv_pd.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
-                             _dp, _dp]
+                             C.c_double, C.c_double, _dp, _dp]
 lib.orc_langevin_coef.restype = C.c_int
 lib.orc_langevin_coef.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
 lib.orc_langevin_noise.restype = None
@@ -100,7 +100,7 @@ lib.orc_md_langevin_pd.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params
                                    C.c_double, C.c_double, _dp, C.POINTER(C.c_int8), _dp, _dp]
 lib.orc_md_beeman_pd.restype = C.c_int
 lib.orc_md_beeman_pd.argtypes = [C.POINTER(System), _dp, _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
-                                 _dp, _dp]
+                                 C.c_double, C.c_double, _dp, _dp]
 lib.orc_md_vv.restype = C.c_int
 lib.orc_md_vv.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                           _dp, _dp]
@@ -290,14 +290,14 @@ def md_vv(sysm, xyz, vel, params, steps, dt, update_nlist=10):
     return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk
 
 
-def md_vv_pd(sysm, xyz, vel, params, steps, dt, update_nlist=10):
+def md_vv_pd(sysm, xyz, vel, params, steps, dt, update_nlist=10, tau=0.0, temperature=300.0):
     """PD's MolecularDynamics with its VelocityVerlet update as written."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).copy()
     vel = np.ascontiguousarray(vel, dtype=np.float64).copy()
     te = np.zeros(steps)
     tk = np.zeros(steps)
     rc = lib.orc_md_vv_pd(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), C.byref(params), steps, dt,
-                          update_nlist, _ptr(te, _dp), _ptr(tk, _dp))
+                          update_nlist, tau, temperature, _ptr(te, _dp), _ptr(tk, _dp))
     return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk
 
 
@@ -337,7 +337,7 @@ def md_langevin_pd(sysm, xyz, vel, params, steps, dt, temperature, gamma, noise,
     return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk
 
 
-def md_beeman_pd(sysm, xyz, vel, params, steps, dt, f_old=None, update_nlist=10):
+def md_beeman_pd(sysm, xyz, vel, params, steps, dt, f_old=None, update_nlist=10, tau=0.0, temperature=300.0):
     """PD's MolecularDynamics with its Beeman integrator; f_old (N) in/out."""
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).copy()
     vel = np.ascontiguousarray(vel, dtype=np.float64).copy()
@@ -345,7 +345,7 @@ def md_beeman_pd(sysm, xyz, vel, params, steps, dt, f_old=None, update_nlist=10)
     te = np.zeros(steps)
     tk = np.zeros(steps)
     rc = lib.orc_md_beeman_pd(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), _ptr(fo, _dp), C.byref(params),
-                              steps, dt, update_nlist, _ptr(te, _dp), _ptr(tk, _dp))
+                              steps, dt, update_nlist, tau, temperature, _ptr(te, _dp), _ptr(tk, _dp))
     return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), fo.reshape(-1, 3), te, tk
 
 

@@ -188,3 +188,30 @@ def test_langevin_rejects_bad_noise_shape():
         i = ctx.add_system(g, bonded=True)
         with pytest.raises(ValueError):
             ctx.md_langevin_pd(i, 4, 1e-15, noise=np.zeros((3, len(g["mass"]), 6)))
+
+
+@pytest.mark.parametrize("tag", ["vb", "bb"])
+@pytest.mark.parametrize("name", [c for c in CASES if "vb_xyz" in G.load(c)])
+@pytest.mark.parametrize("precision", [pdgpu.FP64, pdgpu.MIXED])
+def test_pd_berendsen_matches_reference(name, precision, tag):
+    """PD's VelocityVerlet (vb) and Beeman (bb) runs with its Berendsen
+    thermostat (300 K, BerendsenTau 100 fs)."""
+    g = G.load(name)
+    steps = int(g["md_steps"])
+    tx, tv, te = TOL[precision]
+    with context_for(g, precision) as ctx:
+        i = ctx.add_system(g, xyz=g["md_xyz0"], bonded=True)
+        ctx.set_velocities(i, g["md_v0"].reshape(-1, 3))
+        ctx.set_thermostat(float(g[tag + "_tau"]), 300.0)
+        run = ctx.md_vv_pd if tag == "vb" else ctx.md_beeman_pd
+        e, ek = run(i, steps, float(g["md_dt"]), int(g["md_update_nlist"]))
+        assert e["status"] == 0
+        x = ctx.get_positions(i)
+        v = ctx.get_velocities(i)
+        # PD's momentum removal at step 200 is not built: longer runs refuse
+        with pytest.raises(pdgpu.PdgpuError):
+            run(i, 201, float(g["md_dt"]))
+    assert np.abs(x - g[tag + "_xyz"]).max() <= tx
+    assert np.abs(v - g[tag + "_v"]).max() <= tv * np.abs(g[tag + "_v"]).max()
+    assert rel(e["epot"], g[tag + "_epot"]) <= te
+    assert rel(ek, g[tag + "_ekin"]) <= te

@@ -163,3 +163,29 @@ def test_beeman_free_particle_and_constant_force():
     # ten moves (the last step does not move), each dt * v / A
     assert np.allclose(x, x0 + 10 * dt * v0 / 1e-10, rtol=1e-13, atol=0)
     assert np.array_equal(v, v0)
+
+
+@pytest.mark.parametrize("tag", ["vb", "bb"])
+@pytest.mark.parametrize("name", [n for n in CASES if "vb_xyz" in G.load(n)])
+def test_pd_berendsen_matches_reference(name, tag):
+    """PD's VelocityVerlet (vb) and Beeman (bb) runs with its Berendsen
+    thermostat (300 K, BerendsenTau 100 fs), from the same start."""
+    d = G.load(name)
+    assert np.array_equal(d[tag + "_xyz0"], d["md_xyz0"])
+    s = _system(d)
+    p = O.default_params(**d["param_dict"])
+    steps, dt, tau = int(d["md_steps"]), float(d["md_dt"]), float(d[tag + "_tau"])
+    if tag == "vb":
+        rc, x, v, te, tk = O.md_vv_pd(s, d["md_xyz0"], d["md_v0"], p, steps, dt, int(d["md_update_nlist"]),
+                                      tau=tau, temperature=300.0)
+        plain = d["md_xyz"]
+    else:
+        rc, x, v, _, te, tk = O.md_beeman_pd(s, d["md_xyz0"], d["md_v0"], p, steps, dt,
+                                             update_nlist=int(d["md_update_nlist"]), tau=tau, temperature=300.0)
+        plain = d["bm_xyz"]
+    assert rc == steps
+    assert np.abs(x - d[tag + "_xyz"]).max() <= 1e-9
+    assert np.abs(v - d[tag + "_v"]).max() <= 1e-9 * np.abs(d[tag + "_v"]).max()
+    assert rel(te[-1], d[tag + "_epot"]) <= 1e-10
+    assert rel(tk[-1], d[tag + "_ekin"]) <= 1e-10
+    assert np.abs(x - plain).max() > 1e-6  # the thermostat acted
