@@ -273,11 +273,31 @@ int pdgpu_md_vv(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t up
  * protocols/protocolbase.cpp:113-122): each step refreshes the list when
  * step % update_nlist == 0, evaluates forces and applies v += f*dt/2m
  * (ekin taken), v += f*dt/2m, p += dt*(v + f*dt/2m), v += f*dt/2m. No
- * thermostat or barostat (PD's defaults). energies are those of the last
- * force evaluation and ekin the last step's, as in PD's workspace after
- * run(). */
+ * barostat; with pdgpu_set_thermostat(tau > 0) the Berendsen thermostat
+ * scales the velocities after each step's kinetic energy, and such runs are
+ * limited to 200 steps. energies are those of the last force evaluation and
+ * ekin the last step's, as in PD's workspace after run(). */
 int pdgpu_md_vv_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
                    pdgpu_energies *energies, double *ekin);
+
+/* MolecularDynamics::Thermostat = Berendsen for the PD MD calls
+ * (pdgpu_md_vv_pd, pdgpu_md_beeman_pd; applyBerendsenThermostat,
+ * protocols/md.cpp:580-597, applied after each step's kinetic energy as
+ * applyThermostat is, md.cpp:816 and :874): every velocity of an instance
+ * times sqrt(1 + dt (3 N kB T / 2 / ekin - 1) / tau). tau in s (PD's
+ * BerendsenTau, default 1e-13), temperature in K; tau = 0 turns it off.
+ * PD also removes the total momenta every 200 steps while a thermostat is
+ * on, for every integrator (md.cpp:246-258); that is not built, so any MD
+ * call (pdgpu_md_langevin_pd included) runs at most 200 steps while tau > 0
+ * (PDGPU_ERR_UNSUPPORTED beyond). */
+int pdgpu_set_thermostat(pdgpu_ctx *ctx, double tau, double temperature);
+
+/* Particle::isHydrogen flags of an instance's atoms (natoms bytes, 0/1;
+ * system/fundamentals.h:223), used by the Langevin update when
+ * langevin_on_hydrogens is 0 (applyForces_LangevinIntegration_NonHydrogens,
+ * protocols/md.cpp:1049-1195: hydrogens take PD's plain half kicks and draw
+ * no deviates). Zero when an instance is added. */
+int pdgpu_set_hydrogens(pdgpu_ctx *ctx, int32_t id, const uint8_t *is_hydrogen);
 
 /* MolecularDynamics::runcore with Integrator = Langevin and
  * LangevinOnHydrogens (PD's default; protocols/md.cpp:906-1043 update,
@@ -294,30 +314,13 @@ int pdgpu_md_vv_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t
  * noise = [steps][natoms][6] doubles (n1x, n1y, n1z, n2x, n2y, n2z), e.g.
  * the reference's libc rand() stream, for a step-for-step reproduction of
  * a PD run. */
-/* MolecularDynamics::Thermostat = Berendsen for the PD MD calls
- * (pdgpu_md_vv_pd, pdgpu_md_beeman_pd; applyBerendsenThermostat,
- * protocols/md.cpp:580-597, applied after each step's kinetic energy as
- * applyThermostat is, md.cpp:816 and :874): every velocity of an instance
- * times sqrt(1 + dt (3 N kB T / 2 / ekin - 1) / tau). tau in s (PD's
- * BerendsenTau, default 1e-13), temperature in K; tau = 0 turns it off.
- * PD also removes the total momenta every 200 steps while a thermostat is
- * on; that is not built, so such runs are limited to 200 steps
- * (PDGPU_ERR_UNSUPPORTED beyond). */
-int pdgpu_set_thermostat(pdgpu_ctx *ctx, double tau, double temperature);
-
-/* Particle::isHydrogen flags of an instance's atoms (natoms bytes, 0/1;
- * system/fundamentals.h:223), used by the Langevin update when
- * langevin_on_hydrogens is 0 (applyForces_LangevinIntegration_NonHydrogens,
- * protocols/md.cpp:1049-1195: hydrogens take PD's plain half kicks and draw
- * no deviates). Zero when an instance is added. */
-int pdgpu_set_hydrogens(pdgpu_ctx *ctx, int32_t id, const uint8_t *is_hydrogen);
-
 int pdgpu_md_langevin_pd(pdgpu_ctx *ctx, int32_t id, int32_t steps, double dt, int32_t update_nlist,
                          double temperature, double friction, uint64_t seed, const double *noise,
                          pdgpu_energies *energies, double *ekin);
 
 /* MolecularDynamics::runcore with Integrator = Beeman (PD's default
- * integrator; protocols/md.cpp:840-904), no thermostat: after the first
+ * integrator; protocols/md.cpp:840-904; Berendsen as for pdgpu_md_vv_pd
+ * when pdgpu_set_thermostat set tau > 0, at most 200 steps): after the first
  * step v += 2 f dt/6m, ekin taken, and before the last step p += dt (v +
  * (4 f - f_old) dt/6m), v += (5 f - f_old) dt/6m, f_old = f. The old
  * forces (WorkSpace::old in PD) are kept per atom between calls and are

@@ -1056,13 +1056,16 @@ void orc_langevin_noise(unsigned seed, int natoms, int steps, const int8_t *skip
  * dr = dt (c1 v + c2 f dt/m) + sigma_r n1, v = c0 v + c0 c2 f dt/m / c1 +
  * sigma_v (crv n1 + sqrt(1 - crv^2) n2) with sigma_v^2 = kB T/m varv and
  * sigma_r^2 = kB T/m varr / gamma^2 (md.cpp:1003-1041). noise as written
- * by orc_langevin_noise. Below gamma dt = 1e-5 the VelocityVerlet update. */
+ * by orc_langevin_noise. Below gamma dt = 1e-5 the VelocityVerlet update,
+ * with the Berendsen thermostat when tau > 0. */
 int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p, int steps, double dt,
-                       int update_nlist, double temperature, double gamma, const double *noise,
+                       int update_nlist, double temperature, double gamma, double tau, const double *noise,
                        const int8_t *hydrogen, double *trace_epot, double *trace_ekin) {
     double c0, c1, c2, varv, varr, crv;
+    /* PD's fallback is applyForces_VelocityVerletIntegration, which applies
+     * the thermostat (md.cpp:816); the Langevin update itself does not */
     if (!orc_langevin_coef(gamma, dt, &c0, &c1, &c2, &varv, &varr, &crv))
-        return orc_md_vv_pd(s, xyz, vel, p, steps, dt, update_nlist, 0.0, 0.0, trace_epot, trace_ekin);
+        return orc_md_vv_pd(s, xyz, vel, p, steps, dt, update_nlist, tau, temperature, trace_epot, trace_ekin);
     const double gt = gamma * dt;
     const int n = s->n;
     orc_nlist nl;

@@ -194,9 +194,11 @@ void orc_langevin_noise(unsigned seed, int natoms, int steps, const int8_t *skip
 
 /* MolecularDynamics::run_core, Integrator = Langevin, constant target
  * temperature [K], friction gamma [1/s], noise as from orc_langevin_noise;
- * hydrogen NULL: LangevinOnHydrogens, else those atoms on plain kicks. */
+ * hydrogen NULL: LangevinOnHydrogens, else those atoms on plain kicks.
+ * tau (Berendsen, s) only acts in the VelocityVerlet fallback below
+ * gamma dt = 1e-5, as in PD. */
 int orc_md_langevin_pd(const orc_system *s, double *xyz, double *vel, const orc_params *p, int steps, double dt,
-                       int update_nlist, double temperature, double gamma, const double *noise,
+                       int update_nlist, double temperature, double gamma, double tau, const double *noise,
                        const int8_t *hydrogen, double *trace_epot, double *trace_ekin);
 
 /* MolecularDynamics::run_core, Integrator = Beeman; f_old (N, natoms*3)

@@ -97,7 +97,7 @@ lib.orc_langevin_noise.restype = None
 lib.orc_langevin_noise.argtypes = [C.c_uint, C.c_int, C.c_int, C.POINTER(C.c_int8), _dp]
 lib.orc_md_langevin_pd.restype = C.c_int
 lib.orc_md_langevin_pd.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
-                                   C.c_double, C.c_double, _dp, C.POINTER(C.c_int8), _dp, _dp]
+                                   C.c_double, C.c_double, C.c_double, _dp, C.POINTER(C.c_int8), _dp, _dp]
 lib.orc_md_beeman_pd.restype = C.c_int
 lib.orc_md_beeman_pd.argtypes = [C.POINTER(System), _dp, _dp, _dp, C.POINTER(Params), C.c_int, C.c_double, C.c_int,
                                  C.c_double, C.c_double, _dp, _dp]
@@ -322,9 +322,11 @@ def langevin_noise(seed, natoms, steps, skip=None):
     return out
 
 
-def md_langevin_pd(sysm, xyz, vel, params, steps, dt, temperature, gamma, noise, update_nlist=10, hydrogen=None):
+def md_langevin_pd(sysm, xyz, vel, params, steps, dt, temperature, gamma, noise, update_nlist=10, hydrogen=None,
+                   tau=0.0):
     """PD's MolecularDynamics with its Langevin integrator (hydrogen: the
-    LangevinOnHydrogens = false variant's hydrogen flags)."""
+    LangevinOnHydrogens = false variant's hydrogen flags; tau: Berendsen,
+    which acts only in PD's VelocityVerlet fallback below gamma dt 1e-5)."""
     hy = _i8(hydrogen)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64).copy()
     vel = np.ascontiguousarray(vel, dtype=np.float64).copy()
@@ -332,7 +334,7 @@ def md_langevin_pd(sysm, xyz, vel, params, steps, dt, temperature, gamma, noise,
     te = np.zeros(steps)
     tk = np.zeros(steps)
     rc = lib.orc_md_langevin_pd(C.byref(sysm.c), _ptr(xyz, _dp), _ptr(vel, _dp), C.byref(params), steps, dt,
-                                update_nlist, temperature, gamma, _ptr(noise, _dp),
+                                update_nlist, temperature, gamma, tau, _ptr(noise, _dp),
                                 None if hy is None else _ptr(hy, C.POINTER(C.c_int8)), _ptr(te, _dp), _ptr(tk, _dp))
     return rc, xyz.reshape(-1, 3), vel.reshape(-1, 3), te, tk
 

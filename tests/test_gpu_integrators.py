@@ -215,3 +215,45 @@ def test_pd_berendsen_matches_reference(name, precision, tag):
     assert np.abs(v - g[tag + "_v"]).max() <= tv * np.abs(g[tag + "_v"]).max()
     assert rel(e["epot"], g[tag + "_epot"]) <= te
     assert rel(ek, g[tag + "_ekin"]) <= te
+
+
+@pytest.mark.parametrize("run_name", ["md_vv_pd", "md_beeman_pd"])
+def test_pd_md_graph_replay_follows_thermostat(run_name):
+    """Mixed precision (fixed-point sums, so runs are bit-reproducible):
+    with use_graphs 0 and 1, the same sequence of calls -- thermostat off,
+    on, then at a changed tau and temperature -- gives identical bits. A
+    replayed step captured under other thermostat settings would not."""
+    g = G.load("tri_ala")
+    dt = float(g["md_dt"])
+    out = []
+    for graphs in (0, 1):
+        states = []
+        with context_for(g, pdgpu.MIXED, use_graphs=graphs) as ctx:
+            i = ctx.add_system(g, xyz=g["md_xyz0"], bonded=True)
+            ctx.set_velocities(i, g["md_v0"].reshape(-1, 3))
+            run = getattr(ctx, run_name)
+            for tau, temp in ((0.0, 300.0), (1e-13, 300.0), (5e-14, 350.0), (0.0, 300.0)):
+                ctx.set_thermostat(tau, temp)
+                run(i, 12, dt, 5)
+                states.append((ctx.get_positions(i), ctx.get_velocities(i)))
+        out.append(states)
+    for (xa, va), (xb, vb) in zip(*out):
+        assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    # the thermostat changed the trajectory
+    assert not np.array_equal(out[0][0][1], out[0][1][1])
+
+
+def test_langevin_with_thermostat_refuses_past_momentum_removal():
+    """PD zeroes the total momenta at step 200 whenever a thermostat is set,
+    for every integrator (md.cpp:246-258); that is not built, so Langevin
+    runs beyond 200 steps with tau > 0 are refused like the VV and Beeman
+    calls."""
+    g = G.load("tri_ala")
+    with context_for(g, pdgpu.FP64) as ctx:
+        i = ctx.add_system(g, xyz=g["md_xyz0"], bonded=True)
+        ctx.set_thermostat(1e-13, 300.0)
+        with pytest.raises(pdgpu.PdgpuError):
+            ctx.md_langevin_pd(i, 201, 1e-15, 300.0, 1e13, seed=1)
+        ctx.md_langevin_pd(i, 5, 1e-15, 300.0, 1e13, seed=1)
+        ctx.set_thermostat(0.0, 300.0)
+        ctx.md_langevin_pd(i, 201, 1e-15, 300.0, 1e13, seed=1)
