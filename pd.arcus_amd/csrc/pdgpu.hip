@@ -2,6 +2,7 @@
 // declared in include/pdgpu.h.
 #include "../../include/pdgpu.h"
 #include "pdgpu_device.h"
+#include "pdgpu_pair32.h"
 #include "pdgpu_bonded.h"
 #include "pdgpu_integrate.h"
 
@@ -147,7 +148,7 @@ struct pdgpu_ctx {
 
     // device buffers
     // per atom
-    DevBuf px, py, pz, q, lj64, gb64, t123, xyzq32, lj32, gb32, born, acc_g, acc_deda, acc_fx, acc_fy, acc_fz;
+    DevBuf px, py, pz, q, lj64, gb64, t123, xyzq32, lj32, gb32, pp32, born, acc_g, acc_deda, acc_fx, acc_fy, acc_fz;
     DevBuf fx, fy, fz, deda_out, vx, vy, vz, minv, opx, opy, opz, rx, ry, rz;
     // per block
     DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, row_count, row_off, tiles, tile_slot;
@@ -213,6 +214,7 @@ struct pdgpu_ctx {
         s.lj64 = (const double2 *)lj64.p; s.gb64 = (const double2 *)gb64.p;
         s.t123 = (const double *)t123.p;
         s.xyzq32 = (float4 *)xyzq32.p; s.lj32 = (const float2 *)lj32.p; s.gb32 = (const float2 *)gb32.p;
+        s.pp32 = (float4 *)pp32.p;
         s.born = born.p; s.acc_g = acc_g.p; s.acc_deda = acc_deda.p;
         s.acc_fx = acc_fx.p; s.acc_fy = acc_fy.p; s.acc_fz = acc_fz.p;
         s.fx = (double *)fx.p; s.fy = (double *)fy.p; s.fz = (double *)fz.p;
@@ -367,6 +369,7 @@ int grow_atoms(pdgpu_ctx *c, int64_t blocks) {
         TRY(resize(c, c->xyzq32, na, 16));
         TRY(resize(c, c->lj32, na, 8));
         TRY(resize(c, c->gb32, na, 8));
+        TRY(resize(c, c->pp32, na, 16));
     }
     TRY(resize(c, c->blk_inst, blocks, 4));
     TRY(resize(c, c->blk_n, blocks, 4));
@@ -652,8 +655,12 @@ int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, in
     if (mode == 0) {
         {
             ProfScope ps(c, PDGPU_KERNEL_PAIR);
-            hipLaunchKernelGGL((k_pair<R, 0>), dim3(g[kGridPair]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
-                               t[kGridPair]);
+            if constexpr (std::is_same<R, float>::value)
+                hipLaunchKernelGGL(c->P.same_switch ? k_pair_mixed<true> : k_pair_mixed<false>, dim3(g[kGridPair]),
+                                   dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, t[kGridPair]);
+            else
+                hipLaunchKernelGGL((k_pair<R, 0>), dim3(g[kGridPair]), dim3(256), 0, c->stream, S, c->P, sp.b0,
+                                   sp.b1, t[kGridPair]);
         }
         ProfScope ps(c, PDGPU_KERNEL_CHAIN);
         hipLaunchKernelGGL(k_chain<R>, dim3(g[kGridChain]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
@@ -1000,7 +1007,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf *all[] = {&c->px, &c->py, &c->pz, &c->q, &c->lj64, &c->gb64, &c->t123, &c->xyzq32, &c->lj32, &c->gb32,
-                     &c->born, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy, &c->acc_fz, &c->fx, &c->fy, &c->fz,
+                     &c->pp32, &c->born, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy, &c->acc_fz, &c->fx, &c->fy, &c->fz,
                      &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
                      &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->row_count, &c->row_off,
                      &c->tiles, &c->tile_slot, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,

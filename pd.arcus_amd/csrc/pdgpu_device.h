@@ -84,6 +84,7 @@ struct DevState {
     const double *t123;           // terms123
     float4 *xyzq32;               // mixed: block-relative position + charge
     const float2 *lj32, *gb32;
+    float4 *pp32;                 // mixed: (R_min/2, sqrt eps, Born radius a, c/a), c^2 = log2(e)/4
     void *born;                   // R[]: Born radii
     void *acc_g;                  // Born sums (int64 fixed / double)
     void *acc_deda;               // dE/da (int64 fixed / double)
@@ -951,6 +952,10 @@ __global__ void k_born_final(DevState S, PairParams P, int atom_begin, int atom_
         const double G = S.t123[a] + acc_read<R>(S.acc_g, a, kGpolScale);
         const double ai = -P.halfeconv / G + P.born_off;
         reinterpret_cast<R *>(S.born)[a] = (R)ai;
+        if constexpr (Prec<R>::fixed) {
+            const float2 lj = S.lj32[a];
+            S.pp32[a] = make_float4(lj.x, lj.y, (float)ai, (float)(0.60056120439322491 / ai));
+        }
         const double qi = S.q[a];
         eself = P.premul * (qi * qi) / ai;
         if (!isfinite(eself)) atomicOr(S.inst_status + inst, kStatNonFinite);
