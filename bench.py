@@ -13,18 +13,26 @@ FP64 path runs the same steps beside it.
 
 Sections added to the same JSON line:
   md       one 500-residue chain (~8,200 atoms, phi/psi from the alpha /
-           beta basins, bonded terms + GB), 100 steepest-descent steps of
+           beta basins, bonded terms + GB), 4,000 steepest-descent steps of
            clash relief, then velocity-Verlet MD at 1 fs from 300 K, 12 A
-           cutoff, 2 A skin, list rebuilt every 10 steps; mixed and FP64.
+           cutoff, 2 A skin, list rebuilt every 10 steps, in mixed
+           precision; every 100 steps the FP64 path evaluates the mixed
+           positions and the mixed energy and forces at the same positions
+           are compared with it; FP64 MD timed beside it for the ratio.
   serving  minimisation requests with Poisson arrivals at several rates,
            sizes log-uniform in 20-400 residues (extended build, bonded +
            GB, 10 A cutoff); steepest descent until the force RMS is below
            0.1 kcal/mol/A or 1,000 steps; continuous batching with
            admission by device-memory budget; p50/p99 latency and
            minimisations/s.
+  allpairs one 2,000-residue chain (~32,000 atoms), all-pairs Coulomb +
+           Still GB, LJ at 12 A, 1,000 mixed velocity-Verlet steps with the
+           same FP64 check every 100 steps.
 
-Inputs are synthetic (pd.arcus_amd/pdgpu/synth.py) and resident in HBM
-when the timed region starts. Prints one JSON line.
+The workloads (molecules, options) are pd.arcus_amd/pdgpu/workloads.py,
+the same inputs the configuration tests run. Inputs are synthetic
+(pd.arcus_amd/pdgpu/synth.py) and resident in HBM when the timed region
+starts. Prints one JSON line.
 """
 import argparse
 import json
@@ -50,24 +58,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def md_options(precision, pdgpu):
-    return pdgpu.default_options(precision=precision, cutoff=12.0, inner_cutoff=9.0, vdw_cutoff=12.0,
-                                 vdw_inner_cutoff=9.0, vdw14_scaling=0.5, elec14_scaling=0.83333,
-                                 dielectric_solvent=78.5, nlist_padding=2.0)
-
-
-def maxwell(mass, T, seed):
-    rng = np.random.default_rng(seed)
-    sig = np.sqrt(KB * T / mass)[:, None]
-    v = rng.normal(size=(len(mass), 3)) * sig
-    v -= (mass[:, None] * v).sum(0) / mass.sum()  # no net momentum
-    return v
-
-
 def gpu_md(precision, mol, xyz0, vel0, steps, warmup, profile, pdgpu):
     """Runs warmup + steps MD steps; returns the timed steps' wall time, the
     energies before and after them and a profile of a further short run."""
-    ctx = pdgpu.Context(md_options(precision, pdgpu))
+    from pdgpu import workloads as W
+    ctx = pdgpu.Context(W.options("chain", precision))
     i = ctx.add_system(mol, xyz=xyz0, bonded=True)
     ctx.set_velocities(i, vel0)
     e0, ek0 = ctx.md_vv(i, max(1, warmup), 1e-15, 10)
@@ -82,6 +77,15 @@ def gpu_md(precision, mol, xyz0, vel0, steps, warmup, profile, pdgpu):
         ctx.profile(False)
     ctx.close()
     return dt, (e0, ek0), (e, ek), prof
+
+
+def energy_contract(e_mixed, e_ref, pairs):
+    """The mixed path's per-instance contract against FP64 (tests/gpu_helpers.py):
+    |dE| <= max(2e-5 |E|, 2e-5 kcal/mol x sqrt(pairs))."""
+    import pdgpu
+    d = abs(e_mixed - e_ref)
+    bound = max(2e-5 * abs(e_ref), 2e-5 * np.sqrt(max(pairs, 1)) * pdgpu.J_PER_KCALMOL)
+    return d, bound
 
 
 def _oracle_pairs(O, s, xyz, rs, ent, cutoff):
@@ -182,67 +186,85 @@ def load_traffic(name="kernels_batch4096_traffic.json"):
 
 
 def build_md():
-    from pdgpu import synth
+    from pdgpu import workloads as W
     log("md: building the 500-residue chain")
-    return synth.peptide(seed=2, residues=500, mode="basins", bonded=True)
+    return W.chain_molecule()
 
 
 def run_md(args, pdgpu, mol):
+    """Mixed-precision velocity Verlet on the relaxed 500-residue chain with
+    the FP64 path evaluated at the mixed positions every 100 steps (the two
+    precisions' own trajectories part chaotically within a few hundred
+    steps, so their end energies are not comparable); FP64 MD timed over
+    1,000 steps for the ratio."""
+    from pdgpu import workloads as W
     n = mol["n"]
-    # clash relief: 100 steepest-descent steps (PD semantics) of the full
-    # forcefield (bonded + GB) in mixed precision
-    ctx = pdgpu.Context(md_options(pdgpu.MIXED, pdgpu))
-    i = ctx.add_system(mol, bonded=True)
-    ctx.minimise_sd(i, 100, 2e5, 1)
-    xyz = ctx.get_positions(i)
-    e0, f0 = ctx.calc_forces(i)
-    ctx.close()
-    vel = maxwell(mol["mass"], 300.0, seed=7)
-    log(f"md: {n} atoms, {e0['pairs']} pairs in cutoff, epot {e0['epot'] / pdgpu.J_PER_KCALMOL:.1f} kcal/mol")
-    steps = args.md_steps
-    res = {}
-    for name, prec in (("mixed", pdgpu.MIXED), ("fp64", pdgpu.FP64)):
-        t, (ea, eka), (eb, ekb), prof = gpu_md(prec, mol, xyz, vel, steps, 10, True, pdgpu)
-        pairs = 0.5 * (ea["pairs"] + eb["pairs"])
-        etot_a, etot_b = ea["epot"] + eka, eb["epot"] + ekb
-        res[name] = {"value": pairs * steps / t, "ms_per_step": t / steps * 1e3, "pairs_per_step": pairs,
-                     "epot_kcal_end": eb["epot"] / pdgpu.J_PER_KCALMOL,
-                     "etot_drift_kcal": (etot_b - etot_a) / pdgpu.J_PER_KCALMOL,
-                     "temperature_end_K": 2.0 * ekb / (3.0 * n * KB),
-                     "roofline": roofline(prof, eb["pairs"], eb["born_pairs"], name)}
-        log(f"md {name}: {steps} steps in {t:.3f} s")
-    m, d = res["mixed"], res["fp64"]
-    return {"workload": f"single 500-residue chain ({n} atoms), velocity-Verlet MD, dt 1 fs, 300 K, "
-                        f"12 A cutoff + Still GB + bonded terms", "steps": steps, "atoms": n,
-            "unit": "atom-pair evals/s", "mixed": m, "fp64": d, "mixed_over_fp64": m["value"] / d["value"],
-            "epot_rel_diff_mixed_vs_fp64_end": abs(m["epot_kcal_end"] - d["epot_kcal_end"]) /
-            abs(d["epot_kcal_end"])}
-
-
-COMPONENTS = ("epot_vdw", "epot_elec", "epot_pol_self", "epot_pol_cross", "epot_bond", "epot_angle",
-              "epot_torsion")
-
-
-def component_sum(e):
-    """Sum of the magnitudes of the energy components, the scale against
-    which an error in a total that cancels to near zero is judged."""
-    return sum(abs(e.get(k, 0.0)) for k in COMPONENTS)
-
-
-def allpairs_options(precision, pdgpu):
-    """All-pairs Still GB: Cutoff (Coulomb + GB pair term, which PD's fast
-    path evaluates together) and the Still cutoff beyond any distance in the
-    molecule, so neither is switched or truncated; LJ at 12 A, switched from
-    9 A."""
-    return pdgpu.default_options(precision=precision, cutoff=1.0e4, inner_cutoff=1.0e4, vdw_cutoff=12.0,
-                                 vdw_inner_cutoff=9.0, vdw14_scaling=0.5, elec14_scaling=0.83333,
-                                 dielectric_solvent=78.5, gbsa_still_cutoff=1.0e4, nlist_padding=2.0)
+    t0 = time.perf_counter()
+    xyz = W.relax(lambda: pdgpu.Context(W.options("chain", pdgpu.MIXED)), mol)
+    log(f"md: relaxed by 4000 SD steps in {time.perf_counter() - t0:.1f} s")
+    vel = W.maxwell(mol["mass"], 300.0, seed=7)
+    steps, every = args.md_steps, 100
+    cm = pdgpu.Context(W.options("chain", pdgpu.MIXED))
+    c64 = pdgpu.Context(W.options("chain", pdgpu.FP64))
+    i = cm.add_system(mol, xyz=xyz, bonded=True)
+    j = c64.add_system(mol, xyz=xyz, bonded=True)
+    cm.set_velocities(i, vel)
+    e0, ek0 = cm.md_vv(i, 10, 1e-15, 10)  # warm-up
+    checks, t_md = [], 0.0
+    for k in range(max(1, steps // every)):
+        t0 = time.perf_counter()
+        e, ek = cm.md_vv(i, every, 1e-15, 10)
+        t_md += time.perf_counter() - t0
+        x = cm.get_positions(i)
+        em, fm = cm.calc_forces(i)
+        c64.set_positions(j, x)
+        c64.nlist_build(j)  # the list of the mixed positions, as the mixed run keeps it
+        ed, fd = c64.calc_forces(j)
+        d, bound = energy_contract(em["epot"], ed["epot"], ed["pairs"])
+        checks.append({"step": (k + 1) * every, "epot_fp64_kcal": ed["epot"] / pdgpu.J_PER_KCALMOL,
+                       "energy_rel_err": d / abs(ed["epot"]), "energy_err_over_bound": d / bound,
+                       "force_rms_rel_err": float(np.sqrt(np.mean((fm - fd) ** 2)) / np.sqrt(np.mean(fd ** 2))),
+                       "temperature_K": 2.0 * ek / (3.0 * n * KB)})
+    done = len(checks) * every
+    pairs = em["pairs"]
+    cm.profile(True)
+    cm.md_vv(i, 1000, 1e-15, 10)
+    prof = cm.profile_read()
+    cm.close()
+    c64.set_velocities(j, vel)
+    c64.md_vv(j, 10, 1e-15, 10)
+    t0 = time.perf_counter()
+    c64.md_vv(j, 1000, 1e-15, 10)
+    t64 = (time.perf_counter() - t0) / 1000
+    c64.profile(True)
+    e64, _ = c64.md_vv(j, 1000, 1e-15, 10)
+    prof64 = c64.profile_read()
+    c64.close()
+    temps = [c["temperature_K"] for c in checks]
+    log(f"md mixed: {done} steps in {t_md:.3f} s; T {min(temps):.0f}-{max(temps):.0f} K; "
+        f"worst energy error / bound {max(c['energy_err_over_bound'] for c in checks):.2f}")
+    ms = t_md / done * 1e3
+    return {"workload": f"single 500-residue chain ({n} atoms), 4000 SD steps of clash relief, velocity-Verlet "
+                        "MD, dt 1 fs, from 300 K, 12 A cutoff + Still GB + bonded terms, mixed precision; FP64 "
+                        "evaluated at the mixed positions every 100 steps",
+            "steps": done, "atoms": n, "unit": "atom-pair evals/s",
+            "mixed": {"value": pairs / (ms * 1e-3), "ms_per_step": ms, "pairs_per_step": pairs,
+                      "roofline": roofline(prof, em["pairs"], em["born_pairs"], "mixed")},
+            "fp64": {"value": e64["pairs"] / t64, "ms_per_step": t64 * 1e3,
+                     "roofline": roofline(prof64, e64["pairs"], e64["born_pairs"], "fp64")},
+            "mixed_over_fp64": t64 * 1e3 / ms,
+            "epot_start_kcal": e0["epot"] / pdgpu.J_PER_KCALMOL,
+            "temperature_K": {"min": min(temps), "max": max(temps), "end": temps[-1]},
+            "max_energy_rel_err": max(c["energy_rel_err"] for c in checks),
+            "max_energy_err_over_contract": max(c["energy_err_over_bound"] for c in checks),
+            "max_force_rms_rel_err": max(c["force_rms_rel_err"] for c in checks),
+            "checks": checks[:: max(1, len(checks) // 10)]}
 
 
 def build_allpairs(args):
-    from pdgpu import synth
+    from pdgpu import workloads as W
     log(f"allpairs: building a {args.allpairs_residues}-residue chain")
-    return synth.peptide(seed=2000, residues=args.allpairs_residues, bonded=True)
+    return W.allpairs_molecule(args.allpairs_residues)
 
 
 def run_allpairs(args, pdgpu, mol):
@@ -250,19 +272,15 @@ def run_allpairs(args, pdgpu, mol):
     steps the mixed trajectory's positions are evaluated by the FP64 path
     (the path held to the oracle at 1e-10) and the mixed energy and forces
     at the same positions are compared with it."""
+    from pdgpu import workloads as W
     n = mol["n"]
     steps, every = args.allpairs_steps, 100
-    # clash relief as for the single-chain MD: 100 steepest-descent steps
-    # (12 A cutoff, bonded + GB); the extended build leaves a few side-chain
-    # contacts near 0.9 A (Asp k / aromatic k+2)
-    ctx = pdgpu.Context(md_options(pdgpu.MIXED, pdgpu))
-    k0 = ctx.add_system(mol, bonded=True)
-    ctx.minimise_sd(k0, 100, 2e5, 1)
-    xyz = ctx.get_positions(k0)
-    ctx.close()
-    vel = maxwell(mol["mass"], 300.0, seed=9)
-    cm = pdgpu.Context(allpairs_options(pdgpu.MIXED, pdgpu))
-    c64 = pdgpu.Context(allpairs_options(pdgpu.FP64, pdgpu))
+    # clash relief as for the single-chain MD (12 A cutoff, bonded + GB); the
+    # extended build leaves a few side-chain contacts near 0.9 A
+    xyz = W.relax(lambda: pdgpu.Context(W.options("chain", pdgpu.MIXED)), mol)
+    vel = W.maxwell(mol["mass"], 300.0, seed=9)
+    cm = pdgpu.Context(W.options("allpairs", pdgpu.MIXED))
+    c64 = pdgpu.Context(W.options("allpairs", pdgpu.FP64))
     i = cm.add_system(mol, xyz=xyz, bonded=True)
     j = c64.add_system(mol, xyz=xyz, bonded=True)
     cm.set_velocities(i, vel)
@@ -277,10 +295,11 @@ def run_allpairs(args, pdgpu, mol):
         x = cm.get_positions(i)
         em, fm = cm.calc_forces(i)
         c64.set_positions(j, x)
+        c64.nlist_build(j)  # the list of the mixed positions, as the mixed run keeps it
         ed, fd = c64.calc_forces(j)
+        d, bound = energy_contract(em["epot"], ed["epot"], ed["pairs"])
         checks.append({"step": (k + 1) * every, "epot_fp64_kcal": ed["epot"] / pdgpu.J_PER_KCALMOL,
-                       "energy_rel_err": abs(em["epot"] - ed["epot"]) / abs(ed["epot"]),
-                       "energy_err_rel_to_component_sum": abs(em["epot"] - ed["epot"]) / component_sum(ed),
+                       "energy_rel_err": d / abs(ed["epot"]), "energy_err_over_bound": d / bound,
                        "force_rms_rel_err": float(np.sqrt(np.mean((fm - fd) ** 2)) / np.sqrt(np.mean(fd ** 2))),
                        "temperature_K": 2.0 * ek / (3.0 * n * KB)})
         log(f"allpairs step {(k + 1) * every}: epot {checks[-1]['epot_fp64_kcal']:.1f} kcal/mol, "
@@ -306,21 +325,18 @@ def run_allpairs(args, pdgpu, mol):
             "value": v, "ms_per_step": t_md / done * 1e3, "fp64_ms_per_step": t64 * 1e3,
             "mixed_over_fp64": t64 / (t_md / done),
             "max_energy_rel_err": max(c["energy_rel_err"] for c in checks),
-            "max_energy_err_rel_to_component_sum": max(c["energy_err_rel_to_component_sum"] for c in checks),
+            "max_energy_err_over_contract": max(c["energy_err_over_bound"] for c in checks),
             "max_force_rms_rel_err": max(c["force_rms_rel_err"] for c in checks),
             "autotuned_waves_per_cu": grids,
             "checks": checks, "roofline": roofline(prof, em["pairs"], em["born_pairs"], "mixed")}
 
 
 def build_serving(args):
-    from pdgpu import synth
-    rng = np.random.default_rng(17)
+    from pdgpu import workloads as W
     nreq = args.serve_requests
-    sizes = np.exp(rng.uniform(np.log(20), np.log(400), size=nreq)).astype(int)
     log(f"serving: building {nreq} requests (20-400 residues, bonded terms)")
     t0 = time.perf_counter()
-    mols = synth.many([5000 + k for k in range(nreq)], [int(x) for x in sizes], bonded=True,
-                      workers=args.build_workers)
+    mols = W.serving_molecules(nreq, workers=args.build_workers)
     log(f"serving: built in {time.perf_counter() - t0:.1f} s")
     return mols
 
@@ -329,9 +345,8 @@ def run_serving(args, pdgpu, mols):
     """Continuous-batching minimisation server at several Poisson rates."""
     from pdgpu import serve
     nreq = len(mols)
-    opts = pdgpu.default_options(precision=pdgpu.MIXED, cutoff=10.0, inner_cutoff=7.0, vdw_cutoff=10.0,
-                                 vdw_inner_cutoff=7.0, vdw14_scaling=0.5, elec14_scaling=0.83333,
-                                 dielectric_solvent=78.5, nlist_padding=2.0, max_instances=nreq)
+    from pdgpu import workloads as W
+    opts = W.options("serving", pdgpu.MIXED, max_instances=nreq)
     out = {"workload": f"{nreq} minimisation requests per rate, sizes log-uniform 20-400 residues "
                        f"(mean {int(np.mean([m['n'] for m in mols]))} atoms), steepest descent until force "
                        "RMS < 0.1 kcal/mol/A or 1000 steps, 10 A cutoff + Still GB + bonded, mixed precision",
@@ -357,22 +372,17 @@ def run_serving(args, pdgpu, mols):
 
 
 def build_batch(args):
-    from pdgpu import synth
-    rng = np.random.default_rng(11)
-    lengths = rng.integers(10, 151, size=args.conformers)
+    from pdgpu import workloads as W
     log(f"building {args.conformers} conformers")
     t0 = time.perf_counter()
-    mols = synth.many([1000 + k for k in range(args.conformers)], [int(L) for L in lengths],
-                      workers=args.build_workers)
+    mols = W.batch_molecules(args.conformers, workers=args.build_workers)
     log(f"built in {time.perf_counter() - t0:.1f} s")
     return mols
 
 
 def run_batch(args, pdgpu, mols):
-    opt = lambda prec: pdgpu.default_options(precision=prec, cutoff=10.0, inner_cutoff=7.0, vdw_cutoff=10.0,
-                                             vdw_inner_cutoff=7.0, vdw14_scaling=0.5, elec14_scaling=0.83333,
-                                             dielectric_solvent=78.5, nlist_padding=2.0,
-                                             max_instances=args.conformers)
+    from pdgpu import workloads as W
+    opt = lambda prec: W.options("batch", prec, max_instances=args.conformers)
     natoms = sum(m["n"] for m in mols)
     res, grids = {}, {}
     for name, prec in (("mixed", pdgpu.MIXED), ("fp64", pdgpu.FP64)):
@@ -404,8 +414,10 @@ def run_batch(args, pdgpu, mols):
     dt, pairs, born, prof, r = res["mixed"]
     dt64, pairs64, born64, prof64, r64 = res["fp64"]
     errs = [abs(r[k]["epot"] - r64[k]["epot"]) / abs(r64[k]["epot"]) for k in r]
-    errs_c = [abs(r[k]["epot"] - r64[k]["epot"]) / component_sum(r64[k]) for k in r]
-    worst = max(r, key=lambda k: errs[k] if False else abs(r[k]["epot"] - r64[k]["epot"]) / abs(r64[k]["epot"]))
+    over = [energy_contract(r[k]["epot"], r64[k]["epot"], r64[k]["pairs"]) for k in r]
+    abs_sqrt = [abs(r[k]["epot"] - r64[k]["epot"]) / pdgpu.J_PER_KCALMOL / np.sqrt(max(1, r64[k]["pairs"]))
+                for k in r]
+    worst = max(r, key=lambda k: abs(r[k]["epot"] - r64[k]["epot"]) / abs(r64[k]["epot"]))
     tot_err = abs(sum(r[k]["epot"] for k in r) - sum(r64[k]["epot"] for k in r)) / abs(sum(r64[k]["epot"] for k in r))
     cpu = args.cpu_baseline
     traffic = load_traffic()
@@ -422,10 +434,16 @@ def run_batch(args, pdgpu, mols):
         "fp64": {"value": pairs64 * args.steps / dt64, "ms_per_step": dt64 / args.steps * 1e3,
                  "roofline": roofline(prof64, pairs64, born64, "fp64", traffic)},
         "mixed_over_fp64": (pairs * args.steps / dt) / (pairs64 * args.steps / dt64),
-        "max_energy_rel_err_mixed_vs_fp64": max(errs),
-        "median_energy_rel_err_mixed_vs_fp64": float(np.median(errs)),
-        "max_energy_err_rel_to_component_sum": max(errs_c),
-        "batch_total_energy_rel_err": tot_err,
+        "accuracy": {
+            "contract": "per conformer |E_mixed - E_fp64| <= max(2e-5 |E_fp64|, 2e-5 kcal/mol x sqrt(pairs))",
+            "conformers_within_contract": sum(1 for d, b in over if d <= b),
+            "conformers": len(over),
+            "max_err_over_contract": max(d / b for d, b in over),
+            "fraction_within_1e-5_relative": float(np.mean([e <= 1e-5 for e in errs])),
+            "max_energy_rel_err": max(errs),
+            "median_energy_rel_err": float(np.median(errs)),
+            "max_abs_err_kcal_per_sqrt_pair": max(abs_sqrt),
+            "batch_total_energy_rel_err": tot_err},
         "worst_instance": {"epot_fp64_kcal": r64[worst]["epot"] / pdgpu.J_PER_KCALMOL,
                            "epot_mixed_kcal": r[worst]["epot"] / pdgpu.J_PER_KCALMOL,
                            "components_fp64_kcal": {k: r64[worst][k] / pdgpu.J_PER_KCALMOL for k in

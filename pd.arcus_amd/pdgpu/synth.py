@@ -536,14 +536,15 @@ def _peptide_job(args):
     return peptide(seed, residues, mode, bonded)
 
 
-def many(seeds, residues, mode="extended", workers=None, bonded=False):
-    """Builds many molecules in parallel worker processes (call this before
-    any GPU context exists in the process: workers are forked)."""
+def many(seeds, residues, mode="extended", workers=None, bonded=False, start="fork"):
+    """Builds many molecules in parallel worker processes. start="fork" is
+    the fast choice before any GPU context exists in the process; a process
+    that has used the GPU passes start="spawn" (fresh interpreters)."""
     import multiprocessing as mp
     import os
     jobs = list(zip(seeds, residues, [mode] * len(seeds), [bonded] * len(seeds)))
     workers = workers or min(16, os.cpu_count() or 1)
     if workers <= 1 or len(jobs) < 8:
         return [_peptide_job(j) for j in jobs]
-    with mp.get_context("fork").Pool(workers) as pool:
+    with mp.get_context(start).Pool(workers) as pool:
         return pool.map(_peptide_job, jobs, chunksize=max(1, len(jobs) // (4 * workers)))

@@ -1,0 +1,107 @@
+"""The five workloads of the benchmark, as data: molecules, forcefield
+options and run lengths, shared by bench.py and the configuration tests so
+that both run exactly the same inputs.
+
+  single   one 20-residue peptide, 12 A cutoff + 2 A skin (the golden
+           fixtures c1_seed1..3 hold the reference's own runs of it)
+  chain    one 500-residue chain (~8,000 atoms), backbone phi/psi from the
+           alpha and beta basins, 12 A cutoff; clash relief, then
+           velocity-Verlet MD at 300 K
+  batch    4,096 conformers of 10-150 residues, 10 A cutoff, one batched
+           energy + force evaluation per step
+  serving  minimisation requests of 20-400 residues (log-uniform), 10 A
+           cutoff, steepest descent to force RMS < 0.1 kcal/mol/A or 1,000
+           steps
+  allpairs one 2,000-residue chain (~32,000 atoms), Coulomb and Still GB over
+           every pair, LJ at 12 A, velocity-Verlet MD
+
+Molecules come from pdgpu.synth (seeded; see its docstring for what it
+builds)."""
+import numpy as np
+
+from . import PDGPU_ALL, default_options, synth  # noqa: F401
+
+KB = 1.380649e-23  # J/K
+
+# forcefield knobs common to every workload: PD's amber99 1-4 scalings,
+# Still GB with eps_in 1, eps_out 78.5, InnerCutoff = Cutoff - 3 A,
+# VdwCutoff = Cutoff, VdwInnerCutoff = InnerCutoff
+_COMMON = dict(vdw14_scaling=0.5, elec14_scaling=0.83333, dielectric_solvent=78.5, nlist_padding=2.0)
+
+
+def options(name, precision, **extra):
+    """pdgpu options of a workload (precision: pdgpu.FP64 or pdgpu.MIXED)."""
+    if name in ("single", "chain"):
+        kw = dict(cutoff=12.0, inner_cutoff=9.0, vdw_cutoff=12.0, vdw_inner_cutoff=9.0)
+    elif name in ("batch", "serving"):
+        kw = dict(cutoff=10.0, inner_cutoff=7.0, vdw_cutoff=10.0, vdw_inner_cutoff=7.0)
+    elif name == "allpairs":
+        # Cutoff (Coulomb + the GB pair term, which PD's fast path evaluates
+        # together) and the Still cutoff beyond any distance in the molecule,
+        # so neither is switched or truncated; LJ at 12 A, switched from 9 A
+        kw = dict(cutoff=1.0e4, inner_cutoff=1.0e4, vdw_cutoff=12.0, vdw_inner_cutoff=9.0,
+                  gbsa_still_cutoff=1.0e4)
+    else:
+        raise KeyError(name)
+    kw = dict(_COMMON, **kw)
+    kw.update(extra)
+    return default_options(precision=precision, **kw)
+
+
+def oracle_params(name, **extra):
+    """The same knobs under the oracle's field names (tests only)."""
+    o = options(name, 0)
+    d = dict(cutoff=o.cutoff, inner_cutoff=o.inner_cutoff, vdw_cutoff=o.vdw_cutoff,
+             vdw_inner_cutoff=o.vdw_inner_cutoff, vdw14_scaling=o.vdw14_scaling, elec14_scaling=o.elec14_scaling,
+             dielectric_solvent=o.dielectric_solvent, still_cutoff=o.gbsa_still_cutoff, padding=o.nlist_padding)
+    d.update(extra)
+    return d
+
+
+def batch_lengths(n=4096):
+    return [int(L) for L in np.random.default_rng(11).integers(10, 151, size=n)]
+
+
+def batch_molecules(n=4096, workers=0, start="fork"):
+    """The batch: n extended conformers, lengths uniform in 10-150 residues,
+    conformer k built from seed 1000 + k."""
+    return synth.many([1000 + k for k in range(n)], batch_lengths(n), workers=workers, start=start)
+
+
+def chain_molecule():
+    return synth.peptide(seed=2, residues=500, mode="basins", bonded=True)
+
+
+def serving_sizes(n=200):
+    return [int(x) for x in np.exp(np.random.default_rng(17).uniform(np.log(20), np.log(400), size=n))]
+
+
+def serving_molecules(n=200, workers=0, start="fork"):
+    """n requests, sizes log-uniform in 20-400 residues, request k built
+    from seed 5000 + k, with FF_Bonded terms."""
+    return synth.many([5000 + k for k in range(n)], serving_sizes(n), bonded=True, workers=workers, start=start)
+
+
+def allpairs_molecule(residues=2000):
+    return synth.peptide(seed=2000, residues=residues, bonded=True)
+
+
+def maxwell(mass, temperature, seed):
+    """Maxwell-Boltzmann velocities (m/s) without net momentum."""
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(len(mass), 3)) * np.sqrt(KB * temperature / mass)[:, None]
+    v -= (mass[:, None] * v).sum(0) / mass.sum()
+    return v
+
+
+def relax(ctx_factory, mol, steps=4000):
+    """Clash relief of a synthetic chain: `steps` of PD's steepest descent
+    (StepSize 2e5, list rebuilt every step) over FF_Bonded + GB in mixed
+    precision; returns the positions. For the 500-residue chain: +5.1e5
+    kcal/mol as built, +1.6e4 after 100 steps, -2.2e3 after 4,000; velocity
+    Verlet from 300 K Maxwell velocities then holds ~235 K over 10,000
+    steps (after 100 steps it heated to ~600 K)."""
+    with ctx_factory() as ctx:
+        i = ctx.add_system(mol, bonded=True)
+        ctx.minimise_sd(i, steps, 2e5, 1)
+        return ctx.get_positions(i)
