@@ -212,6 +212,20 @@ int pdgpu_nlist_build(pdgpu_ctx *ctx, int32_t id);
 int pdgpu_nlist_export(pdgpu_ctx *ctx, int32_t id, int32_t shadow, int32_t *row_start,
                        uint32_t *entries, int64_t capacity, int64_t *count);
 
+/* The same export with no device allocation (no counterpart in the
+ * reference): for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. The
+ * per-atom counts, row offsets and entries are staged in one static 64 MiB
+ * device array per GPU instead of hipMalloc'd buffers; instances whose
+ * staging (about 20 B per atom + 4 B per entry) exceeds it get
+ * PDGPU_ERR_NOMEM with a message, as does a too small capacity (*count is
+ * then the size needed). The instance's list must already exist
+ * (pdgpu_nlist_build or an evaluation), else PDGPU_ERR_INVALID. All calls
+ * on one GPU share the array: calls that overlap on one GPU race. Output
+ * is bit-identical to pdgpu_nlist_export. */
+int pdgpu_nlist_export_noalloc(pdgpu_ctx *ctx, int32_t id, int32_t shadow, int32_t *row_start,
+                               uint32_t *entries, int64_t capacity, int64_t *count);
+
 /* Forcefield::calcForces over the context's components
  * (forcefields/forcefield.cpp:220-227): FF_Bonded::calcForces
  * (forcefields/ffbonded.cpp:809-824) and FF_GeneralizedBorn_Still::calcForces

@@ -18,6 +18,34 @@
 
 using namespace pdgpu;
 
+// ------------------------------------------- neighbour-list export, no allocation
+// pdgpu_nlist_export_noalloc stages the per-atom counts, row offsets and the
+// packed entries in one static device array instead of allocating them per
+// call. Each GPU holds its own copy until the process ends; host code uses
+// the per-device address from hipGetSymbolAddress, never the symbol itself.
+// Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+// race (one context or one thread per GPU at a time).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+unsigned char *g_scratch[kMaxDevices];     // device address of scratch per ordinal
+int g_scratch_rc[kMaxDevices];             // 0 unset, 1 ready, -1 lookup failed
+
+// Called once per device with that device current (context creation).
+int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return PDGPU_ERR_NODEVICE;
+    if (g_scratch_rc[device] != 0) return g_scratch_rc[device] > 0 ? PDGPU_OK : PDGPU_ERR_HIP;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch[device] = (unsigned char *)ptr;
+    g_scratch_rc[device] = (e == hipSuccess && ptr) ? 1 : -1;
+    return g_scratch_rc[device] > 0 ? PDGPU_OK : PDGPU_ERR_HIP;
+}
+} // namespace
+
+
 namespace {
 
 constexpr double kKcal2J = 4184.0;
@@ -948,6 +976,8 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     if (hipGetDeviceProperties(&prop, o->device) != hipSuccess) return PDGPU_ERR_NODEVICE;
     if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) return PDGPU_ERR_NODEVICE;
     if (hipSetDevice(o->device) != hipSuccess) return PDGPU_ERR_NODEVICE;
+    // the device address of the static scratch array, once per device
+    if (scratch_setup(o->device) != PDGPU_OK) return PDGPU_ERR_HIP;
     pdgpu_ctx *c = new pdgpu_ctx();
     c->opt = *o;
     c->device = o->device;
@@ -1423,6 +1453,90 @@ extern "C" int pdgpu_nlist_export(pdgpu_ctx *c, int32_t id, int32_t shadow, int3
     (void)hipFree(d_cnt);
     if (d_off) (void)hipFree(d_off);
     if (d_ent) (void)hipFree(d_ent);
+    if (e != hipSuccess) return fail(c, PDGPU_ERR_HIP, hipGetErrorString(e));
+    return PDGPU_OK;
+}
+
+namespace pdgpu {
+// Exclusive scan of counts[0, n) into offs (int64) and row32 (int32,
+// clamped), total at offs[n]; one workgroup of 1024 threads, deterministic.
+__global__ void __launch_bounds__(1024) k_scan_counts(const int *counts, int n, long long *offs, int32_t *row32) {
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b0 = t * per, b1 = min(n, b0 + per);
+    long long sum = 0;
+    for (int i = b0; i < b1; ++i) sum += counts[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const long long v = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = part[t] - sum;
+    for (int i = b0; i < b1; ++i) {
+        offs[i] = run;
+        row32[i] = (int32_t)min(run, (long long)INT32_MAX);
+        run += counts[i];
+    }
+    if (t == 1023) {
+        offs[n] = part[1023];
+        row32[n] = (int32_t)min(part[1023], (long long)INT32_MAX);
+    }
+}
+
+// Zero fill (instead of hipMemsetAsync, so a captured graph replays it).
+__global__ void k_zero_i32(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+} // namespace pdgpu
+
+extern "C" int pdgpu_nlist_export_noalloc(pdgpu_ctx *c, int32_t id, int32_t shadow, int32_t *row_start,
+                                          uint32_t *entries, int64_t capacity, int64_t *count) {
+    if (!c || !valid_id(c, id) || !row_start || !count || capacity < 0 || (capacity > 0 && !entries))
+        return PDGPU_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    if (!c->inst[id].listed)
+        return fail(c, PDGPU_ERR_INVALID, "nlist_export_noalloc: build the list first (pdgpu_nlist_build)");
+    TRY(scratch_setup(c->device));
+    unsigned char *base = g_scratch[c->device];
+    const Instance &I = c->inst[id];
+    const size_t n = (size_t)I.natoms;
+    // layout: counts int32[n] | row32 int32[n+1] | offs int64[n+1] | entries
+    const size_t o_row = (n * 4 + 255) / 256 * 256;
+    const size_t o_off = (o_row + (n + 1) * 4 + 255) / 256 * 256;
+    const size_t o_ent = (o_off + (n + 1) * 8 + 255) / 256 * 256;
+    if (o_ent >= kScratchBytes)
+        return fail(c, PDGPU_ERR_NOMEM, "nlist_export_noalloc: instance beyond the 64 MiB scratch array");
+    int *d_cnt = (int *)base;
+    int32_t *d_row = (int32_t *)(base + o_row);
+    long long *d_off = (long long *)(base + o_off);
+    uint32_t *d_ent = (uint32_t *)(base + o_ent);
+    const DevState S = c->S();
+    // the scratch array keeps the last call's contents: counts start at zero
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, d_cnt, (int)n);
+    hipLaunchKernelGGL(k_export<0>, dim3((I.nblocks + 3) / 4), dim3(256), 0, c->stream, S, c->P, I.block0, I.nblocks,
+                       shadow ? 1 : 0, d_cnt, (const long long *)nullptr, (uint32_t *)nullptr);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, c->stream, (const int *)d_cnt, (int)n, d_off, d_row);
+    long long total = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&total, d_off + n, 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(row_start, d_row, (n + 1) * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, PDGPU_ERR_HIP, hipGetErrorString(e));
+    *count = total;
+    if (total > capacity) return fail(c, PDGPU_ERR_NOMEM, "export capacity too small");
+    if ((size_t)total * 4 > kScratchBytes - o_ent)
+        return fail(c, PDGPU_ERR_NOMEM, "nlist_export_noalloc: list beyond the 64 MiB scratch array");
+    hipLaunchKernelGGL(k_export<1>, dim3((I.nblocks + 3) / 4), dim3(256), 0, c->stream, S, c->P, I.block0, I.nblocks,
+                       shadow ? 1 : 0, (int *)nullptr, (const long long *)d_off, d_ent);
+    e = hipGetLastError();
+    if (e == hipSuccess && total > 0)
+        e = hipMemcpyAsync(entries, d_ent, (size_t)total * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(c, PDGPU_ERR_HIP, hipGetErrorString(e));
     return PDGPU_OK;
 }
