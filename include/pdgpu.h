@@ -114,6 +114,17 @@ typedef struct pdgpu_options {
  * PDGPU_ERR_UNSUPPORTED. The soft pairs (r < R_i + R_j - OlapOffset) must
  * lie within Cutoff, which holds for any Cutoff above ~5 A. */
 #define PDGPU_FF_SOFTVDW 32
+/* FF_NonBonded on its own (forcefields/nonbonded.cpp:1137-1229, the
+ * FF_NonBonded_CalcForces_T_fast2 pair loop :542-893): vacuum LJ +
+ * Coulomb over the cutoff list with the context's elec_mode / vdw_mode
+ * (PD's FF_NonBonded default is the force switch, elec_mode 4, see
+ * pdgpu_options_default_nonbonded), energies in epot_vdw / epot_elec, its
+ * InternalVirial in virial. calcForces and calcEnergies are the same full
+ * evaluation (PD's UsePartialRecalc is off by default). Not part of
+ * PDGPU_FF_ALL; combines with the FF_Bonded terms, not with PDGPU_FF_GB
+ * (which holds its own vacuum terms) or PDGPU_FF_SOFTVDW
+ * (PDGPU_ERR_UNSUPPORTED). */
+#define PDGPU_FF_NONBONDED 64
 
 /* One molecule. Per-atom arrays have natoms entries. The bond-order band and
  * the sparse list are PD's BondOrder (workspace/bondorder.h:43-112): band has
@@ -175,6 +186,14 @@ typedef struct pdgpu_ctx pdgpu_ctx;
 /* Fills *o with the defaults of a new FF_GeneralizedBorn_Still
  * (forcefields/gbff.cpp:17-37, forcefields/nonbonded.cpp:968-1004). */
 void pdgpu_options_default(pdgpu_options *o);
+
+/* Fills *o with the defaults of a new FF_NonBonded (FF_NonBonded::
+ * settodefault, forcefields/nonbonded.cpp:968-1004: Cutoff 15, InnerCutoff
+ * 12, VdwCutoff 15, VdwInnerCutoff 12, force switching, Dielectric 1) and
+ * components = PDGPU_FF_NONBONDED | PDGPU_FF_BONDED. The 1-4 scalings are
+ * 1.0 as constructed; PD's AutoScaling14 copies the parameter file's values
+ * (0.5, 0.83333 for amber99) at setup, which the caller sets here. */
+void pdgpu_options_default_nonbonded(pdgpu_options *o);
 
 /* Creation of the component and its device state; stands in for the
  * constructor plus ForcefieldBase::ensuresetup (forcefields/forcefield.h:143). */

@@ -713,6 +713,7 @@ int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2
     const int ns = sp.s1 - sp.s0;
     const bool gb = c->opt.components & PDGPU_FF_GB;
     const bool soft = c->opt.components & PDGPU_FF_SOFTVDW;
+    const bool nonb = c->opt.components & PDGPU_FF_NONBONDED;
     {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
         const int nt = std::max(na, ns);
@@ -721,10 +722,24 @@ int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2
         else
             hipLaunchKernelGGL(k_clear_grad<R>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, S, a0, a1, sp.s0,
                                sp.s1);
-        if (soft && c->mixed) // block-relative FP32 positions (k_prepare's job when GB runs)
+        if ((soft || nonb) && c->mixed) // block-relative FP32 positions (k_prepare's job when GB runs)
             hipLaunchKernelGGL(k_xyzq32, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
     }
     if (gb) TRY(launch_gb<R>(c, S, sp, span_tiles(c, id), mode));
+    if (nonb) {
+        // FF_NonBonded alone: the energy path's pair term without the GB
+        // energy (MODE 2); its calcEnergies is the same evaluation
+        int g[kGridKinds], t[kGridKinds];
+        for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, span_tiles(c, id), &g[k], &t[k]);
+        {
+            ProfScope ps(c, PDGPU_KERNEL_ENUM);
+            hipLaunchKernelGGL(k_filter<R>, dim3(g[kGridEnum]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                               t[kGridEnum]);
+        }
+        ProfScope ps(c, PDGPU_KERNEL_PAIR);
+        hipLaunchKernelGGL((k_pair<R, 2>), dim3(g[kGridPair]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                           t[kGridPair]);
+    }
     if (soft) {
         // FF_SoftVDW over the cutoff-filtered lists (calcEnergies is its
         // calcForces, ffsoftvdw.cpp:56-59)
@@ -945,7 +960,7 @@ bool use_graphs(const pdgpu_ctx *c) { return c->opt.use_graphs != 0 && !c->prof;
 // eagerly (a rebuild reads counts back); the kernels of the evaluation are
 // then replayed as a HIP graph when graphs are on.
 int eval(pdgpu_ctx *c, int id, int mode) {
-    if (c->opt.components & (PDGPU_FF_GB | PDGPU_FF_SOFTVDW)) TRY(ensure_listed(c, id));
+    if (c->opt.components & (PDGPU_FF_GB | PDGPU_FF_SOFTVDW | PDGPU_FF_NONBONDED)) TRY(ensure_listed(c, id));
     auto issue = [&] { return c->mixed ? launch_eval<float>(c, id, mode) : launch_eval<double>(c, id, mode); };
     if (!use_graphs(c)) return issue();
     return graph_run(c, "calc:" + std::to_string(id) + ":" + std::to_string(mode), graph_sig(c, id), issue);
@@ -962,8 +977,15 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     *out = nullptr;
     if (o->fast_mode != 1) return PDGPU_ERR_UNSUPPORTED;
     if (o->precision != PDGPU_FP64 && o->precision != PDGPU_MIXED) return PDGPU_ERR_INVALID;
-    if ((o->components & ~(PDGPU_FF_ALL | PDGPU_FF_SOFTVDW)) != 0) return PDGPU_ERR_INVALID;
-    if ((o->components & PDGPU_FF_GB) && (o->components & PDGPU_FF_SOFTVDW)) return PDGPU_ERR_UNSUPPORTED;
+    if ((o->components & ~(PDGPU_FF_ALL | PDGPU_FF_SOFTVDW | PDGPU_FF_NONBONDED)) != 0) return PDGPU_ERR_INVALID;
+    {
+        const int pairwise = !!(o->components & PDGPU_FF_GB) + !!(o->components & PDGPU_FF_SOFTVDW) +
+                             !!(o->components & PDGPU_FF_NONBONDED);
+        if (pairwise > 1) return PDGPU_ERR_UNSUPPORTED; // one pair component per context
+    }
+    if ((o->components & PDGPU_FF_NONBONDED) && o->elec_mode != 0 && o->elec_mode != 1 && o->elec_mode != 3 &&
+        o->elec_mode != 4)
+        return PDGPU_ERR_INVALID;
     // instance slots are stored in 24-bit fields (term records, noise counters)
     if (o->max_instances >= (1 << 24)) return PDGPU_ERR_INVALID;
     if (!(o->cutoff > 0) || !(o->vdw_cutoff > 0) || o->inner_cutoff > o->cutoff ||

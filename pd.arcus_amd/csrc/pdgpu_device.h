@@ -1103,8 +1103,8 @@ __device__ __forceinline__ R pair_term(const PairParams &P, const PairAtom<R> &A
         const R fm = vdw_force + elec_force;
         E.vir += r * fm;
         g = fm * rinv;
-        // exact pairwise GB energy, no threshold branch
-        if (r <= (R)P.cut) {
+        // exact pairwise GB energy, no threshold branch (MODE 2: FF_NonBonded alone)
+        if (MODE == 1 && r <= (R)P.cut) {
             R Sw = (R)1, dS;
             if (r >= inner) switch_fn<R>(r, inner, invSw, Sw, dS);
             const R s2 = r * r;
@@ -1121,6 +1121,7 @@ __device__ __forceinline__ R pair_term(const PairParams &P, const PairAtom<R> &A
 // MODE 1: energy path -- FF_NonBonded with the context's switching modes
 //         (forcefields/nonbonded.cpp:670-873) plus the exact pairwise GB
 //         energy of calcBornEnergy (forcefields/gbff.cpp:520-579).
+// MODE 2: FF_NonBonded alone (forcefields/nonbonded.cpp:1205-1229).
 template <typename R, int MODE>
 __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
     __shared__ PairAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];

@@ -50,6 +50,16 @@ extern "C" void pdgpu_options_default(pdgpu_options *o) {
     o->softvdw_cutoff = 15.0;
 }
 
+extern "C" void pdgpu_options_default_nonbonded(pdgpu_options *o) {
+    if (!o) return;
+    pdgpu_options_default(o);
+    // FF_NonBonded::settodefault (forcefields/nonbonded.cpp:968-1004):
+    // ForceSwitch = true, EnergySwitch = false
+    o->elec_mode = 4;
+    o->vdw_mode = 2;
+    o->components = PDGPU_FF_NONBONDED | PDGPU_FF_BONDED;
+}
+
 // BondOrder::reinit (workspace/bondorder.cpp:77-195). The band holds the
 // lowest topological order per pair for index gaps up to max_index_delta;
 // wider pairs are returned in the sparse arrays (i<j), each pair once.

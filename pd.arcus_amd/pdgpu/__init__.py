@@ -49,12 +49,13 @@ EXPORTS = (
     "pdgpu_bonded_set", "pdgpu_bonded_bytes", "pdgpu_autotune", "pdgpu_md_vv_pd", "pdgpu_md_langevin_pd",
     "pdgpu_md_beeman_pd", "pdgpu_mime_encode", "pdgpu_snapshot_mime", "pdgpu_dcd_header", "pdgpu_dcd_frame",
     "pdgpu_dcd_create", "pdgpu_dcd_append", "pdgpu_traj_dcd", "pdgpu_set_hydrogens",
-    "pdgpu_set_thermostat",
+    "pdgpu_set_thermostat", "pdgpu_nlist_export_noalloc", "pdgpu_options_default_nonbonded",
 )
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
 FF_BONDED = FF_BONDS | FF_ANGLES | FF_TORSIONS | FF_IMPROPERS
 FF_SOFTVDW = 32  # FF_SoftVDW soft-core repulsion (not in FF_ALL; not with FF_GB)
+FF_NONBONDED = 64  # FF_NonBonded alone: vacuum LJ + Coulomb (not in FF_ALL; not with FF_GB)
 FF_ALL = FF_GB | FF_BONDED
 KERNEL_GROUPS = ("nlist", "atom", "born", "pair", "chain", "integrate", "enum", "bonded")
 
@@ -139,6 +140,9 @@ def _load():
     P = C.POINTER
     sig = {
         "pdgpu_options_default": (None, [P(Options)]),
+        "pdgpu_options_default_nonbonded": (None, [P(Options)]),
+        "pdgpu_nlist_export_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, _up, C.c_int64,
+                                                 P(C.c_int64)]),
         "pdgpu_ctx_create": (C.c_int, [P(Options), P(C.c_void_p)]),
         "pdgpu_ctx_destroy": (None, [C.c_void_p]),
         "pdgpu_last_error": (C.c_char_p, [C.c_void_p]),
@@ -207,6 +211,18 @@ def _f64(a):
 
 def _i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def default_options_nonbonded(**kw):
+    """Options of a new FF_NonBonded (force switching, Cutoff 15 / 12) with
+    components FF_NONBONDED | FF_BONDED; keywords override fields."""
+    o = Options()
+    lib.pdgpu_options_default_nonbonded(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
 
 
 def default_options(**kw):
