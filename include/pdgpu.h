@@ -389,6 +389,66 @@ int64_t pdgpu_dcd_frame(pdgpu_ctx *ctx, int32_t id, uint8_t *out, int64_t cap);
 int pdgpu_dcd_create(const char *path, int32_t natoms);
 int pdgpu_dcd_append(pdgpu_ctx *ctx, int32_t id, const char *path);
 
+/* Bristol trajectory format (OutTra_BTF, fileio/tra.cpp:28-180, records of
+ * fileio/tratypes.cpp:12-226), the reference's own trajectory file.
+ * Includes are PD's BFT_StandardIncludes flags; positions are always
+ * written. */
+#define PDGPU_BTF_PHIPSI 2  /* per residue (phi, psi), radians, as float    */
+#define PDGPU_BTF_ENERGY 8  /* the 64-float energy record (DefaultIncludes) */
+#define PDGPU_BTF_FORCE  16 /* per atom p + f * 5e8 (f in N), as float      */
+
+/* One atom of the system definition block (BTF_SystemDefinitionEntry::
+ * setFrom, fileio/tratypes.cpp:130-172): the Particle's names (pdbname,
+ * type_name, rawname; NUL-terminated, at most the first 8 bytes are kept,
+ * zero-padded as strncpy does), its residue's name and index (parentname,
+ * ir), the reference position posRef (A) and isKnownStructure, the
+ * covalent 1-2 partners in cov12atom order (at most 6), and charge
+ * (e), radius (A), epsilon and mass in PD's units. */
+typedef struct pdgpu_btf_atom {
+    const char *pdbname, *type_name, *rawname, *parentname;
+    int32_t ir;
+    int32_t known_structure;
+    double ref[3];
+    int32_t n_cov12;
+    int32_t cov12[6];
+    double charge, radius, epsilon, mass;
+} pdgpu_btf_atom;
+
+/* What a frame's energy record takes from the workspace besides the last
+ * evaluation's energies: WorkSpace::Step (written as Step and as time),
+ * ene.cRMS, ene.dRMS, ene.ekin and ene.etot (J/molecule) and ene.data1..10
+ * (BTF_Energy::save, fileio/tratypes.cpp:194-231). */
+typedef struct pdgpu_btf_state {
+    int64_t step;
+    double crms, drms, ekin, etot;
+    double data[10];
+} pdgpu_btf_state;
+
+/* The file's start (BTF_Header::save then OutTra_BTF::create,
+ * tratypes.cpp:60-84, tra.cpp:28-89): the 18,084-byte header (version 2,
+ * Type = 1 | includes, residue and atom counts, the frame size, the offset
+ * of the first frame, the date, ID strings "null", PD's property and
+ * energy names; text areas zero), "SYSTDEFI", one 220-byte entry per atom,
+ * "TRASTART". date_dmy (day, month, year) may be NULL for today's local
+ * date. Where PD leaves bytes uninitialised (the ID strings after "null",
+ * parentname after its terminator, unused cov12atom slots) this writes
+ * zeros. */
+int64_t pdgpu_btf_header(const pdgpu_btf_atom *atoms, int32_t natoms, int32_t nresidues, int32_t includes,
+                         const int32_t *date_dmy, uint8_t *out, int64_t cap);
+
+/* One OutTra_BTF::append record of an instance (tra.cpp:91-180): "TRAE",
+ * its device positions as float x, y, z per atom; with PDGPU_BTF_PHIPSI,
+ * (phi, psi) of each residue from its backbone atoms (backbone: N, CA, C
+ * instance-local indices per residue, -1 where absent; MoleculeBase::
+ * calcResiduePhiPsi, system/molecule.cpp:613-651, 0 where undefined); with
+ * PDGPU_BTF_ENERGY, the energy record from the instance's last evaluation
+ * (kcal/mol) and *state; with PDGPU_BTF_FORCE, p + f * 5e8 of the last
+ * forces. pdgpu_btf_append does the same on a file. */
+int64_t pdgpu_btf_frame(pdgpu_ctx *ctx, int32_t id, int32_t includes, const int32_t *backbone, int32_t nresidues,
+                        const pdgpu_btf_state *state, uint8_t *out, int64_t cap);
+int pdgpu_btf_append(pdgpu_ctx *ctx, int32_t id, int32_t includes, const int32_t *backbone, int32_t nresidues,
+                     const pdgpu_btf_state *state, const char *path);
+
 /* MolecularDynamics::UpdateTra with an OutTra_NAMD in the workspace's
  * trajectory container (protocols/md.cpp:247, ProtocolBase::every): with
  * path non-NULL, the PD MD calls (pdgpu_md_vv_pd, _langevin_pd,

@@ -23,6 +23,7 @@
 #include "protocols/minimise.h"
 #include "protocols/md.h"
 #include "fileio/psfdcd.h"
+#include "fileio/tra.h"
 
 #include <stdint.h>
 #include <cstdio>
@@ -603,6 +604,49 @@ void run_case(FFParamSet &ffps, const Case &c, const std::string &outdir) {
                 }
                 r.d("io_dcd_xyz", fx);
                 r.b("io_dcd", slurp(stem + ".dcd"));
+                // the same three frames as a Bristol trajectory with every
+                // include (OutTra_BTF, fileio/tra.cpp:28-180), and what its
+                // header and records take from the workspace
+                {
+                    IO::OutTra_BTF btf(stem, ws);
+                    btf.create(IO::AllIncludes);
+                    for (int frame = 0; frame < 3; ++frame) {
+                        for (int i = 0; i < n; ++i) {
+                            const double s0 = frame == 2 ? 0.1 : 0.0;
+                            if (frame == 0) ws.cur.atom[i].p.setTo(x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]);
+                            else ws.cur.atom[i].p.setTo(xyz[3 * i] + s0, xyz[3 * i + 1] - s0, xyz[3 * i + 2] + 2 * s0);
+                        }
+                        btf.append();
+                    }
+                    r.b("io_btf", slurp(stem + ".tra"));
+                    std::vector<int8_t> names;
+                    std::vector<int32_t> cov, ncov, known, bb;
+                    std::vector<double> ref;
+                    for (int i = 0; i < n; ++i) {
+                        const Particle &a = ws.atom[i];
+                        const std::string f[4] = {a.pdbname, a.type_name, a.rawname, a.parentname};
+                        for (int q = 0; q < 4; ++q)
+                            for (int k = 0; k < 16; ++k) names.push_back(k < (int)f[q].size() ? f[q][k] : 0);
+                        ncov.push_back((int32_t)a.cov12atom.size());
+                        for (int k = 0; k < 6; ++k) cov.push_back(k < (int)a.cov12atom.size() ? a.cov12atom[k].i : -1);
+                        known.push_back(a.isKnownStructure() ? 1 : 0);
+                        ref.push_back(a.posRef().x); ref.push_back(a.posRef().y); ref.push_back(a.posRef().z);
+                    }
+                    for (size_t k = 0; k < ws.res.size(); ++k) {
+                        bb.push_back(ws.res[k].iN); bb.push_back(ws.res[k].iCA); bb.push_back(ws.res[k].iC);
+                    }
+                    r.b("io_names", names);
+                    r.i("io_cov12", cov); r.i("io_ncov12", ncov); r.i("io_known", known); r.d("io_ref", ref);
+                    r.i("io_backbone", bb);
+                    std::vector<double> st;
+                    st.push_back((double)ws.Step); st.push_back(ws.ene.cRMS); st.push_back(ws.ene.dRMS);
+                    st.push_back(ws.ene.ekin); st.push_back(ws.ene.etot);
+                    st.push_back(ws.ene.data1); st.push_back(ws.ene.data2); st.push_back(ws.ene.data3);
+                    st.push_back(ws.ene.data4); st.push_back(ws.ene.data5); st.push_back(ws.ene.data6);
+                    st.push_back(ws.ene.data7); st.push_back(ws.ene.data8); st.push_back(ws.ene.data9);
+                    st.push_back(ws.ene.data10);
+                    r.d("io_state", st);
+                }
             }
         }
     }

@@ -50,7 +50,11 @@ EXPORTS = (
     "pdgpu_md_beeman_pd", "pdgpu_mime_encode", "pdgpu_snapshot_mime", "pdgpu_dcd_header", "pdgpu_dcd_frame",
     "pdgpu_dcd_create", "pdgpu_dcd_append", "pdgpu_traj_dcd", "pdgpu_set_hydrogens",
     "pdgpu_set_thermostat", "pdgpu_nlist_export_noalloc", "pdgpu_options_default_nonbonded",
+    "pdgpu_btf_header", "pdgpu_btf_frame", "pdgpu_btf_append",
 )
+# BTF includes (PD's BFT_StandardIncludes)
+BTF_PHIPSI, BTF_ENERGY, BTF_FORCE = 2, 8, 16
+BTF_ALL = BTF_PHIPSI | BTF_ENERGY | BTF_FORCE
 # forcefield components (pdgpu_options.components)
 FF_GB, FF_BONDS, FF_ANGLES, FF_TORSIONS, FF_IMPROPERS = 1, 2, 4, 8, 16
 FF_BONDED = FF_BONDS | FF_ANGLES | FF_TORSIONS | FF_IMPROPERS
@@ -80,6 +84,55 @@ def dcd_header(natoms):
     buf = C.create_string_buffer(need)
     lib.pdgpu_dcd_header(natoms, buf, need)
     return buf.raw
+
+
+class BtfAtom(C.Structure):
+    _fields_ = [("pdbname", C.c_char_p), ("type_name", C.c_char_p), ("rawname", C.c_char_p),
+                ("parentname", C.c_char_p), ("ir", C.c_int32), ("known_structure", C.c_int32),
+                ("ref", C.c_double * 3), ("n_cov12", C.c_int32), ("cov12", C.c_int32 * 6),
+                ("charge", C.c_double), ("radius", C.c_double), ("epsilon", C.c_double), ("mass", C.c_double)]
+
+
+class BtfState(C.Structure):
+    _fields_ = [("step", C.c_int64), ("crms", C.c_double), ("drms", C.c_double), ("ekin", C.c_double),
+                ("etot", C.c_double), ("data", C.c_double * 10)]
+
+
+def btf_header(atoms, nresidues, includes=BTF_ALL, date=None):
+    """The start of a BTF file (bytes): header, system definition, TRASTART.
+    atoms: list of dicts with keys pdbname, type_name, rawname, parentname
+    (str), ir, known_structure (int), ref (3 floats), cov12 (up to 6 ints),
+    charge, radius, epsilon, mass; date: (day, month, year) or None."""
+    arr = (BtfAtom * max(1, len(atoms)))()
+    keep = []
+    for k, a in enumerate(atoms):
+        names = [a[f].encode() for f in ("pdbname", "type_name", "rawname", "parentname")]
+        keep += names
+        r = arr[k]
+        r.pdbname, r.type_name, r.rawname, r.parentname = names
+        r.ir, r.known_structure = int(a["ir"]), int(a["known_structure"])
+        r.ref[:] = [float(x) for x in a["ref"]]
+        cov = [int(x) for x in a["cov12"]]
+        r.n_cov12 = len(cov)
+        for q, j in enumerate(cov[:6]):
+            r.cov12[q] = j
+        r.charge, r.radius, r.epsilon, r.mass = (float(a[f]) for f in ("charge", "radius", "epsilon", "mass"))
+    d = None if date is None else (C.c_int32 * 3)(*[int(x) for x in date])
+    need = lib.pdgpu_btf_header(arr, len(atoms), nresidues, includes, d, None, 0)
+    if need < 0:
+        raise PdgpuError(need, "btf_header")
+    buf = C.create_string_buffer(need)
+    got = lib.pdgpu_btf_header(arr, len(atoms), nresidues, includes, d, buf, need)
+    if got != need:
+        raise PdgpuError(got, "btf_header")
+    return buf.raw
+
+
+def btf_state(step=0, crms=0.0, drms=0.0, ekin=0.0, etot=0.0, data=None):
+    s = BtfState()
+    s.step, s.crms, s.drms, s.ekin, s.etot = int(step), crms, drms, ekin, etot
+    s.data[:] = [float(x) for x in (data if data is not None else [0.0] * 10)]
+    return s
 
 
 class PdgpuError(RuntimeError):
@@ -175,6 +228,12 @@ def _load():
         "pdgpu_dcd_append": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
         "pdgpu_traj_dcd": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
         "pdgpu_set_hydrogens": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p]),
+        "pdgpu_btf_header": (C.c_int64, [P(BtfAtom), C.c_int32, C.c_int32, C.c_int32, P(C.c_int32), C.c_char_p,
+                                         C.c_int64]),
+        "pdgpu_btf_frame": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, P(BtfState),
+                                        C.c_char_p, C.c_int64]),
+        "pdgpu_btf_append": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, C.c_int32, P(BtfState),
+                                       C.c_char_p]),
         "pdgpu_set_thermostat": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
         "pdgpu_bondorder_build": (C.c_int, [C.c_int32, _ip, C.c_int32, _ip, _ip, C.c_int32, _ip, _ip, C.c_int32,
                                             _ip, _ip, _ip, _bp, _ip, _ip, _ip, _ip]),
@@ -551,6 +610,29 @@ class Context:
 
     def dcd_append(self, path, iid):
         self._check(lib.pdgpu_dcd_append(self.h, iid, str(path).encode()), "dcd_append")
+
+    def btf_frame(self, iid, includes=BTF_ALL, backbone=None, state=None):
+        """One BTF trajectory record of instance iid (bytes). backbone:
+        (nres, 3) instance-local N, CA, C indices (-1 where absent); state:
+        btf_state(...) for the energy record."""
+        bb = np.ascontiguousarray(backbone if backbone is not None else np.zeros((0, 3)), dtype=np.int32)
+        nres = len(bb)
+        st = state if state is not None else btf_state()
+        args = (self.h, iid, includes, _p(bb.ravel(), _ip), nres, C.byref(st))
+        need = lib.pdgpu_btf_frame(*args, None, 0)
+        if need < 0:
+            self._check(int(need), "btf_frame")
+        buf = C.create_string_buffer(need)
+        got = lib.pdgpu_btf_frame(*args, buf, need)
+        if got != need:
+            self._check(int(got) if got < 0 else PDGPU_ERR_HIP, "btf_frame")
+        return buf.raw
+
+    def btf_append(self, path, iid, includes=BTF_ALL, backbone=None, state=None):
+        bb = np.ascontiguousarray(backbone if backbone is not None else np.zeros((0, 3)), dtype=np.int32)
+        st = state if state is not None else btf_state()
+        self._check(lib.pdgpu_btf_append(self.h, iid, includes, _p(bb.ravel(), _ip), len(bb), C.byref(st),
+                                         str(path).encode()), "btf_append")
 
     def set_thermostat(self, tau, temperature=300.0):
         """Berendsen thermostat of the PD MD calls (tau in s; 0: off)."""

@@ -33,3 +33,35 @@ def load(name):
         if isinstance(d[k], np.ndarray) and d[k].shape == (1,) and d[k].dtype == np.float64:
             d[k] = float(d[k][0])
     return d
+
+
+def btf_atoms(g):
+    """The per-atom system-definition inputs of a BTF file (pdgpu.btf_header's
+    atom dicts) from a fixture's io_* records."""
+    n = g["n"]
+    names = g["io_names"].view(np.uint8).reshape(n, 4, 16)
+    text = lambda row: bytes(row).split(b"\0")[0].decode()
+    cov = g["io_cov12"].reshape(n, 6)
+    ref = g["io_ref"].reshape(n, 3)
+    return [dict(pdbname=text(names[i, 0]), type_name=text(names[i, 1]), rawname=text(names[i, 2]),
+                 parentname=text(names[i, 3]), ir=int(g["resid"][i]), known_structure=int(g["io_known"][i]),
+                 ref=ref[i], cov12=list(cov[i, :g["io_ncov12"][i]]), charge=g["charge"][i], radius=g["radius"][i],
+                 epsilon=g["epsilon"][i], mass=g["mass"][i]) for i in range(n)]
+
+
+def btf_undefined_bytes(g, length):
+    """Positions in a BTF file that PD writes from uninitialised memory: the
+    ID strings after "null" (BTF_Header, fileio/tratypes.cpp:24-27),
+    parentname after its terminator (strcpy into a reused entry) and the
+    cov12atom slots past n_cov12atoms (tratypes.cpp:144-171)."""
+    mask = np.zeros(length, bool)
+    for q in range(4):
+        mask[36 + 32 * q + 5:36 + 32 * (q + 1)] = True
+    n = g["n"]
+    names = g["io_names"].view(np.uint8).reshape(n, 4, 16)
+    for i in range(n):
+        base = 18084 + 8 + 220 * i
+        plen = len(bytes(names[i, 3]).split(b"\0")[0])
+        mask[base + 32 + plen + 1:base + 40] = True
+        mask[base + 56 + 4 * int(g["io_ncov12"][i]):base + 80] = True
+    return mask

@@ -1,13 +1,15 @@
 """PD's result hand-off formats written by the library's host code
-(pdgpu_mime_encode, pdgpu_dcd_header), against the bytes the compiled
-reference wrote (golden io_mime: WorkSpace::save().writeMIME after a
-VelocityVerlet run; io_dcd: OutTra_NAMD with three frames).
+(pdgpu_mime_encode, pdgpu_dcd_header, pdgpu_btf_header), against the bytes
+the compiled reference wrote (golden io_mime: WorkSpace::save().writeMIME
+after a VelocityVerlet run; io_dcd: OutTra_NAMD with three frames; io_btf:
+OutTra_BTF with every include and the same three frames).
 
 PD leaves some bytes uninitialised: 100 bytes of the MIME header
 (workspace/snapshot.cpp:321-330 allocates it with new[]) and each DCD title
 record past its 25 characters (fileio/psfdcd.cpp: an 80-byte fwrite of a
-stack buffer). Those bytes are zero here and are masked in the comparison;
-every other byte must match."""
+stack buffer), and in BTF the ID strings after "null", parentname past its
+terminator and unused cov12atom slots. Those bytes are zero here and are
+masked in the comparison; every other byte must match."""
 import numpy as np
 import pytest
 
@@ -101,3 +103,33 @@ def test_dcd_header_matches_reference(name):
     assert np.frombuffer(ours[268:272], np.int32)[0] == n
     # the frames that follow are 3 records of n floats each
     assert len(ref) == 276 + 3 * 3 * (8 + 4 * n)
+
+
+@pytest.mark.parametrize("name", [c for c in IO_CASES if "io_btf" in G.load(c)])
+def test_btf_header_matches_reference(name):
+    """pdgpu_btf_header against the start of the BTF file the compiled
+    reference's OutTra_BTF wrote (create(AllIncludes)): header, system
+    definition and TRASTART equal in every byte PD defines (the date is the
+    reference run's, read from its file)."""
+    g = G.load(name)
+    ref = np.frombuffer(g["io_btf"].view(np.uint8).tobytes(), np.uint8)
+    nres = len(g["io_backbone"]) // 3
+    date = np.frombuffer(ref[24:36].tobytes(), np.int32)
+    h = np.frombuffer(pdgpu.btf_header(G.btf_atoms(g), nres, pdgpu.BTF_ALL, date), np.uint8)
+    assert len(h) == 18084 + 8 + 220 * g["n"] + 8
+    undefined = G.btf_undefined_bytes(g, len(h))
+    assert undefined.sum() < 0.1 * len(h)
+    assert np.array_equal(h[~undefined], ref[:len(h)][~undefined])
+    # every undefined byte is zero in ours
+    assert not h[undefined].any()
+
+
+def test_btf_header_rejects_bad_input():
+    a = [dict(pdbname="N", type_name="N3", rawname="N", parentname="ALA", ir=0, known_structure=0, ref=(0, 0, 0),
+              cov12=list(range(7)), charge=0.1, radius=1.8, epsilon=0.1, mass=14.0)]
+    with pytest.raises(pdgpu.PdgpuError):
+        pdgpu.btf_header(a, 1)
+    a[0]["cov12"] = [1]
+    with pytest.raises(pdgpu.PdgpuError):
+        pdgpu.btf_header(a, 1, includes=32)
+    assert len(pdgpu.btf_header(a, 1, includes=pdgpu.BTF_ENERGY)) == 18084 + 8 + 220 + 8
