@@ -504,6 +504,78 @@ int pdgpu_fixed_terms(int32_t natoms, const double *gb_radius,
                       int32_t n13, const int32_t *a13, const int32_t *b13, const double *d13,
                       const pdgpu_options *o, double *vj, double *terms123);
 
+/* Input preparation (host only, no device work): what PD does before a
+ * forcefield sees a molecule.
+ *
+ * pdgpu_ff_read reads a PD parameter file (FFParamSet::readLib,
+ * forcefields/ffparam.cpp:1719-2200: TYPE, BOND, ANGLE, TORSION, IMPROPER,
+ * MOLECULE blocks with LOAD / DELATOM / ATOM_APPEND_MODE / CONNECT /
+ * CHARGE / IMPROPER, ALIAS, CLASS / CLASS_PREFIX, the 1-4 scalings,
+ * INCLUDE, and the SOLVATION_GENERALISED_BORN section's GENERALPARAMETERS
+ * and SOLVATIONTYPE radii, forcefields/gbff.cpp:185-290). Units become
+ * PD's: sqrt(J/molecule) epsilon, kg, J/molecule force constants, radians.
+ * pdgpu_prep_error() describes the last failure of these calls. */
+typedef struct pdgpu_ff pdgpu_ff;
+typedef struct pdgpu_molecule pdgpu_molecule;
+int pdgpu_ff_read(const char *path, pdgpu_ff **out);
+void pdgpu_ff_free(pdgpu_ff *ff);
+const char *pdgpu_prep_error(void);
+/* the file's VDW14SCALING, ELEC14SCALING and GB P1..P5 (gb_p: 5 values) */
+int pdgpu_ff_info(const pdgpu_ff *ff, double *vdw14, double *elec14, double *gb_p);
+
+/* NewProtein(ffps, sequence) (system/genpolymer.cpp:12-164,256-270): the
+ * sequence grammar of Sequence::BioSequence::append (sequence/
+ * sequence.cpp:165-452: '-' separated names, '(...)' one-letter runs,
+ * 'X[n]' and '<...>[n]' repeats, '*' polymer start / end with the class's
+ * terminal codes, aliases), residues placed by their CONNECT link points,
+ * the covalent 1-2 / 1-3 / 1-4 structure (system/molecule.cpp:123-322),
+ * an alpha helix and then the extended chain (phi = psi = omega = pi,
+ * system/molecule.cpp:677-721,1099-1180), and the FF_Bonded term lists
+ * with PD's wildcard parameter lookup and the equilibrium 1-2 / 1-3
+ * distances (forcefields/ffbonded.cpp:13-460). */
+int pdgpu_peptide_build(const pdgpu_ff *ff, const char *sequence, pdgpu_molecule **out);
+void pdgpu_molecule_free(pdgpu_molecule *m);
+/* counts: natoms, nresidues, n12, n13, n14 (pairs i < j), nbonds, nangles,
+ * ntorsions, nimpropers */
+int pdgpu_molecule_counts(const pdgpu_molecule *m, int32_t *counts);
+/* per-atom arrays (any may be NULL): positions (A), charge (e), radius
+ * (A), epsilon (sqrt(J/molecule)), mass (kg), the Still radius of the atom
+ * type, residue index, and Particle::isHydrogen (the type name starts with
+ * 'H', forcefields/ffparam.cpp:739-797) */
+int pdgpu_molecule_atoms(const pdgpu_molecule *m, double *xyz, double *charge, double *radius, double *epsilon,
+                         double *mass, double *gb_radius, int32_t *resid, uint8_t *is_hydrogen);
+/* pointers into the molecule: PDB name, type name, raw name, residue name */
+int pdgpu_molecule_atom_names(const pdgpu_molecule *m, int32_t i, const char **pdbname, const char **type_name,
+                              const char **rawname, const char **parentname);
+/* the 1-2 (order 2), 1-3 or 1-4 pairs i < j in PD's order (atom i
+ * ascending, then its cov12atom / cov13atom / cov14atom order), with the
+ * equilibrium distances of 1-2 and 1-3 pairs in d (A) */
+int pdgpu_molecule_pairs(const pdgpu_molecule *m, int32_t order, int32_t *a, int32_t *b, double *d);
+/* each atom's cov12atom list (6 slots, -1 past n_cov12) */
+int pdgpu_molecule_cov12(const pdgpu_molecule *m, int32_t *cov12, int32_t *n_cov12);
+/* per residue N, CA, C atom indices (-1 where absent) */
+int pdgpu_molecule_backbone(const pdgpu_molecule *m, int32_t *n_ca_c);
+/* MoleculeBase::setResiduePhiPsi, then setResidueOmega unless omega is
+ * NaN or ir == 0 (radians); and calcResiduePhiPsi / calcResidueOmega */
+int pdgpu_molecule_set_phipsi(pdgpu_molecule *m, int32_t ir, double phi, double psi, double omega);
+int pdgpu_molecule_get_phipsi(const pdgpu_molecule *m, int32_t ir, double *phi, double *psi, double *omega);
+/* the FF_Bonded term lists into caller arrays sized by the counts, in the
+ * layout pdgpu_bonded_set takes */
+typedef struct pdgpu_bonded_out {
+    int32_t nbonds, nangles, ntorsions, nimpropers;
+    int32_t *bond_ij;
+    double *bond_lk;
+    int32_t *angle_iaj;
+    double *angle_tk;
+    int32_t *torsion_iabj;
+    int32_t *torsion_terms;
+    double *torsion_vn, *torsion_n, *torsion_gamma;
+    int32_t *improper_iabj;
+    int32_t *improper_terms;
+    double *improper_vn, *improper_n, *improper_gamma;
+} pdgpu_bonded_out;
+int pdgpu_molecule_bonded(const pdgpu_molecule *m, pdgpu_bonded_out *out);
+
 /* Device bytes one instance of natoms atoms with the given bond-order sizes
  * would take in this context (admission control). */
 int64_t pdgpu_instance_bytes(const pdgpu_ctx *ctx, int32_t natoms, int32_t max_index_delta,

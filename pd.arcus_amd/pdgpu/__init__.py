@@ -51,6 +51,10 @@ EXPORTS = (
     "pdgpu_dcd_create", "pdgpu_dcd_append", "pdgpu_traj_dcd", "pdgpu_set_hydrogens",
     "pdgpu_set_thermostat", "pdgpu_nlist_export_noalloc", "pdgpu_options_default_nonbonded",
     "pdgpu_btf_header", "pdgpu_btf_frame", "pdgpu_btf_append",
+    "pdgpu_ff_read", "pdgpu_ff_free", "pdgpu_prep_error", "pdgpu_ff_info", "pdgpu_peptide_build",
+    "pdgpu_molecule_free", "pdgpu_molecule_counts", "pdgpu_molecule_atoms", "pdgpu_molecule_atom_names",
+    "pdgpu_molecule_pairs", "pdgpu_molecule_cov12", "pdgpu_molecule_backbone", "pdgpu_molecule_set_phipsi",
+    "pdgpu_molecule_get_phipsi", "pdgpu_molecule_bonded",
 )
 # BTF includes (PD's BFT_StandardIncludes)
 BTF_PHIPSI, BTF_ENERGY, BTF_FORCE = 2, 8, 16
