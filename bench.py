@@ -439,7 +439,7 @@ def run_batch(args, pdgpu, mols):
         "metric": "atom-pair evals/s", "value": pairs * args.steps / dt, "unit": "atom-pair evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32 pair maths, int64 fixed-point accumulation",
-        "data": "synthetic",
+        "data": "synthetic" if args.ff is None else f"synthetic sequences built by NewProtein from {args.ff}",
         "config": {"workload": f"{args.conformers} extended conformers, 10-150 residues, one batched energy+force "
                                "evaluation per step, 10 A cutoff + Still GB, mixed precision",
                    "atoms": natoms, "pairs_per_step": pairs, "parallelism": "single GPU",
@@ -485,6 +485,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--build-workers", type=int, default=0,
                     help="processes building the synthetic inputs (0: up to 16; 1 under a profiler)")
+    ap.add_argument("--ff", default=None,
+                    help="a PD parameter file: build every workload with PD's NewProtein from it "
+                         "(the same sequences) instead of the synthetic builder")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -492,6 +495,8 @@ def main():
         sys.exit(2)
     sections = args.sections.split(",")
     import pdgpu
+    from pdgpu import workloads as W
+    W.use_parameter_file(args.ff)
     # every input is built before the first GPU context: the builders fork
     batch_mols = build_batch(args) if "batch" in sections else None
     md_mol = build_md() if "md" in sections else None

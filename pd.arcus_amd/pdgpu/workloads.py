@@ -16,10 +16,66 @@ that both run exactly the same inputs.
            every pair, LJ at 12 A, velocity-Verlet MD
 
 Molecules come from pdgpu.synth (seeded; see its docstring for what it
-builds)."""
+builds), or, after use_parameter_file(path) with a PD parameter file, from
+PD's own builder (pdgpu.prep: NewProtein of the same sequences, the
+chain's phi/psi drawn from the same basins), so that the workloads run on
+the reference's molecules wherever its parameter file is at hand."""
 import numpy as np
 
 from . import PDGPU_ALL, default_options, synth  # noqa: F401
+
+_FF_PATH = None
+_FF_CACHE = {}
+
+
+def use_parameter_file(path):
+    """Build every later workload molecule with PD's NewProtein from this
+    parameter file (None: back to pdgpu.synth)."""
+    global _FF_PATH
+    _FF_PATH = None if path is None else str(path)
+
+
+def _ff(path):
+    from . import prep
+    if path not in _FF_CACHE:
+        _FF_CACHE[path] = prep.read_ff(path)
+    return _FF_CACHE[path]
+
+
+def basin_angles(seed, n):
+    """phi/psi/omega (rad) per residue: 60 % alpha basin (-63, -43), 40 %
+    beta (-120, 130), 15 degree spread, trans peptide bonds."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((n, 3))
+    for i in range(n):
+        if rng.random() < 0.6:
+            out[i, :2] = -63.0 + rng.normal(0, 15.0), -43.0 + rng.normal(0, 15.0)
+        else:
+            out[i, :2] = -120.0 + rng.normal(0, 15.0), 130.0 + rng.normal(0, 15.0)
+    out[:, :2] *= np.pi / 180.0
+    out[:, 2] = np.pi
+    return out
+
+
+def _pd_job(job):
+    from . import prep
+    path, seed, residues, mode = job
+    seq = synth.random_sequence(seed, residues)
+    phipsi = basin_angles(seed, residues) if mode == "basins" else None
+    return _ff(path).peptide(prep.pd_grammar(seq), phipsi)
+
+
+def _many(seeds, residues, mode="extended", workers=0, bonded=False, start="fork"):
+    if _FF_PATH is None:
+        return synth.many(seeds, residues, mode=mode, workers=workers, bonded=bonded, start=start)
+    import multiprocessing as mp
+    import os
+    jobs = [(_FF_PATH, s, r, mode) for s, r in zip(seeds, residues)]
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(jobs) < 8:
+        return [_pd_job(j) for j in jobs]
+    with mp.get_context(start).Pool(workers) as pool:
+        return pool.map(_pd_job, jobs, chunksize=max(1, len(jobs) // (4 * workers)))
 
 KB = 1.380649e-23  # J/K
 
@@ -65,10 +121,12 @@ def batch_lengths(n=4096):
 def batch_molecules(n=4096, workers=0, start="fork"):
     """The batch: n extended conformers, lengths uniform in 10-150 residues,
     conformer k built from seed 1000 + k."""
-    return synth.many([1000 + k for k in range(n)], batch_lengths(n), workers=workers, start=start)
+    return _many([1000 + k for k in range(n)], batch_lengths(n), workers=workers, start=start)
 
 
 def chain_molecule():
+    if _FF_PATH is not None:
+        return _pd_job((_FF_PATH, 2, 500, "basins"))
     return synth.peptide(seed=2, residues=500, mode="basins", bonded=True)
 
 
@@ -79,10 +137,12 @@ def serving_sizes(n=200):
 def serving_molecules(n=200, workers=0, start="fork"):
     """n requests, sizes log-uniform in 20-400 residues, request k built
     from seed 5000 + k, with FF_Bonded terms."""
-    return synth.many([5000 + k for k in range(n)], serving_sizes(n), bonded=True, workers=workers, start=start)
+    return _many([5000 + k for k in range(n)], serving_sizes(n), bonded=True, workers=workers, start=start)
 
 
 def allpairs_molecule(residues=2000):
+    if _FF_PATH is not None:
+        return _pd_job((_FF_PATH, 2000, residues, "extended"))
     return synth.peptide(seed=2000, residues=residues, bonded=True)
 
 

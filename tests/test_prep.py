@@ -140,3 +140,25 @@ def test_phipsi_setters_read_back_and_keep_bond_lengths(ff):
 def test_missing_parameter_file_is_an_error():
     with pytest.raises(pdgpu.PdgpuError):
         prep.read_ff("/nonexistent/file.ff")
+
+
+def test_workloads_build_from_the_parameter_file():
+    """use_parameter_file switches the workload builders to NewProtein of
+    the same sequences and lengths."""
+    from pdgpu import workloads as W
+    W.use_parameter_file(PARAM)
+    try:
+        mols = W.batch_molecules(4, workers=1)
+        ch = W.chain_molecule()
+    finally:
+        W.use_parameter_file(None)
+    lengths = W.batch_lengths(4)
+    for k, m in enumerate(mols):
+        assert m["nres"] == lengths[k]
+        assert m["sequence"] == prep.pd_grammar(synth.random_sequence(1000 + k, lengths[k]))
+        for key in ("charge", "radius", "epsilon", "gb_radius", "vj", "terms123", "mid", "band", "xyz", "bd_idx"):
+            assert key in m
+    assert ch["nres"] == 500
+    # the chain takes the basin angles
+    pp = prep.Molecule(prep.read_ff(PARAM), ch["sequence"], W.basin_angles(2, 500)).phipsi()
+    assert abs(np.angle(np.exp(1j * (pp[10, 0] - W.basin_angles(2, 500)[10, 0])))) < 1e-9
