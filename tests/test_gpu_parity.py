@@ -189,3 +189,26 @@ def test_dense_offdiagonal_tiles(name, precision):
         e, f = ctx.calc_energies(i)
         assert rel(e["epot"], g["en_epot"]) <= tol
         assert rms_rel(f, g["en_force"]) <= (1e-9 if precision == pdgpu.FP64 else 1e-4)
+
+
+@pytest.mark.parametrize("steps", [3, 4])
+def test_fp64_sd_uphill_rollback(steps):
+    """A step size 250 times PD's default sends step 0 uphill: the device
+    minimiser restores the saved positions and halves-over-1.2 the step as
+    PD does (three steps end bitwise on the start; four end on the accepted
+    shorter step), as the oracle's closed-form-pinned run does."""
+    import oracle_lib as O
+    g = G.load("c1_seed1")
+    S = 2e5 * 250
+    p = O.default_params(**g["param_dict"])
+    s = O.OSystem(g["charge"], g["radius"], g["epsilon"], g["gb_radius"], g["vj"], g["terms123"], g["mass"],
+                  g["mid"], g["band"], g["sp_i"], g["sp_j"], g["sp_o"])
+    _, xo, trace, _, _ = O.minimise_sd(s, g["xyz"], p, steps, S, 1)
+    assert trace[1] > trace[0]
+    with context_for(g) as ctx:
+        i = ctx.add_system(g)
+        ctx.minimise_sd(i, steps, S, 1)
+        xyz = ctx.get_positions(i)
+    if steps == 3:
+        assert np.array_equal(xyz, g["xyz"])
+    assert np.max(np.abs(xyz - xo)) <= 1e-9
