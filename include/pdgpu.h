@@ -93,6 +93,15 @@ typedef struct pdgpu_options {
     double softvdw_hardness;      /* Hardness, kcal/mol (default 12)               */
     double softvdw_olap_offset;   /* OlapOffset, A (default 0.6)                   */
     double softvdw_cutoff;        /* VdwCutoff of FF_SoftVDW, A (default 15)       */
+    int32_t tile_split;           /* wavefronts per block-pair tile: 0 or 1 (one), */
+                                  /* 2, 4, 8 or 16. Each takes its own share of    */
+                                  /* the tile's candidates / rotation steps and    */
+                                  /* flushes its own sums, so results are          */
+                                  /* deterministic for a given value but differ in */
+                                  /* rounding between values. For single systems   */
+                                  /* of a few thousand atoms, whose few tiles      */
+                                  /* leave the GPU idle behind the longest one;    */
+                                  /* batches run best at 1. Not PD's (no knob).    */
 } pdgpu_options;
 
 /* Forcefield components of a context: FF_GeneralizedBorn_Still and the four

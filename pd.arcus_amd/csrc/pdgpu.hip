@@ -180,7 +180,7 @@ struct pdgpu_ctx {
     DevBuf px, py, pz, q, lj64, gb64, t123, xyzq32, lj32, gb32, pp32, born, acc_g, acc_deda, acc_fx, acc_fy, acc_fz;
     DevBuf fx, fy, fz, deda_out, vx, vy, vz, minv, opx, opy, opz, rx, ry, rz;
     // per block
-    DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, row_count, row_off, tiles, tile_slot;
+    DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, blk_acc, row_count, row_off, tiles, tile_slot;
     // per instance
     DevBuf any_need, inst_need, inst_moved, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
     DevBuf fox, foy, foz; // Beeman's old forces (N), per atom
@@ -249,6 +249,7 @@ struct pdgpu_ctx {
         s.fx = (double *)fx.p; s.fy = (double *)fy.p; s.fz = (double *)fz.p;
         s.deda_out = (double *)deda_out.p;
         s.blk_inst = (const int *)blk_inst.p; s.blk_n = (const int *)blk_n.p; s.blk_first = (const int *)blk_first.p;
+        s.blk_acc = (unsigned long long *)blk_acc.p;
         s.blk_ctr = (double4 *)blk_ctr.p; s.blk_half = (float4 *)blk_half.p; s.blk_box = (double4 *)blk_box.p;
         s.row_count = (int *)row_count.p; s.row_off = (int *)row_off.p; s.tiles = (int2 *)tiles.p;
         s.tile_slot = (int *)tile_slot.p;
@@ -406,6 +407,7 @@ int grow_atoms(pdgpu_ctx *c, int64_t blocks) {
     TRY(resize(c, c->blk_ctr, blocks, 32));
     TRY(resize(c, c->blk_box, blocks, 32));
     TRY(resize(c, c->blk_half, blocks, 16));
+    TRY(resize(c, c->blk_acc, (size_t)blocks * kBAcc, 8));
     TRY(resize(c, c->row_count, blocks, 4));
     TRY(resize(c, c->row_off, blocks + 1, 4));
     const int64_t old = c->cap_blocks;
@@ -487,6 +489,8 @@ void setup_params(pdgpu_ctx *c) {
     }
     P.elec_mode = o.elec_mode;
     P.vdw_mode = o.vdw_mode;
+    P.lg_split = 0;
+    while ((1 << P.lg_split) < o.tile_split) ++P.lg_split;
 }
 
 bool valid_id(const pdgpu_ctx *c, int id) {
@@ -569,7 +573,7 @@ int ensure_plist(pdgpu_ctx *c, int64_t tiles) {
     release(c->tile_cnt);
     c->plist_cap = 0;
     TRY(resize(c, c->plist, (size_t)cap * kTileCap, 2));
-    TRY(resize(c, c->tile_cnt, (size_t)cap, 8));
+    TRY(resize(c, c->tile_cnt, (size_t)cap << c->P.lg_split, 8)); // one record per segment
     c->plist_cap = cap;
     return PDGPU_OK;
 }
@@ -666,7 +670,7 @@ template <typename R>
 int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, int mode) {
     const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk, na = a1 - a0;
     int g[kGridKinds], t[kGridKinds];
-    for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, tiles, &g[k], &t[k]);
+    for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, tiles << c->P.lg_split, &g[k], &t[k]); // work items
     {
         ProfScope ps(c, PDGPU_KERNEL_ENUM);
         hipLaunchKernelGGL(k_filter<R>, dim3(g[kGridEnum]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
@@ -731,7 +735,7 @@ int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2
         // FF_NonBonded alone: the energy path's pair term without the GB
         // energy (MODE 2); its calcEnergies is the same evaluation
         int g[kGridKinds], t[kGridKinds];
-        for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, span_tiles(c, id), &g[k], &t[k]);
+        for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, span_tiles(c, id) << c->P.lg_split, &g[k], &t[k]);
         {
             ProfScope ps(c, PDGPU_KERNEL_ENUM);
             hipLaunchKernelGGL(k_filter<R>, dim3(g[kGridEnum]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
@@ -745,7 +749,7 @@ int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2
         // FF_SoftVDW over the cutoff-filtered lists (calcEnergies is its
         // calcForces, ffsoftvdw.cpp:56-59)
         int g[kGridKinds], t[kGridKinds];
-        for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, span_tiles(c, id), &g[k], &t[k]);
+        for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, span_tiles(c, id) << c->P.lg_split, &g[k], &t[k]);
         {
             ProfScope ps(c, PDGPU_KERNEL_ENUM);
             hipLaunchKernelGGL(k_filter<R>, dim3(g[kGridEnum]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
@@ -993,6 +997,9 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
         o->vdw_inner_cutoff > o->vdw_cutoff || !(o->p5 > 0) || !(o->dielectric_solute > 0) ||
         !(o->dielectric_solvent > 0) || o->nlist_padding < 0)
         return PDGPU_ERR_INVALID;
+    // segments per tile: 0 (= 1) or a power of two up to kMaxSplit
+    if (o->tile_split < 0 || o->tile_split > kMaxSplit || (o->tile_split & (o->tile_split - 1)) != 0)
+        return PDGPU_ERR_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= o->device || o->device < 0) return PDGPU_ERR_NODEVICE;
     hipDeviceProp_t prop;
@@ -1062,7 +1069,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
     DevBuf *all[] = {&c->px, &c->py, &c->pz, &c->q, &c->lj64, &c->gb64, &c->t123, &c->xyzq32, &c->lj32, &c->gb32,
                      &c->pp32, &c->born, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy, &c->acc_fz, &c->fx, &c->fy, &c->fz,
                      &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
-                     &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->row_count, &c->row_off,
+                     &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->blk_acc, &c->row_count, &c->row_off,
                      &c->tiles, &c->tile_slot, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
                      &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
                      &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
@@ -1085,9 +1092,10 @@ extern "C" int64_t pdgpu_instance_bytes(const pdgpu_ctx *c, int32_t natoms, int3
     if (!c || natoms <= 0) return -1;
     const int64_t nb = (natoms + kBlk - 1) / kBlk;
     const int64_t per_atom = 8 * 24 + 16 * 2 + (c->mixed ? 4 + 16 + 16 : 8);
-    const int64_t per_block = 4 * 3 + 32 * 2 + 16 + 4 * 2;
-    // per possible tile: the tile record, a candidate slot and a pair-list slot
-    const int64_t per_tile = 8 + 2 * kTileCap + 4 + 2 * kTileCap + 8;
+    const int64_t per_block = 4 * 3 + 32 * 2 + 16 + 8 * kBAcc + 4 * 2;
+    // per possible tile: the tile record, a candidate slot, a pair-list slot
+    // and its segment counts
+    const int64_t per_tile = 8 + 2 * kTileCap + 4 + 2 * kTileCap + (8 << c->P.lg_split);
     return nb * kBlk * per_atom + nb * per_block + (int64_t)mid * nb * kBlk + nsparse * 16 +
            nb * (nb + 1) / 2 * per_tile;
 }

@@ -85,18 +85,17 @@ template <typename R> struct GradSink {
     }
 };
 
-// Per-instance energy slot += e for every valid lane. Called by the whole
-// wavefront. In the mixed path each term's energy is rounded to fixed point
-// on its own and the integers are summed, so an instance's total does not
-// depend on which other terms share its wavefront.
+// Energy slot of the block record += e for every valid lane (blk: the block
+// of the term's first atom). Called by the whole wavefront; lanes are grouped
+// by record field, one atomic per distinct field (a wavefront's terms mostly
+// share one or two blocks). In the mixed path each term's energy is rounded
+// to fixed point on its own and the integers are summed, so an instance's
+// total does not depend on which other terms share its wavefront.
 template <typename R>
-__device__ void add_term_energy(const DevState &S, bool valid, int inst, int slot, double e) {
-    const unsigned long long vm = __ballot(valid);
-    if (!vm) return;
-    const int lead = __ffsll((unsigned long long)vm) - 1;
-    const int key = valid ? inst * kENum + slot : -1;
-    const int key0 = __shfl(key, lead, 64);
-    const bool same = __ballot(valid && key != key0) == 0;
+__device__ void add_term_energy(const DevState &S, bool valid, int inst, int blk, int slot, double e) {
+    unsigned long long rem = __ballot(valid);
+    if (!rem) return;
+    const int key = valid ? blk * kBAcc + slot : -1;
     if constexpr (Prec<R>::fixed) {
         long long iv = 0;
         if (valid) {
@@ -104,21 +103,23 @@ __device__ void add_term_energy(const DevState &S, bool valid, int inst, int slo
             if (fabs(s) < kFixedLimit) iv = to_fixed(s);
             else atomicOr(S.inst_status + inst, (s == s) ? kStatOverflow : kStatNonFinite);
         }
-        unsigned long long *acc = reinterpret_cast<unsigned long long *>(S.inst_energy);
-        if (same) {
-            const long long tot = wave_sum(iv);
-            if (lane_id() == lead && tot != 0) atomicAdd(acc + key0, (unsigned long long)tot);
-        } else if (valid && iv != 0) {
-            atomicAdd(acc + key, (unsigned long long)iv);
+        while (rem) {
+            const int lead = __ffsll((unsigned long long)rem) - 1;
+            const int key0 = __shfl(key, lead, 64);
+            const bool mine = key == key0;
+            rem &= ~__ballot(mine);
+            const long long tot = wave_sum(mine ? iv : 0ll);
+            if (lane_id() == lead && tot != 0) atomicAdd(S.blk_acc + key0, (unsigned long long)tot);
         }
     } else {
-        double *acc = reinterpret_cast<double *>(S.inst_energy);
         if (valid && !(fabs(e) < 1e300)) atomicOr(S.inst_status + inst, kStatNonFinite);
-        if (same) {
-            const double tot = wave_sum(valid ? e : 0.0);
-            if (lane_id() == lead && tot != 0.0) unsafeAtomicAdd(acc + key0, tot);
-        } else if (valid && e != 0.0) {
-            unsafeAtomicAdd(acc + key, e);
+        while (rem) {
+            const int lead = __ffsll((unsigned long long)rem) - 1;
+            const int key0 = __shfl(key, lead, 64);
+            const bool mine = key == key0;
+            rem &= ~__ballot(mine);
+            const double tot = wave_sum(mine ? e : 0.0);
+            if (lane_id() == lead && tot != 0.0) unsafeAtomicAdd(reinterpret_cast<double *>(S.blk_acc) + key0, tot);
         }
     }
 }
@@ -241,8 +242,8 @@ __global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
         const int meta = B.dh_meta[r];
         e = dihedral_term<R>(S, G, q, (meta >> 24) & 7, B.dh_vn[r], B.dh_n[r], B.dh_gamma[r], S.inst_status + inst);
     }
-    add_term_energy<R>(S, kind >= 0, inst, slot, e);
-    add_term_energy<R>(S, kind == 0, inst, kEVirial, vir);
+    add_term_energy<R>(S, kind >= 0, inst, q.x / kBlk, slot, e);
+    add_term_energy<R>(S, kind == 0, inst, q.x / kBlk, kEVirial, vir);
     __syncthreads();
     // flush the window (only atoms that received something)
     for (int k = threadIdx.x; k < kWin; k += blockDim.x) {
@@ -267,6 +268,7 @@ __global__ void k_clear_grad(DevState S, int atom_begin, int atom_end, int slot_
     if (slot_begin + t < slot_end) inst_clear(S, slot_begin + t); // the per-instance records too
     const int a = atom_begin + t;
     if (a >= atom_end) return;
+    blk_clear(S, a);
     acc_store<R>(S.acc_fx, a, 0.0, 1.0);
     acc_store<R>(S.acc_fy, a, 0.0, 1.0);
     acc_store<R>(S.acc_fz, a, 0.0, 1.0);

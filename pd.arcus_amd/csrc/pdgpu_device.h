@@ -34,6 +34,8 @@ constexpr int kTileSparse = 4; // some off-diagonal bond order falls in the tile
 // slots kEVdw .. kEImproper
 enum { kEVdw = 0, kEElec, kEPolCross, kEPolSelf, kEBond, kEAngle, kETorsion, kEImproper, kEVirial, kENum = 10 };
 enum { kCPairs = 0, kCBornPairs, kCNum = 2 };
+// per-block record of an evaluation: energy slots kEVdw .. then the counters
+constexpr int kBAcc = kENum + kCNum;
 // per-instance status bits
 constexpr int kStatNonFinite = 1;
 constexpr int kStatOverflow = 2;
@@ -72,6 +74,7 @@ struct PairParams {
     double cut, inner2;
     int elec_mode, vdw_mode;
     int dense_pairs; // off-diagonal tiles with at least this many candidates are stored dense
+    int lg_split;    // log2 of the segments each tile is split into (tile_items)
 };
 
 // Device pointers of one context.
@@ -122,6 +125,11 @@ struct DevState {
     void *inst_energy;            // [slot*kENum]
     unsigned long long *inst_count; // [slot*kCNum]
     int *inst_status;
+    // per block [block*kBAcc]: the evaluation's partial energies (fixed point
+    // or double bits, per the precision) and pair counts of the tiles of its
+    // I row and of its bonded terms, folded into the instance record by the
+    // evaluation's last kernel
+    unsigned long long *blk_acc;
 };
 
 // ---------------------------------------------------------------- helpers
@@ -240,6 +248,48 @@ __device__ __forceinline__ int pair_bondorder_lds(const DevState &S, int inst, i
     return pair_bondorder(S, inst, il, jl, flags & kTileSparse);
 }
 
+// Work items of the tile kernels. With PairParams::lg_split = g every tile
+// is split into 2^g segments, each processed by one wavefront on its own:
+//  - candidates: segment s owns candidates [s*c, (s+1)*c) of the tile's
+//    candidate slot, c being the count over 2^g rounded up to a multiple of
+//    4 (so that 8-byte candidate loads stay aligned);
+//  - lists: the region [s*kTileCap/2^g, (s+1)*kTileCap/2^g) of the tile's
+//    list, into which k_filter compacts that segment's candidates, with its
+//    own (born-side, other) counts in tile_cnt[t*2^g + s];
+//  - dense tiles: rotation steps [s*K/2^g, (s+1)*K/2^g) of the tile's K.
+// Items are numbered segment-major: item u of a launch over nt tiles from
+// t_begin is segment u / nt of tile t_begin + u % nt, so the chunk of
+// consecutive items a wavefront takes holds different tiles and a tile's
+// segments go to different wavefronts. g = 0 is one wavefront per tile. A
+// larger g shortens the longest chain of dependent work of a launch (a full
+// list tile is 64 steps of 64 pairs) for systems with too few tiles to fill
+// the GPU; per-segment sums are flushed on their own, so results depend on g
+// (deterministically) and not on the grid. Per-segment list counts are kept
+// at tile_cnt[(t << g) + s].
+struct TileItem {
+    int t, s;
+    size_t rec; // index of the segment's tile_cnt record
+};
+__device__ __forceinline__ TileItem tile_item(int u, int t_begin, int nt, int lg) {
+    if (lg == 0) return {t_begin + u, 0, (size_t)(t_begin + u)};
+    const int s = u / nt, t = t_begin + (u - s * nt);
+    return {t, s, ((size_t)t << lg) + s};
+}
+
+// First and one-past-last rotation step of segment s of a dense tile with
+// `steps` steps.
+__device__ __forceinline__ int seg_step(int s, int steps, int lg) { return (s * steps) >> lg; }
+
+// v as lane (l + k*ldir) & 63 holds it, i.e. after k rotations
+template <typename T> __device__ __forceinline__ T rot_to(T v, int k, int ldir) {
+    return (k * ldir) & 63 ? __shfl(v, (lane_id() + k * ldir) & 63, 64) : v;
+}
+// j-side sums travelling with their atoms for k rotations, brought back to
+// the atom's own lane
+template <typename T> __device__ __forceinline__ T rot_home(T v, int k, int ldir) {
+    return (k * ldir) & 63 ? __shfl(v, (lane_id() - k * ldir) & 63, 64) : v;
+}
+
 // Persistent-grid chunk assignment. Consecutive chunks go to workgroups on
 // the same XCD (workgroups are dealt round-robin over 8 XCDs), so the j-atom
 // data of neighbouring tiles is shared through one L2.
@@ -251,6 +301,31 @@ __device__ __forceinline__ int wave_global_id() {
 }
 
 // ------------------------------------------------------ per-instance setup
+// Lane a % 64 < kBAcc of a block-aligned per-atom launch clears one field of
+// its block's record.
+__device__ __forceinline__ void blk_clear(const DevState &S, int a) {
+    const int f = a & (kBlk - 1);
+    if (f < kBAcc) S.blk_acc[(size_t)(a / kBlk) * kBAcc + f] = 0ull;
+}
+
+// Adds block b's record into its instance's (one wavefront per block, in the
+// evaluation's last kernel): at most kBAcc atomics per block instead of one
+// per wavefront of every pair kernel on the same few addresses.
+template <typename R> __device__ __forceinline__ void blk_fold(const DevState &S, int b, int inst) {
+    const int f = lane_id();
+    if (inst < 0 || f >= kBAcc) return;
+    const unsigned long long v = S.blk_acc[(size_t)b * kBAcc + f];
+    if (v == 0ull) return;
+    if (f >= kENum) {
+        atomicAdd(S.inst_count + (size_t)inst * kCNum + (f - kENum), v);
+    } else if constexpr (Prec<R>::fixed) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(S.inst_energy) + (size_t)inst * kENum + f, v);
+    } else {
+        unsafeAtomicAdd(reinterpret_cast<double *>(S.inst_energy) + (size_t)inst * kENum + f,
+                        __longlong_as_double((long long)v));
+    }
+}
+
 __device__ __forceinline__ void inst_clear(const DevState &S, int s) {
     unsigned long long *e = reinterpret_cast<unsigned long long *>(S.inst_energy) + (size_t)s * kENum;
     for (int k = 0; k < kENum; ++k) e[k] = 0;
@@ -420,6 +495,7 @@ __global__ void k_prepare(DevState S, int atom_begin, int atom_end, int slot_beg
     if (slot_begin + t < slot_end) inst_clear(S, slot_begin + t); // the per-instance records too
     const int a = atom_begin + t;
     if (a >= atom_end) return;
+    blk_clear(S, a);
     acc_store<R>(S.acc_g, a, 0.0, 1.0);
     if constexpr (Prec<R>::fixed) store_xyzq32(S, a);
 }
@@ -481,6 +557,7 @@ template <typename R> __device__ __forceinline__ R ld_born(const DevState &S, si
 // more LDS wait cycles than ds_add_f64 for the same access pattern (pair
 // kernel 2.3 ms -> 0.39 ms on 1024 conformers).
 constexpr int kTileCap = 4096;
+constexpr int kMaxSplit = 16; // segments per tile at most (256-entry list regions)
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -618,14 +695,18 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
     __shared__ PosR<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
     const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
+    const int lg = P.lg_split;
+    const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
-    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
     const R cut2 = (R)P.cut2, still2 = (R)P.still2;
     const unsigned long long below = lanes_below();
-    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
-        const int cend = min(t_end, c + tpw);
-        for (int t = c; t < cend; ++t) {
+    for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
+        const int cend = min(u_end, c + tpw);
+        for (int u = c; u < cend; ++u) {
+            const TileItem it = tile_item(u, t_begin, nt, lg);
+            const int t = it.t;
             // the tile record and its candidate slot come first; the slot's
             // count and first 256 candidates (every slot holds kTileCap
             // entries, and tile_slot only ever names allocated slots) are
@@ -641,19 +722,28 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0 || t >= S.plist_cap) {
-                if (l == 0 && t < S.plist_cap) S.tile_cnt[t] = make_int2(0, 0);
+                if (l == 0 && t < S.plist_cap) S.tile_cnt[it.rec] = make_int2(0, 0);
                 if (l == 0 && t >= S.plist_cap) atomicOr(S.overflow, 1);
                 continue;
             }
             if ((tw.y >> 24) & kTileDiag) { // dense tiles are evaluated from their bit rows
-                if (l == 0) S.tile_cnt[t] = make_int2(0, 0);
+                if (l == 0) S.tile_cnt[it.rec] = make_int2(0, 0);
                 continue;
             }
             if (n < 0) { // dense off-diagonal tile
-                if (l == 0) S.tile_cnt[t] = make_int2(-1, -1);
+                if (l == 0) S.tile_cnt[it.rec] = make_int2(-1, -1);
                 continue;
             }
-            uint16_t *L = S.plist + (size_t)t * kTileCap;
+            // this segment's candidates [c0, c1) and list region
+            const int segc = (((n + (1 << lg) - 1) >> lg) + 3) & ~3;
+            const int c0 = it.s * segc, c1 = min(n, c0 + segc);
+            if (c0 >= c1) {
+                if (l == 0) S.tile_cnt[it.rec] = make_int2(0, 0);
+                continue;
+            }
+            if (c0 > 0 && (c0 >> 2) + l < kTileCap / 4) v_nx = C4[(c0 >> 2) + l];
+            const int capm1 = (kTileCap >> lg) - 1;
+            uint16_t *L = S.plist + (size_t)t * kTileCap + (size_t)it.s * (kTileCap >> lg);
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -661,16 +751,18 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
             sj[w][l] = {pj.x, pj.y, pj.z};
             wave_lds_sync();
             int nb = 0, no = 0;
-            // chunks of 256 candidates: lane l loads 4l .. 4l+3 (q0 + 255 <
-            // kTileCap whenever q0 < n, so every load stays in the slot) and
-            // step k hands lane l candidate q0 + 64k + l through a lane
-            // permute, so entries are taken in candidate order: within 64
-            // consecutive list entries (one rotation step of the
-            // enumeration) no i or j atom repeats, which keeps the later
-            // passes' LDS sums free of same-address conflicts
-            for (int q0 = 0; q0 < n; q0 += 4 * kBlk) {
+            // chunks of 256 candidates: lane l loads 4l .. 4l+3 (loads that
+            // would pass the end of the slot are not issued; their lanes only
+            // hand out candidates at or beyond c1) and step k hands lane l
+            // candidate q0 + 64k + l through a lane permute, so entries are
+            // taken in candidate order: within 64 consecutive list entries
+            // (one rotation step of the enumeration) no i or j atom repeats,
+            // which keeps the later passes' LDS sums free of same-address
+            // conflicts
+            for (int q0 = c0; q0 < c1; q0 += 4 * kBlk) {
                 const unsigned long long v = v_nx;
-                if (q0 + 4 * kBlk < n) v_nx = C4[(q0 >> 2) + kBlk + l];
+                if (q0 + 4 * kBlk < c1 && ((q0 + 4 * kBlk) >> 2) + l < kTileCap / 4)
+                    v_nx = C4[((q0 + 4 * kBlk) >> 2) + l];
                 const unsigned int vlo = (unsigned int)v, vhi = (unsigned int)(v >> 32);
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
@@ -679,7 +771,7 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
                     const unsigned int word = (l & 2) ? hi : lo;
                     const int e = (int)((word >> (16 * (l & 1))) & 0xFFFFu);
                     const int q = q0 + kBlk * k + l;
-                    const bool in = q < n;
+                    const bool in = q < c1;
                     const PosR<R> A = si[w][e & 63], B = sj[w][(e >> 6) & 63];
                     const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
                     const R r2 = dx * dx + dy * dy + dz * dz;
@@ -687,20 +779,20 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
                     const bool oth = in && !born && r2 <= cut2;
                     const unsigned long long mb = __ballot(born), mo = __ballot(oth);
                     if (born) L[nb + (int)__popcll(mb & below)] = (uint16_t)e;
-                    if (oth) L[kTileCap - 1 - (no + (int)__popcll(mo & below))] = (uint16_t)e;
+                    if (oth) L[capm1 - (no + (int)__popcll(mo & below))] = (uint16_t)e;
                     nb += (int)__popcll(mb);
                     no += (int)__popcll(mo);
                 }
             }
-            if (l == 0) S.tile_cnt[t] = make_int2(nb, no);
+            if (l == 0) S.tile_cnt[it.rec] = make_int2(nb, no);
             wave_lds_sync();
         }
     }
 }
 
-// Position of the q-th entry of a tile's list: born-side entries from the
-// front, the others from the back.
-__device__ __forceinline__ int entry_pos(int q, int nb) { return q < nb ? q : kTileCap - 1 - (q - nb); }
+// Position of the q-th entry of a list region of capm1 + 1 entries: born-side
+// entries from the front, the others from the back.
+__device__ __forceinline__ int entry_pos(int q, int nb, int capm1) { return q < nb ? q : capm1 - (q - nb); }
 
 // Per-lane energy sums over one tile, in the pass's precision (a lane adds
 // at most 64 terms before the tile's sums are reduced in FP64).
@@ -711,23 +803,27 @@ template <typename R> struct LaneEnergy {
 
 // Per-wavefront running sums of the instance-level results of the pair
 // passes (energies, pair counts) over the tiles the wavefront processes,
-// written with one atomic per field when the wavefront moves on to another
-// instance and at its end. A single large instance would otherwise take one
-// atomic per tile on the same few addresses. In fixed point every tile's
-// energy is rounded on its own before it is added, exactly as a per-tile
-// atomic would round it, so totals do not depend on how tiles are spread
-// over wavefronts.
+// kept per I block: they are added to the block's record (blk_acc) with one
+// atomic per field when the wavefront moves on to a tile of another row and
+// at its end, and the records are folded into the instance's by the
+// evaluation's last kernel. One atomic per field per wavefront on the
+// instance record itself serialises a single large instance (thousands of
+// wavefronts on the same few addresses). In fixed point every tile's energy
+// is rounded on its own before it is added, exactly as a per-tile atomic
+// would round it, so totals do not depend on how tiles are spread over
+// wavefronts.
 template <typename R> struct InstSink {
     using Acc = typename std::conditional<Prec<R>::fixed, long long, double>::type;
-    int inst = -1;
+    int blk = -1, inst = -1;
     Acc e[4] = {0, 0, 0, 0}; // vdw, elec, pol cross, virial
     unsigned long long n[kCNum] = {0, 0};
     int bad = 0;
 
-    __device__ __forceinline__ void to(const DevState &S, int next) {
-        if (next != inst) {
+    __device__ __forceinline__ void to(const DevState &S, int next_blk, int next_inst) {
+        if (next_blk != blk) {
             flush(S);
-            inst = next;
+            blk = next_blk;
+            inst = next_inst;
         }
     }
     __device__ __forceinline__ void energy(int k, double v) {
@@ -745,8 +841,8 @@ template <typename R> struct InstSink {
         }
     }
     // wave-uniform: call from every lane of the wavefront
-    __device__ __forceinline__ void take(const DevState &S, int next, const LaneEnergy<R> &le) {
-        to(S, next);
+    __device__ __forceinline__ void take(const DevState &S, int next_blk, int next_inst, const LaneEnergy<R> &le) {
+        to(S, next_blk, next_inst);
         energy(0, wave_sum((double)le.vdw));
         energy(1, wave_sum((double)le.elec));
         energy(2, wave_sum((double)le.pol));
@@ -754,18 +850,18 @@ template <typename R> struct InstSink {
         n[kCPairs] += wave_sum(le.pairs);
     }
     __device__ __forceinline__ void flush(const DevState &S) {
-        if (inst >= 0 && lane_id() == 0) {
+        if (blk >= 0 && lane_id() == 0) {
             const int field[4] = {kEVdw, kEElec, kEPolCross, kEVirial};
+            unsigned long long *rec = S.blk_acc + (size_t)blk * kBAcc;
             for (int k = 0; k < 4; ++k) {
                 if (e[k] == 0) continue;
-                const size_t i = (size_t)inst * kENum + field[k];
                 if constexpr (Prec<R>::fixed)
-                    atomicAdd(reinterpret_cast<unsigned long long *>(S.inst_energy) + i, (unsigned long long)e[k]);
+                    atomicAdd(rec + field[k], (unsigned long long)e[k]);
                 else
-                    unsafeAtomicAdd(reinterpret_cast<double *>(S.inst_energy) + i, e[k]);
+                    unsafeAtomicAdd(reinterpret_cast<double *>(rec) + field[k], e[k]);
             }
             for (int k = 0; k < kCNum; ++k)
-                if (n[k]) atomicAdd(S.inst_count + (size_t)inst * kCNum + k, n[k]);
+                if (n[k]) atomicAdd(rec + kENum + k, n[k]);
             if (bad) atomicOr(S.inst_status + inst, bad);
         }
         for (int k = 0; k < 4; ++k) e[k] = 0;
@@ -806,7 +902,6 @@ __device__ __forceinline__ bool diag_take(const DiagBits &D, int jlane, int k, i
 __device__ __forceinline__ int diag_bo(const DiagBits &D, int jlane) {
     return ((D.mx >> jlane) & 1ull) ? 2 : (((D.m4 >> jlane) & 1ull) ? 3 : 7);
 }
-template <typename T> __device__ __forceinline__ T diag_home(T v) { return __shfl(v, (lane_id() + 32) & 63, 64); }
 
 // Still's pairwise Born term P4*CCF/r^4 (forcefields/gbff.cpp:448-460)
 template <typename R>
@@ -827,20 +922,24 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
     __shared__ BornAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
     __shared__ double gi_acc[kWavesPerGroup][kBlk], gj_acc[kWavesPerGroup][kBlk];
     const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
+    const int lg = P.lg_split;
+    const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
-    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
     const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5, still2 = (R)P.still2;
     InstSink<R> sink;
-    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
-        const int cend = min(t_end, c + tpw);
-        for (int t = c; t < cend; ++t) {
+    for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
+        const int cend = min(u_end, c + tpw);
+        for (int u = c; u < cend; ++u) {
+            const TileItem it = tile_item(u, t_begin, nt, lg);
+            const int t = it.t;
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
             const bool diag = (tw.y >> 24) & kTileDiag;
-            const int2 cnt = S.tile_cnt[t];
+            const int2 cnt = S.tile_cnt[it.rec];
             const int tslot = S.tile_slot[t];
             if (diag || cnt.x < 0) {
                 const DiagBits D = diag_bits(S, tslot);
@@ -860,7 +959,10 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
                 int np = 0;
                 const int ldir = (rot_i(l) - l) & 63;
                 const int steps = diag ? 32 : kBlk;
-                for (int k = 1; k <= steps; ++k) {
+                const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
+                xj = rot_to(xj, k0, ldir); yj = rot_to(yj, k0, ldir); zj = rot_to(zj, k0, ldir);
+                rj = rot_to(rj, k0, ldir); vjj = rot_to(vjj, k0, ldir);
+                for (int k = k0 + 1; k <= k1; ++k) {
                     xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj); vjj = rot(vjj); gj = rot(gj);
                     const int jlane = (l + k * ldir) & 63;
                     const R dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
@@ -873,22 +975,23 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
                     }
                 }
                 int *st = S.inst_status + inst;
+                gj = rot_home(gj, k1, ldir);
                 if (diag) {
-                    const double tot = gi + (double)diag_home(gj);
+                    const double tot = gi + (double)gj;
                     if (l < S.blk_n[I]) acc_add<R>(S.acc_g, a, tot, kGpolScale, st);
-                } else { // 64 rotations bring the j-side sums home
+                } else {
                     if (l < S.blk_n[I]) acc_add<R>(S.acc_g, a, gi, kGpolScale, st);
                     if (l < S.blk_n[J]) acc_add<R>(S.acc_g, b, (double)gj, kGpolScale, st);
                 }
                 const unsigned int npw = wave_sum((unsigned int)np);
-                sink.to(S, inst);
+                sink.to(S, I, inst);
                 sink.n[kCBornPairs] += 2ull * npw;
                 continue;
             }
             const int nb = cnt.x;
             if (nb == 0) continue;
             // list entries are fetched one chunk ahead of their use
-            const uint16_t *L = S.plist + (size_t)t * kTileCap;
+            const uint16_t *L = S.plist + (size_t)t * kTileCap + (size_t)it.s * (kTileCap >> lg);
             int e_nx = l < nb ? (int)L[l] : 0;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
@@ -923,7 +1026,7 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
             int *st = S.inst_status + inst;
             if (l < S.blk_n[I]) acc_add<R>(S.acc_g, a, gi_acc[w][l], kGpolScale, st);
             if (l < S.blk_n[J]) acc_add<R>(S.acc_g, b, gj_acc[w][l], kGpolScale, st);
-            sink.to(S, inst);
+            sink.to(S, I, inst);
             sink.n[kCBornPairs] += (unsigned long long)(2 * nb);
             wave_lds_sync();
         }
@@ -963,7 +1066,7 @@ __global__ void k_born_final(DevState S, PairParams P, int atom_begin, int atom_
     }
     const double tot = wave_sum(eself);
     if (lane_id() == 0 && inst >= 0 && tot != 0.0)
-        acc_add<R>(S.inst_energy, (size_t)inst * kENum + kEPolSelf, tot, kEnergyScale, S.inst_status + inst);
+        acc_add<R>(S.blk_acc, (size_t)b * kBAcc + kEPolSelf, tot, kEnergyScale, S.inst_status + inst);
 }
 
 // Switching function S and dS/dr of PD (forcefields/gbff.cpp:1605-1614).
@@ -1127,20 +1230,24 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
     __shared__ PairAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
     __shared__ double fi[kWavesPerGroup][4][kBlk], fj[kWavesPerGroup][4][kBlk];
     const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
+    const int lg = P.lg_split;
+    const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
-    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
     const R cut2 = (R)P.cut2;
     InstSink<R> sink;
-    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
-        const int cend = min(t_end, c + tpw);
-        for (int t = c; t < cend; ++t) {
+    for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
+        const int cend = min(u_end, c + tpw);
+        for (int u = c; u < cend; ++u) {
+            const TileItem it = tile_item(u, t_begin, nt, lg);
+            const int t = it.t;
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
             const bool diag = (tw.y >> 24) & kTileDiag;
-            const int2 cnt = S.tile_cnt[t];
+            const int2 cnt = S.tile_cnt[it.rec];
             const int tslot = S.tile_slot[t];
             if (diag || cnt.x < 0) {
                 const DiagBits D = diag_bits(S, tslot);
@@ -1162,7 +1269,11 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                 LaneEnergy<R> E;
                 const int ldir = (rot_i(l) - l) & 63;
                 const int steps = diag ? 32 : kBlk;
-                for (int k = 1; k <= steps; ++k) {
+                const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
+                B.x = rot_to(B.x, k0, ldir); B.y = rot_to(B.y, k0, ldir); B.z = rot_to(B.z, k0, ldir);
+                B.q = rot_to(B.q, k0, ldir); B.rad = rot_to(B.rad, k0, ldir); B.eps = rot_to(B.eps, k0, ldir);
+                B.a = rot_to(B.a, k0, ldir);
+                for (int k = k0 + 1; k <= k1; ++k) {
                     B.x = rot(B.x); B.y = rot(B.y); B.z = rot(B.z); B.q = rot(B.q);
                     B.rad = rot(B.rad); B.eps = rot(B.eps); B.a = rot(B.a);
                     fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
@@ -1180,17 +1291,18 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                     }
                 }
                 int *st = S.inst_status + inst;
+                fjx = rot_home(fjx, k1, ldir); fjy = rot_home(fjy, k1, ldir); fjz = rot_home(fjz, k1, ldir);
+                if (MODE == 0) daj_acc = rot_home(daj_acc, k1, ldir);
                 if (diag) {
-                    const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
-                                 gz = fiz + (double)diag_home(fjz);
-                    const double gd = MODE == 0 ? dai_acc + (double)diag_home(daj_acc) : 0.0;
+                    const double gx = fix + (double)fjx, gy = fiy + (double)fjy, gz = fiz + (double)fjz;
+                    const double gd = MODE == 0 ? dai_acc + (double)daj_acc : 0.0;
                     if (l < S.blk_n[I]) {
                         acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
                         acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
                         acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
                         if (MODE == 0) acc_add<R>(S.acc_deda, a, gd, kForceScale, st);
                     }
-                } else { // 64 rotations bring the j-side sums home
+                } else {
                     if (l < S.blk_n[I]) {
                         acc_add<R>(S.acc_fx, a, fix, kForceScale, st);
                         acc_add<R>(S.acc_fy, a, fiy, kForceScale, st);
@@ -1204,14 +1316,15 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                         if (MODE == 0) acc_add<R>(S.acc_deda, b, (double)daj_acc, kForceScale, st);
                     }
                 }
-                sink.take(S, inst, E);
+                sink.take(S, I, inst, E);
                 continue;
             }
             const int ntot = cnt.x + cnt.y;
             if (ntot == 0) continue;
             // list entries are fetched one chunk ahead of their use
-            const uint16_t *L = S.plist + (size_t)t * kTileCap;
-            int e_nx = l < ntot ? (int)L[entry_pos(l, cnt.x)] : 0;
+            const int capm1 = (kTileCap >> lg) - 1;
+            const uint16_t *L = S.plist + (size_t)t * kTileCap + (size_t)it.s * (kTileCap >> lg);
+            int e_nx = l < ntot ? (int)L[entry_pos(l, cnt.x, capm1)] : 0;
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
@@ -1231,7 +1344,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
             for (int q0 = 0; q0 < ntot; q0 += kBlk) {
                 const int q = q0 + l;
                 const int e = e_nx;
-                e_nx = q + kBlk < ntot ? (int)L[entry_pos(q + kBlk, cnt.x)] : 0;
+                e_nx = q + kBlk < ntot ? (int)L[entry_pos(q + kBlk, cnt.x, capm1)] : 0;
                 if (q < ntot) {
                     const int ii = e & 63, jj = (e >> 6) & 63, bo = (e >> 12) & 7;
                     const PairAtom<R> A = si[w][ii], B = sj[w][jj];
@@ -1268,7 +1381,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                 acc_add<R>(S.acc_fz, b, fj[w][2][l], kForceScale, st);
                 if (MODE == 0) acc_add<R>(S.acc_deda, b, fj[w][3][l], kForceScale, st);
             }
-            sink.take(S, inst, E);
+            sink.take(S, I, inst, E);
             wave_lds_sync();
         }
     }
@@ -1308,19 +1421,23 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
     __shared__ ChainAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
     __shared__ double fi[kWavesPerGroup][3][kBlk], fj[kWavesPerGroup][3][kBlk];
     const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
+    const int lg = P.lg_split;
+    const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
-    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
     const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5, still2 = (R)P.still2;
-    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
-        const int cend = min(t_end, c + tpw);
-        for (int t = c; t < cend; ++t) {
+    for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
+        const int cend = min(u_end, c + tpw);
+        for (int u = c; u < cend; ++u) {
+            const TileItem it = tile_item(u, t_begin, nt, lg);
+            const int t = it.t;
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
             const bool diag = (tw.y >> 24) & kTileDiag;
-            const int2 cnt = S.tile_cnt[t];
+            const int2 cnt = S.tile_cnt[it.rec];
             const int tslot = S.tile_slot[t];
             if (diag || cnt.x < 0) {
                 const DiagBits D = diag_bits(S, tslot);
@@ -1342,7 +1459,10 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
                 double fix = 0, fiy = 0, fiz = 0;
                 const int ldir = (rot_i(l) - l) & 63;
                 const int steps = diag ? 32 : kBlk;
-                for (int k = 1; k <= steps; ++k) {
+                const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
+                B.x = rot_to(B.x, k0, ldir); B.y = rot_to(B.y, k0, ldir); B.z = rot_to(B.z, k0, ldir);
+                B.rad = rot_to(B.rad, k0, ldir); B.vj = rot_to(B.vj, k0, ldir); B.wt = rot_to(B.wt, k0, ldir);
+                for (int k = k0 + 1; k <= k1; ++k) {
                     B.x = rot(B.x); B.y = rot(B.y); B.z = rot(B.z);
                     B.rad = rot(B.rad); B.vj = rot(B.vj); B.wt = rot(B.wt);
                     fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
@@ -1358,15 +1478,15 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
                     }
                 }
                 int *st = S.inst_status + inst;
+                fjx = rot_home(fjx, k1, ldir); fjy = rot_home(fjy, k1, ldir); fjz = rot_home(fjz, k1, ldir);
                 if (diag) {
-                    const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
-                                 gz = fiz + (double)diag_home(fjz);
+                    const double gx = fix + (double)fjx, gy = fiy + (double)fjy, gz = fiz + (double)fjz;
                     if (vi) {
                         acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
                         acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
                         acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
                     }
-                } else { // 64 rotations bring the j-side sums home
+                } else {
                     if (vi) {
                         acc_add<R>(S.acc_fx, a, fix, kForceScale, st);
                         acc_add<R>(S.acc_fy, a, fiy, kForceScale, st);
@@ -1383,7 +1503,7 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
             const int nb = cnt.x;
             if (nb == 0) continue;
             // list entries are fetched one chunk ahead of their use
-            const uint16_t *L = S.plist + (size_t)t * kTileCap;
+            const uint16_t *L = S.plist + (size_t)t * kTileCap + (size_t)it.s * (kTileCap >> lg);
             int e_nx = l < nb ? (int)L[l] : 0;
             const bool vi = l < S.blk_n[I], vj = l < S.blk_n[J];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
@@ -1473,19 +1593,23 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
     __shared__ SoftAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
     __shared__ double fi[kWavesPerGroup][3][kBlk], fj[kWavesPerGroup][3][kBlk];
     const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
+    const int lg = P.lg_split;
+    const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
-    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
     InstSink<R> sink;
-    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
-        const int cend = min(t_end, c + tpw);
-        for (int t = c; t < cend; ++t) {
+    for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
+        const int cend = min(u_end, c + tpw);
+        for (int u = c; u < cend; ++u) {
+            const TileItem it = tile_item(u, t_begin, nt, lg);
+            const int t = it.t;
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
             const bool diag = (tw.y >> 24) & kTileDiag;
-            const int2 cnt = S.tile_cnt[t];
+            const int2 cnt = S.tile_cnt[it.rec];
             const int tslot = S.tile_slot[t];
             const size_t a = (size_t)I * kBlk + l, b = (size_t)J * kBlk + l;
             int *st = S.inst_status + inst;
@@ -1501,7 +1625,10 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
                 const unsigned long long skip = D.mx | D.m4;
                 const int ldir = (rot_i(l) - l) & 63;
                 const int steps = diag ? 32 : kBlk;
-                for (int k = 1; k <= steps; ++k) {
+                const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
+                xj = rot_to(xj, k0, ldir); yj = rot_to(yj, k0, ldir); zj = rot_to(zj, k0, ldir);
+                rj = rot_to(rj, k0, ldir);
+                for (int k = k0 + 1; k <= k1; ++k) {
                     xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj);
                     fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
                     const int jlane = (l + k * ldir) & 63;
@@ -1516,15 +1643,15 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
                         E.vdw += e;
                     }
                 }
+                fjx = rot_home(fjx, k1, ldir); fjy = rot_home(fjy, k1, ldir); fjz = rot_home(fjz, k1, ldir);
                 if (diag) {
-                    const double gx = fix + (double)diag_home(fjx), gy = fiy + (double)diag_home(fjy),
-                                 gz = fiz + (double)diag_home(fjz);
+                    const double gx = fix + (double)fjx, gy = fiy + (double)fjy, gz = fiz + (double)fjz;
                     if (l < S.blk_n[I]) {
                         acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
                         acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
                         acc_add<R>(S.acc_fz, a, gz, kForceScale, st);
                     }
-                } else { // 64 rotations bring the j-side sums home
+                } else {
                     if (l < S.blk_n[I]) {
                         acc_add<R>(S.acc_fx, a, fix, kForceScale, st);
                         acc_add<R>(S.acc_fy, a, fiy, kForceScale, st);
@@ -1536,13 +1663,14 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
                         acc_add<R>(S.acc_fz, b, (double)fjz, kForceScale, st);
                     }
                 }
-                sink.take(S, inst, E);
+                sink.take(S, I, inst, E);
                 continue;
             }
             const int ntot = cnt.x + cnt.y;
             if (ntot == 0) continue;
-            const uint16_t *L = S.plist + (size_t)t * kTileCap;
-            int e_nx = l < ntot ? (int)L[entry_pos(l, cnt.x)] : 0;
+            const int capm1 = (kTileCap >> lg) - 1;
+            const uint16_t *L = S.plist + (size_t)t * kTileCap + (size_t)it.s * (kTileCap >> lg);
+            int e_nx = l < ntot ? (int)L[entry_pos(l, cnt.x, capm1)] : 0;
             const AtomXYZ<R> pi = load_xyzq<R>(S, a, make_double4(0, 0, 0, 0));
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
             si[w][l] = {pi.x, pi.y, pi.z, ld_radius<R>(S, a)};
@@ -1552,7 +1680,7 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
             for (int q0 = 0; q0 < ntot; q0 += kBlk) {
                 const int q = q0 + l;
                 const int e = e_nx;
-                e_nx = q + kBlk < ntot ? (int)L[entry_pos(q + kBlk, cnt.x)] : 0;
+                e_nx = q + kBlk < ntot ? (int)L[entry_pos(q + kBlk, cnt.x, capm1)] : 0;
                 if (q < ntot && ((e >> 12) & 7) > 3) {
                     const int ii = e & 63, jj = (e >> 6) & 63;
                     const SoftAtom<R> A = si[w][ii], B = sj[w][jj];
@@ -1583,7 +1711,7 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
                 acc_add<R>(S.acc_fy, b, fj[w][1][l], kForceScale, st);
                 acc_add<R>(S.acc_fz, b, fj[w][2][l], kForceScale, st);
             }
-            sink.take(S, inst, E);
+            sink.take(S, I, inst, E);
             wave_lds_sync();
         }
     }
@@ -1599,6 +1727,7 @@ __global__ void k_finalize(DevState S, int atom_begin, int atom_end, double *f2,
     const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
     const int b = a / kBlk;
     const int inst = a < atom_end ? S.blk_inst[b] : -1;
+    blk_fold<R>(S, b, inst);
     double sq = 0;
     if (inst >= 0 && (a % kBlk) < S.blk_n[b]) {
         const double fx = -acc_read<R>(S.acc_fx, a, kForceScale);

@@ -229,6 +229,7 @@ __global__ void k_finalize_kick(DevState S, InstMD *md, const double *minv, doub
     const int b = a / kBlk;
     const bool in = a < atom_end;
     const int inst = in ? S.blk_inst[b] : -1;
+    blk_fold<R>(S, b, inst);
     const bool v = in && inst >= 0 && (a % kBlk) < S.blk_n[b];
     double ek = 0;
     if (v) {

@@ -173,21 +173,25 @@ template <bool SAME>
 __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
     __shared__ MixLdsWave lds[kWavesPerGroup];
     const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
+    const int lg = P.lg_split;
+    const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
-    if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
+    if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
     const int l = lane_id(), w = threadIdx.x / 64;
     MixLdsWave &M = lds[w];
     const MixK K = mix_consts(P);
     InstSink<float> sink;
-    for (int c = t_begin + wave_global_id() * tpw; c < t_end; c += nw * tpw) {
-        const int cend = min(t_end, c + tpw);
-        for (int t = c; t < cend; ++t) {
+    for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
+        const int cend = min(u_end, c + tpw);
+        for (int u = c; u < cend; ++u) {
+            const TileItem it = tile_item(u, t_begin, nt, lg);
+            const int t = it.t;
             const int2 tw = S.tiles[t];
             const int I = tw.x, J = tw.y & 0xFFFFFF;
             const int inst = S.blk_inst[I];
             if (inst < 0) continue;
             const bool diag = (tw.y >> 24) & kTileDiag;
-            const int2 cnt = S.tile_cnt[t];
+            const int2 cnt = S.tile_cnt[it.rec];
             const bool dense = diag || cnt.x < 0;
             const int ntot = dense ? 0 : cnt.x + cnt.y;
             if (!dense && ntot == 0) continue;
@@ -213,14 +217,17 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
                 wave_lds_sync();
                 float fix = 0, fiy = 0, fiz = 0, dai_acc = 0;
                 float fjx = 0, fjy = 0, fjz = 0, daj_acc = 0;
-                // j lane (l + k*ldir) & 63 of step k, read from the doubled copy
-                int jx = ldir == 1 ? l : l + kBlk;
+                // this segment's steps k0+1 .. k1; the j atom of step k is lane
+                // (l + k*ldir) & 63, read from the doubled copy
+                const int steps = diag ? 32 : kBlk;
+                const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
                 const int dj = ldir == 1 ? 1 : -1;
-                const int halves = diag ? 1 : 2;
-                for (int h = 0; h < halves; ++h) {
+                int jx = (ldir == 1 ? l : l + kBlk) + k0 * dj;
+                for (int h = k0 >> 5; h < (k1 + 31) >> 5; ++h) {
                     const unsigned int Tw = (unsigned int)(T >> (32 * h)), Xw = (unsigned int)(X >> (32 * h)),
                                        Fw = (unsigned int)(F >> (32 * h));
-                    for (int kk = 0; kk < 32; ++kk) {
+                    const int kk1 = min(32, k1 - 32 * h);
+                    for (int kk = max(0, k0 - 32 * h); kk < kk1; ++kk) {
                         jx += dj;
                         const float4 qj = M.v[jx], pj = M.v[128 + jx];
                         const float dx = qa.x - qj.x, dy = qa.y - qj.y, dz = qa.z - qj.z;
@@ -236,18 +243,21 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
                         daj_acc = rotd(daj_acc) + daj;
                     }
                 }
-                if (diag) { // after 32 steps the j-side sum of atom a sits in lane (a + 32) & 63
-                    const double gx = (double)fix + (double)diag_home(fjx);
-                    const double gy = (double)fiy + (double)diag_home(fjy);
-                    const double gz = (double)fiz + (double)diag_home(fjz);
-                    const double gd = (double)dai_acc + (double)diag_home(daj_acc);
+                // after step k1 the j-side sum of atom a sits in lane (a - k1*ldir) & 63
+                fjx = rot_home(fjx, k1, ldir); fjy = rot_home(fjy, k1, ldir); fjz = rot_home(fjz, k1, ldir);
+                daj_acc = rot_home(daj_acc, k1, ldir);
+                if (diag) {
+                    const double gx = (double)fix + (double)fjx;
+                    const double gy = (double)fiy + (double)fjy;
+                    const double gz = (double)fiz + (double)fjz;
+                    const double gd = (double)dai_acc + (double)daj_acc;
                     if (l < S.blk_n[I]) {
                         acc_add<float>(S.acc_fx, a, gx, kForceScale, st);
                         acc_add<float>(S.acc_fy, a, gy, kForceScale, st);
                         acc_add<float>(S.acc_fz, a, gz, kForceScale, st);
                         acc_add<float>(S.acc_deda, a, gd, kForceScale, st);
                     }
-                } else { // 64 rotations bring the j-side sums home
+                } else {
                     if (l < S.blk_n[I]) {
                         acc_add<float>(S.acc_fx, a, fix, kForceScale, st);
                         acc_add<float>(S.acc_fy, a, fiy, kForceScale, st);
@@ -262,10 +272,11 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
                     }
                 }
             } else {
-                const uint16_t *L = S.plist + (size_t)t * kTileCap;
+                const int capm1 = (kTileCap >> lg) - 1;
+                const uint16_t *L = S.plist + (size_t)t * kTileCap + (size_t)it.s * (kTileCap >> lg);
                 // list entries are fetched two chunks ahead of their use
-                int e_n1 = l < ntot ? (int)L[entry_pos(l, cnt.x)] : 0;
-                int e_n2 = l + kBlk < ntot ? (int)L[entry_pos(l + kBlk, cnt.x)] : 0;
+                int e_n1 = l < ntot ? (int)L[entry_pos(l, cnt.x, capm1)] : 0;
+                int e_n2 = l + kBlk < ntot ? (int)L[entry_pos(l + kBlk, cnt.x, capm1)] : 0;
                 M.v[l] = qa; M.v[kBlk + l] = qb;
                 M.v[128 + l] = pa; M.v[192 + l] = pb;
                 for (int k = 0; k < 8; ++k) M.f[k][l] = 0;
@@ -274,7 +285,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
                     const int q = q0 + l;
                     const int e = e_n1;
                     e_n1 = e_n2;
-                    e_n2 = q + 2 * kBlk < ntot ? (int)L[entry_pos(q + 2 * kBlk, cnt.x)] : 0;
+                    e_n2 = q + 2 * kBlk < ntot ? (int)L[entry_pos(q + 2 * kBlk, cnt.x, capm1)] : 0;
                     const int ii = e & 63, jj = (e >> 6) & 63, bo = (e >> 12) & 7;
                     const float4 qi = M.v[ii], qj = M.v[kBlk + jj];
                     const float4 pi = M.v[128 + ii], pj = M.v[192 + jj];
@@ -311,7 +322,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
                 }
             }
             // energies: one fixed-point rounding per tile and field
-            sink.to(S, inst);
+            sink.to(S, I, inst);
             sink.energy(0, wave_sum((double)E.vdw));
             sink.energy(1, wave_sum((double)E.elec));
             sink.energy(2, 2.0 * wave_sum((double)E.pol));

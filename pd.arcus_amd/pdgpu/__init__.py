@@ -156,7 +156,7 @@ class Options(C.Structure):
                [("device_memory_budget", C.c_int64), ("max_instances", C.c_int32), ("components", C.c_int32),
                 ("use_graphs", C.c_int32), ("dense_tile_pairs", C.c_int32),
                 ("langevin_on_hydrogens", C.c_int32), ("softvdw_hardness", C.c_double), ("softvdw_olap_offset", C.c_double),
-                ("softvdw_cutoff", C.c_double)]
+                ("softvdw_cutoff", C.c_double), ("tile_split", C.c_int32)]
 
 
 class Bonded(C.Structure):

@@ -61,6 +61,7 @@ class ForceField:
         h = C.c_void_p()
         _check(lib.pdgpu_ff_read(str(path).encode(), C.byref(h)), f"ff_read {path}")
         self.h = h
+        self._free = lib.pdgpu_ff_free  # kept: module globals may be gone when __del__ runs at exit
         v, e = C.c_double(), C.c_double()
         p = np.zeros(5)
         lib.pdgpu_ff_info(self.h, C.byref(v), C.byref(e), _p(p, _dp))
@@ -68,7 +69,7 @@ class ForceField:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib.pdgpu_ff_free(self.h)
+            self._free(self.h)
             self.h = None
 
     def options(self, **kw):

@@ -89,7 +89,8 @@ def test_fixed_terms_isolated_atom_closed_form():
 
 
 def test_ctx_create_rejects_bad_options():
-    for kw in ({"fast_mode": 0}, {"cutoff": -1.0}, {"inner_cutoff": 20.0}, {"precision": 7}):
+    for kw in ({"fast_mode": 0}, {"cutoff": -1.0}, {"inner_cutoff": 20.0}, {"precision": 7}, {"tile_split": 3},
+               {"tile_split": 32}, {"tile_split": -1}):
         o = pdgpu.default_options(**kw)
         h = C.c_void_p()
         rc = pdgpu.lib.pdgpu_ctx_create(C.byref(o), C.byref(h))
