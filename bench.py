@@ -153,12 +153,12 @@ def cpu_baseline_batch(mols, single_s=10.0, multi_s=6.0):
                       f"{dn} conformers on {cores} processes in {wall:.1f} s wall"}
 
 
-KERNEL_NAMES = {"pair": "pdgpu::k_pair<{R}, 0>", "pair_mixed": "pdgpu::k_pair_mixed<{S}>",
+KERNEL_NAMES = {"pair": "pdgpu::k_pair<{R}, 0>", "pair_mixed": "pdgpu::k_pair_mixed<{S}, {A}>",
                 "born": "pdgpu::k_born<{R}>", "chain": "pdgpu::k_chain<{R}>",
                 "enum": "pdgpu::k_filter<{R}>"}
 
 
-def roofline(prof, pairs, born_pairs, precision, traffic=None, same_switch=True):
+def roofline(prof, pairs, born_pairs, precision, traffic=None, same_switch=True, all_in=True):
     """Roofline of the dominant kernel. Algorithmic work per launch: 86 FLOP
     per unordered pair within the cutoff (pair pass), 20 and 30 FLOP per
     ordered Still pair (Born sum, chain rule). The enumeration kernel has no
@@ -174,7 +174,7 @@ def roofline(prof, pairs, born_pairs, precision, traffic=None, same_switch=True)
     peak = PEAK_FP32_TFLOPS if precision == "mixed" else PEAK_FP64_TFLOPS
     key = "pair_mixed" if (name == "pair" and precision == "mixed") else name
     kname = KERNEL_NAMES[key].format(R="float" if precision == "mixed" else "double",
-                                     S="true" if same_switch else "false")
+                                     S="true" if same_switch else "false", A="true" if all_in else "false")
     total_ms = sum(v[0] for k, v in prof.items() if k in ("atom", "born", "pair", "chain", "enum", "bonded"))
     evals = max(1, prof["pair"][1])
     step_s = total_ms / 1e3 / evals

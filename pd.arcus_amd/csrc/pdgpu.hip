@@ -664,6 +664,16 @@ int ensure_listed(pdgpu_ctx *c, int id) {
     return PDGPU_OK;
 }
 
+// The mixed pair pass variant for the context's switching ranges and
+// cutoffs (all list entries lie within Cutoff when GbsaStillCutoff does not
+// exceed it).
+using PairKernel = void (*)(DevState, PairParams, int, int, int);
+PairKernel pair_mixed_kernel(const PairParams &P) {
+    const bool allin = P.still2 <= P.cut2;
+    if (P.same_switch) return allin ? k_pair_mixed<true, true> : k_pair_mixed<true, false>;
+    return allin ? k_pair_mixed<false, true> : k_pair_mixed<false, false>;
+}
+
 // The GB kernels of one evaluation: enumeration, Born sums, Born radii,
 // pair pass and chain rule (MODE 0) or the energy-path pair pass (MODE 1).
 template <typename R>
@@ -689,7 +699,7 @@ int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, in
         {
             ProfScope ps(c, PDGPU_KERNEL_PAIR);
             if constexpr (std::is_same<R, float>::value)
-                hipLaunchKernelGGL(c->P.same_switch ? k_pair_mixed<true> : k_pair_mixed<false>, dim3(g[kGridPair]),
+                hipLaunchKernelGGL(pair_mixed_kernel(c->P), dim3(g[kGridPair]),
                                    dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1, t[kGridPair]);
             else
                 hipLaunchKernelGGL((k_pair<R, 0>), dim3(g[kGridPair]), dim3(256), 0, c->stream, S, c->P, sp.b0,

@@ -141,6 +141,18 @@ __device__ __forceinline__ int rot_i(int v) {
     return __builtin_amdgcn_update_dpp(v, v, 0x13C, 0xF, 0xF, false);
 }
 __device__ __forceinline__ float rot(float v) { return __int_as_float(rot_i(__float_as_int(v))); }
+// Lane offset of one rotation, 1 or 63: after k rotations lane l holds the
+// value of lane (l + k*rot_dir()) & 63. The same in every lane; read from
+// lane 0 so that the compiler treats it (and what derives from it) as
+// wavefront-uniform.
+__device__ __forceinline__ int rot_dir() {
+    return __builtin_amdgcn_readfirstlane((rot_i(lane_id()) - lane_id()) & 63);
+}
+// Wavefront index within the workgroup, as a wavefront-uniform value (the
+// compiler's divergence analysis does not know that threadIdx.x / 64 is
+// uniform, so tile indices, records and loop bounds derived from it would
+// otherwise live in vector registers and branch through exec masks).
+__device__ __forceinline__ int wave_in_group() { return __builtin_amdgcn_readfirstlane(threadIdx.x / 64); }
 __device__ __forceinline__ double rot(double v) {
     const long long b = __double_as_longlong(v);
     const int lo = rot_i((int)(b & 0xffffffffLL));
@@ -297,7 +309,7 @@ __device__ __forceinline__ int wave_global_id() {
     const int nx = 8;
     int b = blockIdx.x;
     if ((gridDim.x % nx) == 0) b = (blockIdx.x % nx) * (gridDim.x / nx) + blockIdx.x / nx;
-    return b * kWavesPerGroup + threadIdx.x / 64;
+    return b * kWavesPerGroup + wave_in_group();
 }
 
 // ------------------------------------------------------ per-instance setup
@@ -341,7 +353,7 @@ __device__ __forceinline__ bool build_idle(const DevState &S) { return S.any_nee
 
 __global__ void k_block_bounds(DevState S, int blk_begin, int blk_end) {
     if (build_idle(S)) return;
-    const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + wave_in_group();
     if (b >= blk_end) return;
     if (S.blk_inst[b] < 0) return;
     if (S.inst_need && !S.inst_need[S.blk_inst[b]]) return;
@@ -372,7 +384,7 @@ __global__ void k_block_bounds(DevState S, int blk_begin, int blk_end) {
 // than half the list padding since the instance's last rebuild (then a pair
 // from beyond the list radius may have come inside the cutoff).
 __global__ void k_moved(DevState S, double lim2, int blk_begin, int blk_end) {
-    const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int b = blk_begin + blockIdx.x * (blockDim.x / 64) + wave_in_group();
     if (b >= blk_end) return;
     const int inst = S.blk_inst[b];
     if (inst < 0) return;
@@ -417,7 +429,7 @@ __device__ __forceinline__ int tile_flags(const DevState &S, int inst, int I, in
 template <int PASS>
 __global__ void k_tile_rows(DevState S, PairParams P, int blk_begin, int blk_end) {
     if (build_idle(S)) return;
-    const int I = blk_begin + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int I = blk_begin + blockIdx.x * (blockDim.x / 64) + wave_in_group();
     if (I >= blk_end) return;
     const int inst = S.blk_inst[I];
     const int l = lane_id();
@@ -587,7 +599,7 @@ __global__ void __launch_bounds__(256) k_cand(DevState S, PairParams P, int blk_
     const int t_begin = S.row_off[blk_begin], t_end = S.row_off[blk_end];
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (t_end - t_begin + nw - 1) / nw);
-    const int l = lane_id(), w = threadIdx.x / 64;
+    const int l = lane_id(), w = wave_in_group();
     // the list radius, widened by a relative 1e-5 so that FP32 rounding never
     // drops a pair the cutoff tests of the evaluation would keep
     const R reach2 = (R)(P.list_lim2 * (1.0 + 1e-5));
@@ -609,7 +621,7 @@ __global__ void __launch_bounds__(256) k_cand(DevState S, PairParams P, int blk_
             const int nJ = S.blk_n[J];
             // lane offset of one rotation (the same for every lane): after k
             // rotations lane l holds the j-atom of lane (l + k*ldir) & 63
-            const int ldir = (rot_i(l) - l) & 63;
+            const int ldir = rot_dir();
             const bool diag = fl & kTileDiag;
             const bool lookup = fl & (kTileNear | kTileSparse);
             const int mid = S.inst_mid[inst], jl0 = (J - first) * kBlk;
@@ -691,6 +703,13 @@ __global__ void __launch_bounds__(256) k_cand(DevState S, PairParams P, int blk_
 
 template <typename R> struct PosR { R x, y, z; };
 
+// r^2 of a candidate. FP32: the mixed pair pass's own operation order, so
+// that the r^2 it computes for a kept entry is exactly the one tested here.
+template <typename R> __device__ __forceinline__ R filter_r2(R dx, R dy, R dz) {
+    if constexpr (Prec<R>::fixed) return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    else return dx * dx + dy * dy + dz * dz;
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
     __shared__ PosR<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
@@ -699,7 +718,7 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
     const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
-    const int l = lane_id(), w = threadIdx.x / 64;
+    const int l = lane_id(), w = wave_in_group();
     const R cut2 = (R)P.cut2, still2 = (R)P.still2;
     const unsigned long long below = lanes_below();
     for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
@@ -774,7 +793,7 @@ __global__ void __launch_bounds__(256) k_filter(DevState S, PairParams P, int bl
                     const bool in = q < c1;
                     const PosR<R> A = si[w][e & 63], B = sj[w][(e >> 6) & 63];
                     const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
-                    const R r2 = dx * dx + dy * dy + dz * dz;
+                    const R r2 = filter_r2<R>(dx, dy, dz);
                     const bool born = in && ((e >> 12) & 7) > 2 && r2 <= still2;
                     const bool oth = in && !born && r2 <= cut2;
                     const unsigned long long mb = __ballot(born), mo = __ballot(oth);
@@ -926,7 +945,7 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
     const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
-    const int l = lane_id(), w = threadIdx.x / 64;
+    const int l = lane_id(), w = wave_in_group();
     const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5, still2 = (R)P.still2;
     InstSink<R> sink;
     for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
@@ -957,7 +976,7 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
                 R xj = pj.x, yj = pj.y, zj = pj.z, gj = 0;
                 double gi = 0;
                 int np = 0;
-                const int ldir = (rot_i(l) - l) & 63;
+                const int ldir = rot_dir();
                 const int steps = diag ? 32 : kBlk;
                 const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
                 xj = rot_to(xj, k0, ldir); yj = rot_to(yj, k0, ldir); zj = rot_to(zj, k0, ldir);
@@ -1234,7 +1253,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
     const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
-    const int l = lane_id(), w = threadIdx.x / 64;
+    const int l = lane_id(), w = wave_in_group();
     const R cut2 = (R)P.cut2;
     InstSink<R> sink;
     for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
@@ -1267,7 +1286,7 @@ __global__ void __launch_bounds__(256) k_pair(DevState S, PairParams P, int blk_
                 R fjx = 0, fjy = 0, fjz = 0, daj_acc = 0;
                 double fix = 0, fiy = 0, fiz = 0, dai_acc = 0;
                 LaneEnergy<R> E;
-                const int ldir = (rot_i(l) - l) & 63;
+                const int ldir = rot_dir();
                 const int steps = diag ? 32 : kBlk;
                 const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
                 B.x = rot_to(B.x, k0, ldir); B.y = rot_to(B.y, k0, ldir); B.z = rot_to(B.z, k0, ldir);
@@ -1425,7 +1444,7 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
     const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
-    const int l = lane_id(), w = threadIdx.x / 64;
+    const int l = lane_id(), w = wave_in_group();
     const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5, still2 = (R)P.still2;
     for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
         const int cend = min(u_end, c + tpw);
@@ -1457,7 +1476,7 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
                 ChainAtom<R> B = {pj.x, pj.y, pj.z, rj, vjj, vjb ? chain_weight<R>(S, b) : (R)0};
                 R fjx = 0, fjy = 0, fjz = 0;
                 double fix = 0, fiy = 0, fiz = 0;
-                const int ldir = (rot_i(l) - l) & 63;
+                const int ldir = rot_dir();
                 const int steps = diag ? 32 : kBlk;
                 const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
                 B.x = rot_to(B.x, k0, ldir); B.y = rot_to(B.y, k0, ldir); B.z = rot_to(B.z, k0, ldir);
@@ -1597,7 +1616,7 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
     const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
-    const int l = lane_id(), w = threadIdx.x / 64;
+    const int l = lane_id(), w = wave_in_group();
     InstSink<R> sink;
     for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
         const int cend = min(u_end, c + tpw);
@@ -1623,7 +1642,7 @@ __global__ void __launch_bounds__(256) k_soft(DevState S, PairParams P, int blk_
                 R fjx = 0, fjy = 0, fjz = 0;
                 double fix = 0, fiy = 0, fiz = 0;
                 const unsigned long long skip = D.mx | D.m4;
-                const int ldir = (rot_i(l) - l) & 63;
+                const int ldir = rot_dir();
                 const int steps = diag ? 32 : kBlk;
                 const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
                 xj = rot_to(xj, k0, ldir); yj = rot_to(yj, k0, ldir); zj = rot_to(zj, k0, ldir);
@@ -1752,7 +1771,7 @@ template <int PASS>
 __global__ void k_export(DevState S, PairParams P, int blk0, int nblk, int shadow,
                          int *counts, const long long *offs, uint32_t *entries) {
 #pragma clang fp contract(off)
-    const int I = blk0 + blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int I = blk0 + blockIdx.x * (blockDim.x / 64) + wave_in_group();
     if (I >= blk0 + nblk) return;
     const int l = lane_id();
     const int inst = S.blk_inst[I];

@@ -8,9 +8,12 @@
 //    written once per evaluation by k_born_final, so a lane gets an atom
 //    with two 16-byte LDS reads; the exponent of the GB term is one product
 //    r^2 (c/a_i)(c/a_j) with c^2 = log2(e)/4, no reciprocal;
-//  - the pair term has no branches: lanes without a pair, far pairs (no GB
-//    term) and pairs without Coulomb / LJ get zero factors, so a wavefront
-//    never runs a code path twice and no exec-mask bookkeeping is issued;
+//  - the pair term has no branches: far pairs (no GB term) and pairs without
+//    Coulomb / LJ get zero factors, so a wavefront never runs a code path
+//    twice; lanes without a pair are masked off around the whole term;
+//  - the pair's class (bond order <= 2, 1-4, other) selects its Coulomb and
+//    LJ scalings from a 12-entry table held in lanes 0-11 (a lane permute,
+//    no LDS allocation) instead of compares and selects;
 //  - dense tiles read the rotating j atom from an LDS copy of the j block
 //    stored twice in a row, so the address of step k is one add; the j-side
 //    sums still rotate with DPP wave_ror:1, fused into their update
@@ -77,16 +80,16 @@ __device__ __forceinline__ void mix_switch(float r, float inner, float invSw, fl
     dS = (dn * s) * m4invSw;
 }
 
-// One candidate pair, branch-free. take: the pair is evaluated (otherwise
-// every output is zero and r2 may be anything); b3: bond order >= 3
-// (Coulomb and LJ); is14: bond order 3 (scaled). Returns (dE/dr)/r, the
-// factor of (x_i - x_j) in dE/dx_i; adds dE/da_i, dE/da_j to dai, daj.
+// One pair inside the cutoff, branch-free (the caller masks lanes without a
+// pair). sc = (Coulomb, LJ) scaling of the pair's class: (0, 0) for bond
+// order <= 2, the 1-4 scalings, or (1, 1). kin / kout: Coulomb prefactors
+// for near pairs (solute dielectric) and far pairs (solvent), held in
+// vector registers by the caller. Returns (dE/dr)/r, the factor of
+// (x_i - x_j) in dE/dx_i; adds dE/da_i to dai and sets dE/da_j in daj.
 // ai, bj = (R, sqrt(eps), a, c/a) of atoms i and j.
 template <bool SAME>
-__device__ __forceinline__ float mix_pair(const MixK &K, float qi, const float4 &ai, float qj, const float4 &bj,
-                                          float r2in, bool take, bool b3, bool is14, MixE &E, float &dai,
-                                          float &daj) {
-    const float r2 = take ? r2in : 1.0f; // keeps lanes without a pair finite
+__device__ __forceinline__ float mix_pair(const MixK &K, float kin, float kout, float qi, const float4 &ai, float qj,
+                                          const float4 &bj, float r2, float2 sc, MixE &E, float &dai, float &daj) {
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float qq = qi * qj;
@@ -94,29 +97,23 @@ __device__ __forceinline__ float mix_pair(const MixK &K, float qi, const float4 
     const bool near = u2 > K.thr2;          // u above ExpApproxThreshold: GB term
     float Sw, dS;
     mix_switch(r, K.inner, K.invSw, K.inner_s, K.m4invSw, Sw, dS);
-    Sw = take ? Sw : 0.0f;
-    dS = take ? dS : 0.0f;
-    // Coulomb: solute dielectric for near pairs, solvent for far ones; none
-    // for bond order <= 2, scaled for 1-4 pairs
-    float ke = near ? K.ke_in : K.ke_out;
-    ke = is14 ? ke * K.s14e : ke;
-    const float ep = (b3 ? qq * ke : 0.0f) * rinv;
-    // LJ in the R_min form; rr = 0 for pairs without LJ keeps bonded pairs
-    // at tiny separations from forming 0 * inf
-    const float rr = (b3 && r < K.vcut) ? (ai.x + bj.x) * rinv : 0.0f;
+    // Coulomb: solute dielectric for near pairs, solvent for far ones,
+    // scaled by the pair's class (none for bond order <= 2)
+    const float ke = (near ? kin : kout) * sc.x;
+    const float ep = (qq * ke) * rinv;
+    // LJ in the R_min form, scaled by the class; rr is clamped (only ever
+    // reached for atoms within 0.002 A of each other) so that a class
+    // scaling of 0 never meets an infinite power
+    float rr = fminf((ai.x + bj.x) * rinv, 1.0e3f);
+    if constexpr (!SAME) rr = r < K.vcut ? rr : 0.0f;
     const float rr2 = rr * rr;
     const float B6 = rr2 * rr2 * rr2;
     const float A12 = B6 * B6;
-    float eps = ai.y * bj.y;
-    eps = is14 ? eps * K.s14v : eps;
+    const float eps = (ai.y * bj.y) * sc.y;
     const float vp = eps * fmaf(-2.0f, B6, A12);   // 2 eps (A/2 - B)
     const float vfr = (eps * -12.0f) * (A12 - B6); // r dE_vdw/dr
     float vS = Sw, vd = dS;
-    if constexpr (!SAME) {
-        mix_switch(r, K.vinner, K.vinvSw, K.vinner_s, K.vm4invSw, vS, vd);
-        vS = take ? vS : 0.0f;
-        vd = take ? vd : 0.0f;
-    }
+    if constexpr (!SAME) mix_switch(r, K.vinner, K.vinvSw, K.vinner_s, K.vm4invSw, vS, vd);
     // GB: u above the threshold only (gbff.cpp:1575)
     const float aa = ai.z * bj.z;
     const float tg = __builtin_amdgcn_exp2f(u2);
@@ -131,11 +128,29 @@ __device__ __forceinline__ float mix_pair(const MixK &K, float qi, const float4 
     E.vdw = fmaf(vS, vp, E.vdw);
     E.elec = fmaf(Sw, ep, E.elec);
     E.pol = fmaf(Sw, epol, E.pol);
-    E.pairs += take ? 1u : 0u;
     dai += dedal * bj.z;
     daj = dedal * ai.z;
     return fr * rinv;
 }
+
+// Table of pair-class scalings (Coulomb, LJ), entry k held by lane k:
+// entries 0-7 by list bond order (0 for orders <= 2, the 1-4 scalings for 3,
+// 1 above); entries 8-11 by dense-tile bits, 8 + excluded + 2 * (1-4).
+struct ClassTable {
+    float e, v;
+    __device__ __forceinline__ explicit ClassTable(const MixK &K) {
+        const int k = lane_id();
+        const bool one = (k >= 4 && k < 8) || k == 8, s14 = k == 3 || k == 10;
+        e = one ? 1.0f : s14 ? K.s14e : 0.0f;
+        v = one ? 1.0f : s14 ? K.s14v : 0.0f;
+    }
+    // entry k (k < 64, the same or another per lane); a lane permute reads
+    // lanes 0-11, so every lane of the wavefront must be active
+    __device__ __forceinline__ float2 at(int k) const {
+        return make_float2(__int_as_float(__builtin_amdgcn_ds_bpermute(k << 2, __float_as_int(e))),
+                           __int_as_float(__builtin_amdgcn_ds_bpermute(k << 2, __float_as_int(v))));
+    }
+};
 
 // DPP wave_ror:1 (every lane has a source, so bound_ctrl never applies);
 // this form lets the compiler fuse the move into the VALU op that uses it.
@@ -169,7 +184,10 @@ struct MixLdsWave {
     double f[8][kBlk];
 };
 
-template <bool SAME>
+// SAME: the vdW and elec/GB switching ranges coincide. ALLIN: every list
+// entry lies within the cutoff (GbsaStillCutoff <= Cutoff, so the born-side
+// entries do too), and only the tail of the last chunk needs masking.
+template <bool SAME, bool ALLIN>
 __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
     __shared__ MixLdsWave lds[kWavesPerGroup];
     const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
@@ -177,9 +195,14 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
     const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
     const int nw = gridDim.x * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
-    const int l = lane_id(), w = threadIdx.x / 64;
+    const int l = lane_id(), w = wave_in_group();
     MixLdsWave &M = lds[w];
     const MixK K = mix_consts(P);
+    const ClassTable sc_tab(K);
+    // the two Coulomb prefactors in vector registers (as the operands of
+    // one select per pair; from scalar registers each would need a move)
+    float kin = K.ke_in, kout = K.ke_out;
+    asm volatile("" : "+v"(kin), "+v"(kout));
     InstSink<float> sink;
     for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
         const int cend = min(u_end, c + tpw);
@@ -210,7 +233,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
                 const DiagBits D = diag_bits(S, S.tile_slot[t]);
                 M.v[l] = qb; M.v[l + kBlk] = qb;
                 M.v[128 + l] = pb; M.v[128 + kBlk + l] = pb;
-                const int ldir = (rot_i(l) - l) & 63; // 1 or 63, the same in every lane
+                const int ldir = rot_dir(); // 1 or 63, the same in every lane
                 unsigned long long T = step_bits(D.mc, l, ldir);
                 const unsigned long long X = step_bits(D.mx, l, ldir), F = step_bits(D.m4, l, ldir);
                 if (diag && l >= 32) T &= ~(1ull << 31); // step 32 meets each pair twice: the lower lane takes it
@@ -230,17 +253,25 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
                     for (int kk = max(0, k0 - 32 * h); kk < kk1; ++kk) {
                         jx += dj;
                         const float4 qj = M.v[jx], pj = M.v[128 + jx];
+                        // both j records are requested together, ahead of the
+                        // pair mask (left to itself the compiler would sink the
+                        // second read into the masked block, a second LDS round
+                        // trip per step)
+                        asm volatile("" ::"v"(pj.x), "v"(pj.y), "v"(pj.z), "v"(pj.w), "v"(qj.w));
                         const float dx = qa.x - qj.x, dy = qa.y - qj.y, dz = qa.z - qj.z;
                         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        const bool take = ((Tw >> kk) & 1u) && r2 <= K.cut2;
-                        const bool b3 = !((Xw >> kk) & 1u), is14 = (Fw >> kk) & 1u;
-                        float daj;
-                        const float g = mix_pair<SAME>(K, qa.w, pa, qj.w, pj, r2, take, b3, is14 && b3, E, dai_acc,
-                                                       daj);
-                        const float gx = g * dx, gy = g * dy, gz = g * dz;
-                        fix += gx; fiy += gy; fiz += gz;
-                        fjx = rotd(fjx) - gx; fjy = rotd(fjy) - gy; fjz = rotd(fjz) - gz;
-                        daj_acc = rotd(daj_acc) + daj;
+                        // the j-side sums travel with their atoms on every lane
+                        fjx = rotd(fjx); fjy = rotd(fjy); fjz = rotd(fjz); daj_acc = rotd(daj_acc);
+                        const float2 sc = sc_tab.at(8 + (int)((Xw >> kk) & 1u) + 2 * (int)((Fw >> kk) & 1u));
+                        if (((Tw >> kk) & 1u) && r2 <= K.cut2) {
+                            float daj;
+                            const float g = mix_pair<SAME>(K, kin, kout, qa.w, pa, qj.w, pj, r2, sc, E, dai_acc, daj);
+                            const float gx = g * dx, gy = g * dy, gz = g * dz;
+                            fix += gx; fiy += gy; fiz += gz;
+                            fjx -= gx; fjy -= gy; fjz -= gz;
+                            daj_acc += daj;
+                            ++E.pairs;
+                        }
                     }
                 }
                 // after step k1 the j-side sum of atom a sits in lane (a - k1*ldir) & 63
@@ -291,11 +322,12 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
                     const float4 pi = M.v[128 + ii], pj = M.v[192 + jj];
                     const float dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    // born-side entries may reach past Cutoff when GbsaStillCutoff is larger
-                    const bool take = q < ntot && r2 <= K.cut2;
-                    float dai = 0, daj;
-                    const float g = mix_pair<SAME>(K, qi.w, pi, qj.w, pj, r2, take, bo >= 3, bo == 3, E, dai, daj);
-                    if (q < ntot) {
+                    const float2 sc = sc_tab.at(bo);
+                    // born-side entries reach past Cutoff only when GbsaStillCutoff is larger
+                    if (q < ntot && (ALLIN || r2 <= K.cut2)) {
+                        float dai = 0, daj;
+                        const float g = mix_pair<SAME>(K, kin, kout, qi.w, pi, qj.w, pj, r2, sc, E, dai, daj);
+                        if (!ALLIN) ++E.pairs;
                         const float gx = g * dx, gy = g * dy, gz = g * dz;
                         atomicAdd(&M.f[0][ii], (double)gx);
                         atomicAdd(&M.f[1][ii], (double)gy);
@@ -307,6 +339,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(DevState S, PairParams P, in
                         atomicAdd(&M.f[7][jj], (double)daj);
                     }
                 }
+                if (ALLIN && l == 0) E.pairs = (unsigned int)ntot;
                 wave_lds_sync();
                 if (l < S.blk_n[I]) {
                     acc_add<float>(S.acc_fx, a, M.f[0][l], kForceScale, st);
