@@ -914,6 +914,17 @@ __device__ __forceinline__ DiagBits diag_bits(const DevState &S, long long slot)
     return {M[l], M[kBlk + l], M[2 * kBlk + l]};
 }
 
+__device__ __forceinline__ unsigned long long rotr64(unsigned long long v, int s) {
+    return s ? ((v >> s) | (v << (64 - s))) : v;
+}
+
+// The tile's pair bits of this lane re-indexed by rotation step: bit k-1
+// describes the pair met at step k, j lane (l + k*ldir) & 63, so a step
+// tests one bit of a 32-bit half at a uniform position.
+__device__ __forceinline__ unsigned long long step_bits(unsigned long long m, int l, int ldir) {
+    return ldir == 1 ? rotr64(m, (l + 1) & 63) : rotr64(__builtin_bitreverse64(m), (64 - l) & 63);
+}
+
 // does lane l take the pair with lane jlane at rotation step k
 __device__ __forceinline__ bool diag_take(const DiagBits &D, int jlane, int k, int l, bool diag) {
     return ((D.mc >> jlane) & 1ull) && (!diag || k < 32 || l < 32);
@@ -938,7 +949,10 @@ template <typename R> struct BornAtom { R x, y, z, rad, vj; };
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
-    __shared__ BornAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
+    // list tiles: I atoms at [0, 64), J atoms at [64, 128); dense tiles: the
+    // J block twice in a row, so that step k reads lane (l + k*ldir) & 63 at
+    // one address per lane
+    __shared__ BornAtom<R> st[kWavesPerGroup][2 * kBlk];
     __shared__ double gi_acc[kWavesPerGroup][kBlk], gj_acc[kWavesPerGroup][kBlk];
     const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
     const int lg = P.lg_split;
@@ -973,26 +987,38 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
                     const double2 g = S.gb64[a], h = S.gb64[b];
                     rad = g.x; vj = g.y; rj = h.x; vjj = h.y;
                 }
-                R xj = pj.x, yj = pj.y, zj = pj.z, gj = 0;
+                st[w][l] = {pj.x, pj.y, pj.z, rj, vjj};
+                st[w][l + kBlk] = {pj.x, pj.y, pj.z, rj, vjj};
+                const int ldir = rot_dir();
+                // candidates that are not excluded, by step
+                unsigned long long T = step_bits(D.mc, l, ldir) & ~step_bits(D.mx, l, ldir);
+                if (diag && l >= 32) T &= ~(1ull << 31); // step 32 meets each pair twice: the lower lane takes it
+                wave_lds_sync();
+                R gj = 0;
                 double gi = 0;
                 int np = 0;
-                const int ldir = rot_dir();
                 const int steps = diag ? 32 : kBlk;
                 const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
-                xj = rot_to(xj, k0, ldir); yj = rot_to(yj, k0, ldir); zj = rot_to(zj, k0, ldir);
-                rj = rot_to(rj, k0, ldir); vjj = rot_to(vjj, k0, ldir);
-                for (int k = k0 + 1; k <= k1; ++k) {
-                    xj = rot(xj); yj = rot(yj); zj = rot(zj); rj = rot(rj); vjj = rot(vjj); gj = rot(gj);
-                    const int jlane = (l + k * ldir) & 63;
-                    const R dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
-                    const R r2 = dx * dx + dy * dy + dz * dz;
-                    if (diag_take(D, jlane, k, l, diag) && !((D.mx >> jlane) & 1ull) && r2 <= still2) {
-                        const R term = born_term<R>(r2, rad + rj, invP5, P4, P5);
-                        gi += (double)(term * vjj);
-                        gj += term * vj;
-                        ++np;
+                const int dj = ldir == 1 ? 1 : -1;
+                int jx = (ldir == 1 ? l : l + kBlk) + k0 * dj;
+                for (int h = k0 >> 5; h < (k1 + 31) >> 5; ++h) {
+                    const unsigned int Tw = (unsigned int)(T >> (32 * h));
+                    const int kk1 = min(32, k1 - 32 * h);
+                    for (int kk = max(0, k0 - 32 * h); kk < kk1; ++kk) {
+                        jx += dj;
+                        const BornAtom<R> B = st[w][jx];
+                        gj = rot(gj);
+                        const R dx = pi.x - B.x, dy = pi.y - B.y, dz = pi.z - B.z;
+                        const R r2 = dx * dx + dy * dy + dz * dz;
+                        if (((Tw >> kk) & 1u) && r2 <= still2) {
+                            const R term = born_term<R>(r2, rad + B.rad, invP5, P4, P5);
+                            gi += (double)(term * B.vj);
+                            gj += term * vj;
+                            ++np;
+                        }
                     }
                 }
+                wave_lds_sync();
                 int *st = S.inst_status + inst;
                 gj = rot_home(gj, k1, ldir);
                 if (diag) {
@@ -1017,12 +1043,12 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
             const AtomXYZ<R> pj = load_xyzq<R>(S, b, tile_shift<R>(S, I, J));
             if constexpr (Prec<R>::fixed) {
                 const float2 gi = S.gb32[a], gj = S.gb32[b];
-                si[w][l] = {pi.x, pi.y, pi.z, gi.x, gi.y};
-                sj[w][l] = {pj.x, pj.y, pj.z, gj.x, gj.y};
+                st[w][l] = {pi.x, pi.y, pi.z, gi.x, gi.y};
+                st[w][kBlk + l] = {pj.x, pj.y, pj.z, gj.x, gj.y};
             } else {
                 const double2 gi = S.gb64[a], gj = S.gb64[b];
-                si[w][l] = {pi.x, pi.y, pi.z, gi.x, gi.y};
-                sj[w][l] = {pj.x, pj.y, pj.z, gj.x, gj.y};
+                st[w][l] = {pi.x, pi.y, pi.z, gi.x, gi.y};
+                st[w][kBlk + l] = {pj.x, pj.y, pj.z, gj.x, gj.y};
             }
             gi_acc[w][l] = 0;
             gj_acc[w][l] = 0;
@@ -1033,7 +1059,7 @@ __global__ void __launch_bounds__(256) k_born(DevState S, PairParams P, int blk_
                 e_nx = q + kBlk < nb ? (int)L[q + kBlk] : 0;
                 if (q < nb) {
                     const int ii = e & 63, jj = (e >> 6) & 63;
-                    const BornAtom<R> A = si[w][ii], B = sj[w][jj];
+                    const BornAtom<R> A = st[w][ii], B = st[w][kBlk + jj];
                     const R dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
                     const R r2 = dx * dx + dy * dy + dz * dz;
                     const R term = born_term<R>(r2, A.rad + B.rad, invP5, P4, P5);

@@ -158,16 +158,6 @@ __device__ __forceinline__ float rotd(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x13C, 0xF, 0xF, true));
 }
 
-__device__ __forceinline__ unsigned long long rotr64(unsigned long long v, int s) {
-    return s ? ((v >> s) | (v << (64 - s))) : v;
-}
-
-// The tile's pair bits of this lane re-indexed by rotation step: bit k-1
-// describes the pair met at step k, j lane (l + k*ldir) & 63.
-__device__ __forceinline__ unsigned long long step_bits(unsigned long long m, int l, int ldir) {
-    return ldir == 1 ? rotr64(m, (l + 1) & 63) : rotr64(__builtin_bitreverse64(m), (64 - l) & 63);
-}
-
 // One tile at a time per wavefront, each wavefront with its own 8 KB of
 // LDS --
 //   dense tile: the j block's (x,y,z,q) at [0, 128) and (R, sqrt eps, a,
