@@ -718,9 +718,12 @@ int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, in
 
 // One evaluation: MODE 0 = calcForces, MODE 1 = calcEnergies. Without
 // finalize the caller launches a kernel that finalises the forces itself
-// (the velocity-Verlet kick).
+// (the velocity-Verlet kick). prepared: the set-up kernel's work was done by
+// the previous launch (a velocity-Verlet closing kernel), so the evaluation
+// starts without it.
 template <typename R>
-int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2 = nullptr, int f2_stride = 0) {
+int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2 = nullptr, int f2_stride = 0,
+                bool prepared = false) {
     const DevState S = c->S();
     const Span sp = span_of(c, id);
     const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk;
@@ -732,9 +735,11 @@ int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2
     {
         ProfScope ps(c, PDGPU_KERNEL_ATOM);
         const int nt = std::max(na, ns);
-        if (gb)
-            hipLaunchKernelGGL(k_prepare<R>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, S, a0, a1, sp.s0, sp.s1);
-        else
+        if (gb) {
+            if (!prepared)
+                hipLaunchKernelGGL(k_prepare<R>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, S, a0, a1, sp.s0,
+                                   sp.s1);
+        } else
             hipLaunchKernelGGL(k_clear_grad<R>, dim3((nt + 255) / 256), dim3(256), 0, c->stream, S, a0, a1, sp.s0,
                                sp.s1);
         if ((soft || nonb) && c->mixed) // block-relative FP32 positions (k_prepare's job when GB runs)

@@ -220,7 +220,12 @@ __global__ void k_vv_pd(DevState S, InstMD *md, const double *minv, double *vx, 
 
 // k_finalize and the closing half kick of a velocity-Verlet step in one
 // pass; with next != 0 also the following step's opening half kick and
-// drift (the same arithmetic as the kernels in sequence).
+// drift (the same arithmetic as the kernels in sequence), and the following
+// evaluation's set-up (k_prepare's work: FP32 positions of the drifted
+// atoms, Born sums and block records cleared, instance records cleared), so
+// that evaluation starts without a set-up launch. Only the last step's
+// energies are read, so intermediate steps clear their block records
+// instead of folding them into the instance records.
 template <typename R>
 __global__ void k_finalize_kick(DevState S, InstMD *md, const double *minv, double *vx, double *vy, double *vz,
                                 double *px, double *py, double *pz, int atom_begin, int atom_end, double dt,
@@ -229,7 +234,7 @@ __global__ void k_finalize_kick(DevState S, InstMD *md, const double *minv, doub
     const int b = a / kBlk;
     const bool in = a < atom_end;
     const int inst = in ? S.blk_inst[b] : -1;
-    blk_fold<R>(S, b, inst);
+    if (!next) blk_fold<R>(S, b, inst);
     const bool v = in && inst >= 0 && (a % kBlk) < S.blk_n[b];
     double ek = 0;
     if (v) {
@@ -248,6 +253,12 @@ __global__ void k_finalize_kick(DevState S, InstMD *md, const double *minv, doub
             px[a] += ux * s; py[a] += uy * s; pz[a] += uz * s;
         }
         vx[a] = ux; vy[a] = uy; vz[a] = uz;
+    }
+    if (next && in) {
+        blk_clear(S, a);
+        acc_store<R>(S.acc_g, a, 0.0, 1.0);
+        if constexpr (Prec<R>::fixed) store_xyzq32(S, a);
+        if (lane_id() == 0 && inst >= 0 && S.blk_first[b] == b) inst_clear(S, inst);
     }
     if (want_ekin) {
         ek = wave_sum(ek);
