@@ -676,8 +676,11 @@ PairKernel pair_mixed_kernel(const PairParams &P) {
 
 // The GB kernels of one evaluation: enumeration, Born sums, Born radii,
 // pair pass and chain rule (MODE 0) or the energy-path pair pass (MODE 1).
+// With nbonded > 0 (MODE 0) the bonded terms B run as extra workgroups of
+// the chain-rule launch.
 template <typename R>
-int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, int mode) {
+int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, int mode, const BondedState &B,
+              int nbonded) {
     const int a0 = sp.b0 * kBlk, a1 = sp.b1 * kBlk, na = a1 - a0;
     int g[kGridKinds], t[kGridKinds];
     for (int k = 0; k < kGridKinds; ++k) pair_grid(c, k, tiles << c->P.lg_split, &g[k], &t[k]); // work items
@@ -706,8 +709,12 @@ int launch_gb(pdgpu_ctx *c, const DevState &S, const Span &sp, int64_t tiles, in
                                    sp.b1, t[kGridPair]);
         }
         ProfScope ps(c, PDGPU_KERNEL_CHAIN);
-        hipLaunchKernelGGL(k_chain<R>, dim3(g[kGridChain]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
-                           t[kGridChain]);
+        if (nbonded > 0)
+            hipLaunchKernelGGL(k_chain_bonded<R>, dim3(g[kGridChain] + (nbonded + 255) / 256), dim3(256), 0,
+                               c->stream, S, c->P, sp.b0, sp.b1, t[kGridChain], g[kGridChain], B);
+        else
+            hipLaunchKernelGGL(k_chain<R>, dim3(g[kGridChain]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
+                               t[kGridChain]);
     } else {
         ProfScope ps(c, PDGPU_KERNEL_PAIR);
         hipLaunchKernelGGL((k_pair<R, 1>), dim3(g[kGridPair]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
@@ -745,7 +752,12 @@ int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2
         if ((soft || nonb) && c->mixed) // block-relative FP32 positions (k_prepare's job when GB runs)
             hipLaunchKernelGGL(k_xyzq32, dim3((na + 255) / 256), dim3(256), 0, c->stream, S, a0, a1);
     }
-    if (gb) TRY(launch_gb<R>(c, S, sp, span_tiles(c, id), mode));
+    const BondedState B = c->B(id);
+    const int nterms = (B.b1 - B.b0) + (B.a1 - B.a0) + (B.d1 - B.d0);
+    const bool bonded = (c->opt.components & PDGPU_FF_BONDED) && nterms > 0;
+    // with GB forces the bonded terms ride along with the chain-rule launch
+    const bool fused = bonded && gb && mode == 0;
+    if (gb) TRY(launch_gb<R>(c, S, sp, span_tiles(c, id), mode, B, fused ? nterms : 0));
     if (nonb) {
         // FF_NonBonded alone: the energy path's pair term without the GB
         // energy (MODE 2); its calcEnergies is the same evaluation
@@ -774,9 +786,7 @@ int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2
         hipLaunchKernelGGL(k_soft<R>, dim3(g[kGridPair]), dim3(256), 0, c->stream, S, c->P, sp.b0, sp.b1,
                            t[kGridPair]);
     }
-    const BondedState B = c->B(id);
-    const int nterms = (B.b1 - B.b0) + (B.a1 - B.a0) + (B.d1 - B.d0);
-    if ((c->opt.components & PDGPU_FF_BONDED) && nterms > 0) {
+    if (bonded && !fused) {
         ProfScope ps(c, PDGPU_KERNEL_BONDED);
         hipLaunchKernelGGL(k_bonded<R>, dim3((nterms + 255) / 256), dim3(256), 0, c->stream, S, B);
     }

@@ -165,13 +165,20 @@ __device__ double dihedral_term(const DevState &S, const GradSink<R> &G, const i
     return e;
 }
 
-// One launch over the record ranges [b0,b1) bonds, [a0,a1) angles,
-// [d0,d1) dihedrals, one thread per record.
-template <typename R>
-__global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
-    __shared__ unsigned long long win[3][kWin];
-    __shared__ int s_amin;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// LDS of one workgroup of the bonded terms.
+struct BondedLds {
+    unsigned long long win[3][kWin];
+    int amin;
+};
+
+// Workgroup `group` (256 threads) of the record ranges [b0,b1) bonds,
+// [a0,a1) angles, [d0,d1) dihedrals, one thread per record. Called by whole
+// workgroups (it synchronises the workgroup).
+template <typename R> __device__ __forceinline__ void bonded_group(const DevState &S, const BondedState &B, int group,
+                                                                   BondedLds &L) {
+    unsigned long long(*win)[kWin] = L.win;
+    int &s_amin = L.amin;
+    const int t = group * 256 + threadIdx.x;
     const int nb = B.b1 - B.b0, na = B.a1 - B.a0, nd = B.d1 - B.d0;
     // decode this thread's record
     int kind = -1, r = 0, inst = -1;
@@ -258,6 +265,28 @@ __global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
             }
         }
     }
+}
+
+// The bonded terms on their own launch (contexts without the GB pass, and
+// the energy path). With GB forces they run as extra workgroups of k_chain.
+template <typename R> __global__ void __launch_bounds__(256) k_bonded(DevState S, BondedState B) {
+    __shared__ BondedLds L;
+    bonded_group<R>(S, B, blockIdx.x, L);
+}
+
+// The chain-rule pass (workgroups [0, ngroups)) and the bonded terms
+// (workgroups [ngroups, gridDim.x)) in one launch: both add to the force
+// sums the Born-radius kernel cleared, and the bonded terms' latency hides
+// behind the tiles instead of taking a launch of its own.
+template <typename R>
+__global__ void __launch_bounds__(256) k_chain_bonded(DevState S, PairParams P, int blk_begin, int blk_end, int tpw,
+                                                      int ngroups, BondedState B) {
+    __shared__ union {
+        ChainLds<R> chain;
+        BondedLds bonded;
+    } U;
+    if ((int)blockIdx.x < ngroups) chain_work<R>(S, P, blk_begin, blk_end, tpw, ngroups, U.chain);
+    else bonded_group<R>(S, B, (int)blockIdx.x - ngroups, U.bonded);
 }
 
 // Clears the force accumulators where the GB kernels (which clear them in

@@ -305,12 +305,14 @@ template <typename T> __device__ __forceinline__ T rot_home(T v, int k, int ldir
 // Persistent-grid chunk assignment. Consecutive chunks go to workgroups on
 // the same XCD (workgroups are dealt round-robin over 8 XCDs), so the j-atom
 // data of neighbouring tiles is shared through one L2.
-__device__ __forceinline__ int wave_global_id() {
+// ngroups: the workgroups taking part (the first ngroups of the grid).
+__device__ __forceinline__ int wave_global_id(int ngroups) {
     const int nx = 8;
     int b = blockIdx.x;
-    if ((gridDim.x % nx) == 0) b = (blockIdx.x % nx) * (gridDim.x / nx) + blockIdx.x / nx;
+    if ((ngroups % nx) == 0) b = (blockIdx.x % nx) * (ngroups / nx) + blockIdx.x / nx;
     return b * kWavesPerGroup + wave_in_group();
 }
+__device__ __forceinline__ int wave_global_id() { return wave_global_id(gridDim.x); }
 
 // ------------------------------------------------------ per-instance setup
 // Lane a % 64 < kBAcc of a block-aligned per-atom launch clears one field of
@@ -1461,18 +1463,29 @@ __device__ __forceinline__ R chain_coef(R r2, R sR, R invP5, R P4, R P5) {
     return P4 * ((R)4 * ccf - dccf) * Prec<R>::rcp(r6);
 }
 
+template <typename R> struct ChainLds {
+    ChainAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
+    double fi[kWavesPerGroup][3][kBlk], fj[kWavesPerGroup][3][kBlk];
+};
+
+// The chain-rule pass over the tiles, by the first ngroups workgroups of the
+// grid (k_chain; with GB forces the bonded terms run as further workgroups
+// of the same launch, k_chain_bonded).
 template <typename R>
-__global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
-    __shared__ ChainAtom<R> si[kWavesPerGroup][kBlk], sj[kWavesPerGroup][kBlk];
-    __shared__ double fi[kWavesPerGroup][3][kBlk], fj[kWavesPerGroup][3][kBlk];
+__device__ __forceinline__ void chain_work(const DevState &S, const PairParams &P, int blk_begin, int blk_end, int tpw,
+                                           int ngroups, ChainLds<R> &Ls) {
+    auto &si = Ls.si;
+    auto &sj = Ls.sj;
+    auto &fi = Ls.fi;
+    auto &fj = Ls.fj;
     const int t_begin = S.row_off[blk_begin], t_end = min(S.row_off[blk_end], S.plist_cap);
     const int lg = P.lg_split;
     const int nt = t_end - t_begin, u_end = max(nt, 0) << lg;
-    const int nw = gridDim.x * kWavesPerGroup;
+    const int nw = ngroups * kWavesPerGroup;
     if (tpw <= 0) tpw = max(1, (u_end + nw - 1) / nw);
     const int l = lane_id(), w = wave_in_group();
     const R invP5 = (R)P.invP5, P4 = (R)P.P4, P5 = (R)P.P5, still2 = (R)P.still2;
-    for (int c = wave_global_id() * tpw; c < u_end; c += nw * tpw) {
+    for (int c = wave_global_id(ngroups) * tpw; c < u_end; c += nw * tpw) {
         const int cend = min(u_end, c + tpw);
         for (int u = c; u < cend; ++u) {
             const TileItem it = tile_item(u, t_begin, nt, lg);
@@ -1600,6 +1613,12 @@ __global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk
             wave_lds_sync();
         }
     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_chain(DevState S, PairParams P, int blk_begin, int blk_end, int tpw) {
+    __shared__ ChainLds<R> Ls;
+    chain_work<R>(S, P, blk_begin, blk_end, tpw, gridDim.x, Ls);
 }
 
 // ------------------------------------------------ FF_SoftVDW soft core
