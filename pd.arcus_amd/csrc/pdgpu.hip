@@ -182,7 +182,11 @@ struct pdgpu_ctx {
     // per block
     DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, blk_acc, row_count, row_off, tiles, tile_slot;
     // per instance
-    DevBuf any_need, inst_need, inst_moved, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_energy, inst_count, inst_status, inst_md;
+    DevBuf any_need, inst_need, inst_moved, inst_nat, inst_mid, inst_band_off, inst_sp_off, inst_sp_n, inst_md;
+    // per-instance records, one allocation: energies [max_inst][kENum], then
+    // counters [max_inst][kCNum], then status [max_inst], so that the records
+    // of every slot come back in one copy
+    DevBuf inst_rec;
     DevBuf fox, foy, foz; // Beeman's old forces (N), per atom
     DevBuf is_h;          // Particle::isHydrogen per atom (Langevin without hydrogens)
     double thermo_tau = 0, thermo_temp = 300; // Berendsen thermostat of the PD MD calls (tau 0: none)
@@ -209,7 +213,7 @@ struct pdgpu_ctx {
     // host mirrors of per-block metadata
     std::vector<int> h_blk_inst, h_blk_n, h_blk_first;
     // pinned host staging for per-instance results (energies, counts, status)
-    void *h_stage = nullptr;
+    void *h_stage = nullptr, *d_stage = nullptr; // pinned, host and device addresses
     size_t h_stage_bytes = 0;
     // staged uploads: host data of an instance (topology, term records,
     // positions) is gathered into one blob, copied once through pinned
@@ -256,8 +260,9 @@ struct pdgpu_ctx {
         s.inst_mid = (const int *)inst_mid.p; s.inst_band_off = (const long long *)inst_band_off.p;
         s.inst_sp_off = (const int *)inst_sp_off.p; s.inst_sp_n = (const int *)inst_sp_n.p;
         s.band = (const int8_t *)band.p; s.sparse = (const int4 *)sparse.p;
-        s.inst_energy = inst_energy.p; s.inst_count = (unsigned long long *)inst_count.p;
-        s.inst_status = (int *)inst_status.p;
+        s.inst_energy = inst_rec.p;
+        s.inst_count = inst_rec.p ? (unsigned long long *)((char *)inst_rec.p + (size_t)max_inst * kENum * 8) : nullptr;
+        s.inst_status = inst_rec.p ? (int *)((char *)s.inst_count + (size_t)max_inst * kCNum * 8) : nullptr;
         s.inst_need = nullptr;
         s.any_need = nullptr;
         s.plist = (uint16_t *)plist.p;
@@ -804,23 +809,25 @@ int launch_eval(pdgpu_ctx *c, int id, int mode, bool finalize = true, double *f2
 int read_energies(pdgpu_ctx *c, int id, pdgpu_energies *out, int mode) {
     const Span sp = span_of(c, id);
     const int ns = sp.s1 - sp.s0;
-    // one pinned staging area: the three copies are plain DMA, no bounce
+    // host-mapped pinned staging, filled by k_publish on the stream
     const size_t eb = (size_t)ns * kENum * 8, cb = (size_t)ns * kCNum * 8, sb = (size_t)ns * 4;
     if (c->h_stage_bytes < eb + cb + sb) {
         if (c->h_stage) (void)hipHostFree(c->h_stage);
-        c->h_stage = nullptr;
+        c->h_stage = c->d_stage = nullptr;
         c->h_stage_bytes = 0;
-        HIPCHECK(c, hipHostMalloc(&c->h_stage, eb + cb + sb, hipHostMallocDefault));
+        HIPCHECK(c, hipHostMalloc(&c->h_stage, eb + cb + sb, hipHostMallocMapped));
+        HIPCHECK(c, hipHostGetDevicePointer(&c->d_stage, c->h_stage, 0));
         c->h_stage_bytes = eb + cb + sb;
     }
     const long long *e = (const long long *)c->h_stage;
     const unsigned long long *cnt = (const unsigned long long *)((char *)c->h_stage + eb);
     const int *st = (const int *)((char *)c->h_stage + eb + cb);
-    HIPCHECK(c, hipMemcpyAsync((void *)e, (char *)c->inst_energy.p + (size_t)sp.s0 * kENum * 8, eb,
-                               hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipMemcpyAsync((void *)cnt, (char *)c->inst_count.p + (size_t)sp.s0 * kCNum * 8, cb,
-                               hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipMemcpyAsync((void *)st, (int *)c->inst_status.p + sp.s0, sb, hipMemcpyDeviceToHost, c->stream));
+    const int nblk = (int)std::min<size_t>(256, ((size_t)ns * kENum + 255) / 256);
+    if (ns > 0) {
+        hipLaunchKernelGGL(k_publish, dim3(nblk), dim3(256), 0, c->stream, c->S(), sp.s0, ns,
+                           (unsigned long long *)c->d_stage);
+        HIPCHECK(c, hipGetLastError());
+    }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     int rc = PDGPU_OK;
     for (int k = 0; k < ns; ++k) {
@@ -1058,9 +1065,7 @@ extern "C" int pdgpu_ctx_create(const pdgpu_options *o, pdgpu_ctx **out) {
     if (rc == PDGPU_OK) rc = resize(c, c->inst_band_off, ms, 8);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_sp_off, ms, 4);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_sp_n, ms, 4);
-    if (rc == PDGPU_OK) rc = resize(c, c->inst_energy, (size_t)ms * kENum, 8);
-    if (rc == PDGPU_OK) rc = resize(c, c->inst_count, (size_t)ms * kCNum, 8);
-    if (rc == PDGPU_OK) rc = resize(c, c->inst_status, ms, 4);
+    if (rc == PDGPU_OK) rc = resize(c, c->inst_rec, (size_t)ms * (kENum * 8 + kCNum * 8 + 4), 1);
     if (rc == PDGPU_OK) rc = resize(c, c->inst_md, (size_t)ms, sizeof(InstMD));
     if (rc == PDGPU_OK) rc = resize(c, c->inst_cslot, (size_t)ms, 8);
     if (rc == PDGPU_OK) rc = grow_atoms(c, 64);
@@ -1095,8 +1100,8 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
                      &c->pp32, &c->born, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy, &c->acc_fz, &c->fx, &c->fy, &c->fz,
                      &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
                      &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->blk_acc, &c->row_count, &c->row_off,
-                     &c->tiles, &c->tile_slot, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_energy,
-                     &c->inst_count, &c->inst_status, &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
+                     &c->tiles, &c->tile_slot, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_rec,
+                     &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
                      &c->bd_idx, &c->bd_par, &c->an_idx, &c->an_par, &c->dh_idx, &c->dh_meta, &c->dh_vn, &c->dh_n,
                      &c->dh_gamma, &c->cand, &c->cand_n, &c->inst_cslot, &c->rx, &c->ry, &c->rz,
                      &c->inst_moved, &c->any_need, &c->fox, &c->foy, &c->foz, &c->rng_ctr, &c->stage, &c->upl, &c->is_h};

@@ -1808,6 +1808,23 @@ __global__ void k_finalize(DevState S, int atom_begin, int atom_end, double *f2,
     }
 }
 
+// Copies the records of slots [s0, s0 + ns) into host-mapped pinned memory
+// laid out as [ns][kENum] energies, [ns][kCNum] counters, [ns] status words,
+// as the last launch of the stream before the host waits. A copy-engine
+// transfer queued behind the closing kernel started ~0.1 ms after that
+// kernel ended; this launch follows it within a few microseconds.
+__global__ void k_publish(DevState S, int s0, int ns, unsigned long long *out) {
+    const size_t ne = (size_t)ns * kENum, nc = (size_t)ns * kCNum;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long *e = (const unsigned long long *)S.inst_energy + (size_t)s0 * kENum;
+    const unsigned long long *c = S.inst_count + (size_t)s0 * kCNum;
+    for (size_t k = t; k < ne; k += stride) out[k] = e[k];
+    for (size_t k = t; k < nc; k += stride) out[ne + k] = c[k];
+    int *st = (int *)(out + ne + nc);
+    for (size_t k = t; k < (size_t)ns; k += stride) st[k] = S.inst_status[s0 + k];
+}
+
 // ------------------------------------------------------ PD-format export
 // Packed list of one instance exactly as calcNewList_InfiniteSpace writes
 // it (workspace/neighbourlist.cpp:263-315): j ascending, r^2 in FP64 with the
