@@ -1513,32 +1513,41 @@ __device__ __forceinline__ void chain_work(const DevState &S, const PairParams &
                 }
                 const ChainAtom<R> A = {pi.x, pi.y, pi.z, rad, vjv, vi ? chain_weight<R>(S, a) : (R)0};
                 ChainAtom<R> B = {pj.x, pj.y, pj.z, rj, vjj, vjb ? chain_weight<R>(S, b) : (R)0};
+                // i-side sums in the pass's precision: at most 64 terms per
+                // lane before the flush
                 R fjx = 0, fjy = 0, fjz = 0;
-                double fix = 0, fiy = 0, fiz = 0;
+                R fix = 0, fiy = 0, fiz = 0;
                 const int ldir = rot_dir();
+                // candidates that are not excluded, by step (bit k-1: step k)
+                unsigned long long T = step_bits(D.mc, l, ldir) & ~step_bits(D.mx, l, ldir);
+                if (diag && l >= 32) T &= ~(1ull << 31); // step 32 meets each pair twice: the lower lane takes it
                 const int steps = diag ? 32 : kBlk;
                 const int k0 = seg_step(it.s, steps, lg), k1 = seg_step(it.s + 1, steps, lg);
                 B.x = rot_to(B.x, k0, ldir); B.y = rot_to(B.y, k0, ldir); B.z = rot_to(B.z, k0, ldir);
                 B.rad = rot_to(B.rad, k0, ldir); B.vj = rot_to(B.vj, k0, ldir); B.wt = rot_to(B.wt, k0, ldir);
-                for (int k = k0 + 1; k <= k1; ++k) {
-                    B.x = rot(B.x); B.y = rot(B.y); B.z = rot(B.z);
-                    B.rad = rot(B.rad); B.vj = rot(B.vj); B.wt = rot(B.wt);
-                    fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
-                    const int jlane = (l + k * ldir) & 63;
-                    const R dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
-                    const R r2 = dx * dx + dy * dy + dz * dz;
-                    if (diag_take(D, jlane, k, l, diag) && !((D.mx >> jlane) & 1ull) && r2 <= still2) {
-                        const R coef = chain_coef<R>(r2, A.rad + B.rad, invP5, P4, P5) *
-                                       (A.wt * B.vj + B.wt * A.vj);
-                        const R gx = coef * dx, gy = coef * dy, gz = coef * dz;
-                        fix += (double)gx; fiy += (double)gy; fiz += (double)gz;
-                        fjx -= gx; fjy -= gy; fjz -= gz;
+                for (int h = k0 >> 5; h < (k1 + 31) >> 5; ++h) {
+                    const unsigned int Tw = (unsigned int)(T >> (32 * h));
+                    const int kk1 = min(32, k1 - 32 * h);
+                    for (int kk = max(0, k0 - 32 * h); kk < kk1; ++kk) {
+                        B.x = rot(B.x); B.y = rot(B.y); B.z = rot(B.z);
+                        B.rad = rot(B.rad); B.vj = rot(B.vj); B.wt = rot(B.wt);
+                        fjx = rot(fjx); fjy = rot(fjy); fjz = rot(fjz);
+                        const R dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
+                        const R r2 = dx * dx + dy * dy + dz * dz;
+                        if (((Tw >> kk) & 1u) && r2 <= still2) {
+                            const R coef = chain_coef<R>(r2, A.rad + B.rad, invP5, P4, P5) *
+                                           (A.wt * B.vj + B.wt * A.vj);
+                            const R gx = coef * dx, gy = coef * dy, gz = coef * dz;
+                            fix += gx; fiy += gy; fiz += gz;
+                            fjx -= gx; fjy -= gy; fjz -= gz;
+                        }
                     }
                 }
                 int *st = S.inst_status + inst;
                 fjx = rot_home(fjx, k1, ldir); fjy = rot_home(fjy, k1, ldir); fjz = rot_home(fjz, k1, ldir);
                 if (diag) {
-                    const double gx = fix + (double)fjx, gy = fiy + (double)fjy, gz = fiz + (double)fjz;
+                    const double gx = (double)fix + (double)fjx, gy = (double)fiy + (double)fjy,
+                                 gz = (double)fiz + (double)fjz;
                     if (vi) {
                         acc_add<R>(S.acc_fx, a, gx, kForceScale, st);
                         acc_add<R>(S.acc_fy, a, gy, kForceScale, st);
@@ -1546,9 +1555,9 @@ __device__ __forceinline__ void chain_work(const DevState &S, const PairParams &
                     }
                 } else {
                     if (vi) {
-                        acc_add<R>(S.acc_fx, a, fix, kForceScale, st);
-                        acc_add<R>(S.acc_fy, a, fiy, kForceScale, st);
-                        acc_add<R>(S.acc_fz, a, fiz, kForceScale, st);
+                        acc_add<R>(S.acc_fx, a, (double)fix, kForceScale, st);
+                        acc_add<R>(S.acc_fy, a, (double)fiy, kForceScale, st);
+                        acc_add<R>(S.acc_fz, a, (double)fiz, kForceScale, st);
                     }
                     if (vjb) {
                         acc_add<R>(S.acc_fx, b, (double)fjx, kForceScale, st);
