@@ -178,7 +178,7 @@ struct pdgpu_ctx {
     // device buffers
     // per atom
     DevBuf px, py, pz, q, lj64, gb64, t123, xyzq32, lj32, gb32, pp32, born, acc_g, acc_deda, acc_fx, acc_fy, acc_fz;
-    DevBuf fx, fy, fz, deda_out, vx, vy, vz, minv, opx, opy, opz, rx, ry, rz;
+    DevBuf fx, fy, fz, vx, vy, vz, minv, opx, opy, opz, rx, ry, rz;
     // per block
     DevBuf blk_inst, blk_n, blk_first, blk_ctr, blk_half, blk_box, blk_acc, row_count, row_off, tiles, tile_slot;
     // per instance
@@ -251,7 +251,6 @@ struct pdgpu_ctx {
         s.born = born.p; s.acc_g = acc_g.p; s.acc_deda = acc_deda.p;
         s.acc_fx = acc_fx.p; s.acc_fy = acc_fy.p; s.acc_fz = acc_fz.p;
         s.fx = (double *)fx.p; s.fy = (double *)fy.p; s.fz = (double *)fz.p;
-        s.deda_out = (double *)deda_out.p;
         s.blk_inst = (const int *)blk_inst.p; s.blk_n = (const int *)blk_n.p; s.blk_first = (const int *)blk_first.p;
         s.blk_acc = (unsigned long long *)blk_acc.p;
         s.blk_ctr = (double4 *)blk_ctr.p; s.blk_half = (float4 *)blk_half.p; s.blk_box = (double4 *)blk_box.p;
@@ -393,8 +392,8 @@ int up_commit(pdgpu_ctx *c) {
 int grow_atoms(pdgpu_ctx *c, int64_t blocks) {
     const size_t na = (size_t)blocks * kBlk;
     DevBuf *d8[] = {&c->px, &c->py, &c->pz, &c->q, &c->t123, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy,
-                    &c->acc_fz, &c->fx, &c->fy, &c->fz, &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv,
-                    &c->opx, &c->opy, &c->opz, &c->rx, &c->ry, &c->rz, &c->fox, &c->foy, &c->foz};
+                    &c->acc_fz, &c->fx, &c->fy, &c->fz, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx,
+                    &c->opy, &c->opz, &c->rx, &c->ry, &c->rz, &c->fox, &c->foy, &c->foz};
     for (DevBuf *b : d8) TRY(resize(c, *b, na, 8));
     TRY(resize(c, c->is_h, na, 1));
     TRY(resize(c, c->lj64, na, 16));
@@ -1098,7 +1097,7 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf *all[] = {&c->px, &c->py, &c->pz, &c->q, &c->lj64, &c->gb64, &c->t123, &c->xyzq32, &c->lj32, &c->gb32,
                      &c->pp32, &c->born, &c->acc_g, &c->acc_deda, &c->acc_fx, &c->acc_fy, &c->acc_fz, &c->fx, &c->fy, &c->fz,
-                     &c->deda_out, &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
+                     &c->vx, &c->vy, &c->vz, &c->minv, &c->opx, &c->opy, &c->opz, &c->blk_inst,
                      &c->blk_n, &c->blk_first, &c->blk_ctr, &c->blk_half, &c->blk_box, &c->blk_acc, &c->row_count, &c->row_off,
                      &c->tiles, &c->tile_slot, &c->inst_need, &c->inst_nat, &c->inst_mid, &c->inst_band_off, &c->inst_sp_off, &c->inst_sp_n, &c->inst_rec,
                      &c->inst_md, &c->band, &c->sparse, &c->plist, &c->tile_cnt, &c->overflow,
@@ -1121,7 +1120,9 @@ extern "C" void pdgpu_ctx_destroy(pdgpu_ctx *c) {
 extern "C" int64_t pdgpu_instance_bytes(const pdgpu_ctx *c, int32_t natoms, int32_t mid, int32_t nsparse) {
     if (!c || natoms <= 0) return -1;
     const int64_t nb = (natoms + kBlk - 1) / kBlk;
-    const int64_t per_atom = 8 * 24 + 16 * 2 + (c->mixed ? 4 + 16 + 16 : 8);
+    // the 26 double arrays of grow_atoms, the hydrogen flags, the FP64 LJ and
+    // GB records, the Born radii and the mixed path's FP32 records
+    const int64_t per_atom = 8 * 26 + 1 + 16 * 2 + (c->mixed ? 4 + 16 + 8 + 8 + 16 : 8);
     const int64_t per_block = 4 * 3 + 32 * 2 + 16 + 8 * kBAcc + 4 * 2;
     // per possible tile: the tile record, a candidate slot, a pair-list slot
     // and its segment counts

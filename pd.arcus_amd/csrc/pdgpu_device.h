@@ -93,7 +93,6 @@ struct DevState {
     void *acc_deda;               // dE/da (int64 fixed / double)
     void *acc_fx, *acc_fy, *acc_fz; // dE/dx (int64 fixed / double)
     double *fx, *fy, *fz;         // finalized forces, kcal/mol/A (= -dE/dx)
-    double *deda_out;             // finalized dE/da, kcal/mol/A
     // per block
     const int *blk_inst;          // instance slot or -1
     const int *blk_n;             // valid atoms in block
@@ -497,6 +496,18 @@ __device__ __forceinline__ void store_xyzq32(const DevState &S, int a) {
                               (float)S.q[a]);
 }
 
+// The same for the coordinates only, where the charge in .w is already
+// there: every list build writes the whole record (k_xyzq32) and charges
+// only change with an upload, which is followed by a build.
+__device__ __forceinline__ void store_xyz32(const DevState &S, int a) {
+    const int b = a / kBlk;
+    const double4 c = S.blk_ctr[b];
+    float *p = reinterpret_cast<float *>(S.xyzq32 + a);
+    p[0] = (float)(S.px[a] - c.x);
+    p[1] = (float)(S.py[a] - c.y);
+    p[2] = (float)(S.pz[a] - c.z);
+}
+
 __global__ void k_xyzq32(DevState S, int atom_begin, int atom_end) {
     if (build_idle(S)) return;
     const int a = atom_begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -511,7 +522,7 @@ __global__ void k_prepare(DevState S, int atom_begin, int atom_end, int slot_beg
     if (a >= atom_end) return;
     blk_clear(S, a);
     acc_store<R>(S.acc_g, a, 0.0, 1.0);
-    if constexpr (Prec<R>::fixed) store_xyzq32(S, a);
+    if constexpr (Prec<R>::fixed) store_xyz32(S, a);
 }
 
 // Per-lane view of a tile's atoms.
@@ -1807,7 +1818,6 @@ __global__ void k_finalize(DevState S, int atom_begin, int atom_end, double *f2,
         const double fy = -acc_read<R>(S.acc_fy, a, kForceScale);
         const double fz = -acc_read<R>(S.acc_fz, a, kForceScale);
         S.fx[a] = fx; S.fy[a] = fy; S.fz[a] = fz;
-        S.deda_out[a] = acc_read<R>(S.acc_deda, a, kForceScale);
         if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicOr(S.inst_status + inst, kStatNonFinite);
         sq = fx * fx + fy * fy + fz * fz;
     }

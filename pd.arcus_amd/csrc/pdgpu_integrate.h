@@ -242,7 +242,6 @@ __global__ void k_finalize_kick(DevState S, InstMD *md, const double *minv, doub
         const double fy = -acc_read<R>(S.acc_fy, a, kForceScale);
         const double fz = -acc_read<R>(S.acc_fz, a, kForceScale);
         S.fx[a] = fx; S.fy[a] = fy; S.fz[a] = fz;
-        S.deda_out[a] = acc_read<R>(S.acc_deda, a, kForceScale);
         if (!(isfinite(fx) && isfinite(fy) && isfinite(fz))) atomicOr(S.inst_status + inst, kStatNonFinite);
         const double h = 0.5 * dt * minv[a] * f_to_n;
         double ux = vx[a] + fx * h, uy = vy[a] + fy * h, uz = vz[a] + fz * h;
@@ -257,7 +256,7 @@ __global__ void k_finalize_kick(DevState S, InstMD *md, const double *minv, doub
     if (next && in) {
         blk_clear(S, a);
         acc_store<R>(S.acc_g, a, 0.0, 1.0);
-        if constexpr (Prec<R>::fixed) store_xyzq32(S, a);
+        if constexpr (Prec<R>::fixed) store_xyz32(S, a);
         if (lane_id() == 0 && inst >= 0 && S.blk_first[b] == b) inst_clear(S, inst);
     }
     if (want_ekin) {
