@@ -967,13 +967,16 @@ __device__ __forceinline__ int diag_bo(const DiagBits &D, int jlane) {
 // Still's pairwise Born term P4*CCF/r^4 (forcefields/gbff.cpp:448-460)
 template <typename R>
 __device__ __forceinline__ R born_term(R r2, R sR, R invP5, R P4, R P5) {
-    const R x = r2 * Prec<R>::rcp(sR * sR);
+    // one reciprocal serves both quotients: t = 1/(sR^2 r^4)
+    const R s2 = sR * sR, r4 = r2 * r2;
+    const R t = Prec<R>::rcp(s2 * r4);
+    const R x = (r2 * r4) * t; // r^2 / sR^2
     R ccf = (R)1;
     if (x <= invP5) {
         const R h = (R)0.5 - (R)0.5 * Prec<R>::cospi_(x * P5);
         ccf = h * h;
     }
-    return P4 * ccf * Prec<R>::rcp(r2 * r2);
+    return P4 * ccf * (s2 * t);
 }
 
 template <typename R> struct BornAtom { R x, y, z, rad, vj; };
@@ -1480,7 +1483,10 @@ template <typename R> struct ChainAtom { R x, y, z, rad, vj, wt; };
 // P4 (4 CCF - dCCF) / r^6 (forcefields/gbff.cpp:1770-1795)
 template <typename R>
 __device__ __forceinline__ R chain_coef(R r2, R sR, R invP5, R P4, R P5) {
-    const R x = r2 * Prec<R>::rcp(sR * sR);
+    // one reciprocal serves both quotients: t = 1/(sR^2 r^6)
+    const R s2 = sR * sR, r6 = r2 * r2 * r2;
+    const R t = Prec<R>::rcp(s2 * r6);
+    const R x = (r2 * r6) * t; // r^2 / sR^2
     R ccf = (R)1, dccf = (R)0;
     if (x <= invP5) {
         const R f = x * P5;
@@ -1488,8 +1494,7 @@ __device__ __forceinline__ R chain_coef(R r2, R sR, R invP5, R P4, R P5) {
         ccf = h * h;
         dccf = (R)2 * h * Prec<R>::sinpi_(f) * (f * (R)3.14159265358979323846);
     }
-    const R r6 = r2 * r2 * r2;
-    return P4 * ((R)4 * ccf - dccf) * Prec<R>::rcp(r6);
+    return P4 * ((R)4 * ccf - dccf) * (s2 * t);
 }
 
 template <typename R> struct ChainLds {
