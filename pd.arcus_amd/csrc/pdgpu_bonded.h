@@ -136,7 +136,9 @@ __device__ double dihedral_term(const DevState &S, const GradSink<R> &G, const i
     const D3 b1 = d3_sub(ra, ri), b2 = d3_sub(rb, ra), b3 = d3_sub(rj, rb);
     const D3 m = d3_cross(b1, b2), n = d3_cross(b2, b3);
     const double m2 = d3_dot(m, m), n2 = d3_dot(n, n), b22 = d3_dot(b2, b2);
-    const double lb2 = sqrt(b22);
+    // reciprocals and square roots as in the FP64 pair terms (hardware
+    // estimate + two Newton steps)
+    const double ib2 = Prec<double>::rsqrt(b22), lb2 = b22 * ib2;
     const double phi = atan2(lb2 * d3_dot(b1, n), d3_dot(m, n));
     const double V[4] = {vn.x, vn.y, vn.z, vn.w}, N[4] = {nn.x, nn.y, nn.z, nn.w}, Ph[4] = {gm.x, gm.y, gm.z, gm.w};
     double e = 0, dedphi = 0;
@@ -155,9 +157,10 @@ __device__ double dihedral_term(const DevState &S, const GradSink<R> &G, const i
             dedphi += 2.0 * V[t] * d;
         }
     }
-    const D3 gi = d3_lin(-dedphi * lb2 / m2, m, 0.0, m);
-    const D3 gj = d3_lin(dedphi * lb2 / n2, n, 0.0, n);
-    const double p1 = d3_dot(b1, b2) / b22, p3 = d3_dot(b3, b2) / b22;
+    const D3 gi = d3_lin(-dedphi * lb2 * Prec<double>::rcp(m2), m, 0.0, m);
+    const D3 gj = d3_lin(dedphi * lb2 * Prec<double>::rcp(n2), n, 0.0, n);
+    const double ib22 = ib2 * ib2;
+    const double p1 = d3_dot(b1, b2) * ib22, p3 = d3_dot(b3, b2) * ib22;
     G.add(q.x, gi, st);
     G.add(q.y, d3_lin(-(1.0 + p1), gi, p3, gj), st);
     G.add(q.z, d3_lin(-(1.0 + p3), gj, p1, gi), st);
@@ -215,12 +218,12 @@ template <typename R> __device__ __forceinline__ void bonded_group(const DevStat
         int *st = S.inst_status + inst;
         const double2 p = B.bd_par[r];
         const D3 v = d3_sub(d3_at(S, q.x), d3_at(S, q.y));
-        const double d = sqrt(d3_dot(v, v));
+        const double d2 = d3_dot(v, v), id = Prec<double>::rsqrt(d2), d = d2 * id;
         const double dd = d - p.x;
         e = p.y * dd * dd;
         const double dedd = 2.0 * p.y * dd;
         vir = dedd * d; // InternalVirial += d * force (ffbonded.cpp:924)
-        const D3 g = d3_lin(dedd / d, v, 0.0, v);
+        const D3 g = d3_lin(dedd * id, v, 0.0, v);
         G.add(q.x, g, st);
         G.add(q.y, d3_lin(-1.0, g, 0.0, g), st);
     } else if (kind == 1) {
@@ -230,7 +233,7 @@ template <typename R> __device__ __forceinline__ void bonded_group(const DevStat
         const double2 p = B.an_par[r];
         const D3 ra = d3_at(S, q.y);
         const D3 vi = d3_sub(d3_at(S, q.x), ra), vj = d3_sub(d3_at(S, q.z), ra);
-        const double li = 1.0 / sqrt(d3_dot(vi, vi)), lj = 1.0 / sqrt(d3_dot(vj, vj));
+        const double li = Prec<double>::rsqrt(d3_dot(vi, vi)), lj = Prec<double>::rsqrt(d3_dot(vj, vj));
         const D3 ui = d3_lin(li, vi, 0.0, vi), uj = d3_lin(lj, vj, 0.0, vj);
         const double c = d3_dot(ui, uj);
         const double th = acos(c), sn = sin(th);
@@ -238,8 +241,9 @@ template <typename R> __device__ __forceinline__ void bonded_group(const DevStat
         e = p.y * dth * dth;
         const double dedth = 2.0 * p.y * dth;
         // dtheta/dx_i = (u_i cos - u_j) / (|v_i| sin)
-        const D3 gi = d3_lin(dedth * li / sn * c, ui, -dedth * li / sn, uj);
-        const D3 gj = d3_lin(dedth * lj / sn * c, uj, -dedth * lj / sn, ui);
+        const double isn = Prec<double>::rcp(sn);
+        const D3 gi = d3_lin(dedth * li * isn * c, ui, -dedth * li * isn, uj);
+        const D3 gj = d3_lin(dedth * lj * isn * c, uj, -dedth * lj * isn, ui);
         G.add(q.x, gi, st);
         G.add(q.z, gj, st);
         G.add(q.y, d3_lin(-1.0, gi, -1.0, gj), st);
